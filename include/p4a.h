@@ -1,0 +1,313 @@
+/*
+ * p4a.h - C ABI of libp4a.so, an MI355X (gfx950) implementation of the
+ * Particles4All simulation step: Position Based Fluids with rigid bodies in
+ * one constraint loop, XSPH viscosity, Akinci surface tension and the
+ * anisotropic-kernel covariance pass.
+ *
+ * Every entry point replaces one member of the reference's `class Sim`
+ * (src/sim.js:9-814) or one host routine it calls; each declaration cites the
+ * reference code it stands in for.  Conventions:
+ *   - every function returns int status, 0 (P4A_OK) on success;
+ *   - p4a_last_error(h) returns a message owned by the library (valid until
+ *     the next call on that handle);
+ *   - a handle is used from one host thread at a time; GPU work is queued on
+ *     the handle's own HIP stream and p4a_sync() waits for it;
+ *   - all pointers are plain host pointers unless the name says `device`;
+ *   - inputs the reference clamps or ignores are clamped or ignored here too
+ *     (appendFluid past capacity, holdBody with an unknown body, a ray
+ *     impulse with radius <= 0, a resize to the same box).
+ */
+#ifndef P4A_H
+#define P4A_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define P4A_API __attribute__((visibility("default")))
+
+typedef struct p4a_sim p4a_sim;          /* one scene on one GPU */
+typedef struct p4a_scene_data p4a_scene_data; /* host-side scene arrays */
+
+enum p4a_status {
+  P4A_OK = 0,
+  P4A_ERR_ARG = 1,     /* null pointer, negative size, unknown enum */
+  P4A_ERR_HIP = 2,     /* a HIP runtime call failed (message says which) */
+  P4A_ERR_STATE = 3,   /* call needs reset() first, or similar */
+  P4A_ERR_NOMEM = 4,   /* device or host allocation failed */
+  P4A_ERR_RANGE = 5,   /* offset/size outside the buffer */
+  P4A_ERR_NODEV = 6    /* no HIP device with that ordinal */
+};
+
+/* Arithmetic form of the neighbour kernels. */
+enum p4a_precision {
+  P4A_PREC_FP32 = 0,         /* float positions, float sums (reference form) */
+  P4A_PREC_FP16_OFFSET = 1   /* neighbour positions as fp16 offsets from the
+                                cell origin, fp32 accumulation */
+};
+
+/* Rounding of sqrt and division inside the neighbour kernels. */
+enum p4a_math {
+  P4A_MATH_FAST = 0,   /* hardware rsq/rcp (1 ulp): production form */
+  P4A_MATH_EXACT = 1   /* IEEE-rounded sqrt and division, no contraction: the
+                          results equal an op-by-op float evaluation */
+};
+
+/*
+ * Simulation parameters: defaultParams (src/scene.js:11-39) plus the URL
+ * flags that reach Sim (src/main.js:361-390).  FP64 host values; the
+ * library derives the float uniforms exactly as Sim.uploadParams does
+ * (src/sim.js:371-413).
+ */
+typedef struct p4a_params {
+  double spacing;            /* particle spacing d; h = 2d (scene.js:160-161) */
+  double rest_density;       /* rho0 */
+  double gravity;            /* g, acting along -y */
+  int32_t substeps;          /* per 1/60 s frame (sim.js:418-419) */
+  int32_t iterations;        /* PBF solver iterations I */
+  double cfm_epsilon_rel;    /* CFM relaxation relative to denomRest */
+  double scorr_k;            /* tensile-instability correction k */
+  double scorr_dq;           /* dq as a fraction of h */
+  double xsph_c;             /* XSPH viscosity c */
+  double omega;              /* over-relaxation of delta-p */
+  int32_t sor_average;       /* 1: divide delta-p by neighbour count */
+  double surface_tension_k;  /* Akinci gamma; 0 disables normals+tension */
+  double box[3];             /* box size; boxMin is the origin */
+  int32_t no_boundary;       /* 1: ignore the wall samples (boundary=0) */
+  int32_t pour_budget;       /* extra capacity for appendFluid; -1 = nFluid
+                                (the reference's default, sim.js:94) */
+  int32_t precision;         /* enum p4a_precision */
+  int32_t math;              /* enum p4a_math */
+  int32_t use_graph;         /* 1: replay each substep as a captured hipGraph */
+} p4a_params;
+
+/* Fill `p` with defaultParams('small') (src/scene.js:14-31). */
+P4A_API int p4a_params_default(p4a_params* p);
+
+/* ---------------------------------------------------------------------
+ * Scene construction (host side).  buildScene (src/scene.js:159-315)
+ * and its helpers fluidBlock (43-64), sampleBody (137-157) and
+ * boundaryParticles (76-135), plus lattice scenes for synthetic workloads.
+ * ------------------------------------------------------------------- */
+
+enum p4a_body_shape {
+  P4A_SHAPE_SPHERE = 0,     /* sampleBody 'sphere' (scene.js:148) */
+  P4A_SHAPE_TORUS = 1,      /* sampleBody 'torus' (scene.js:149-151) */
+  P4A_SHAPE_BOX = 2,        /* sampleBody box: half extents .9s/.7s/.9s */
+  P4A_SHAPE_LATTICE_CUBE = 3,   /* `size` particles per edge */
+  P4A_SHAPE_LATTICE_SPHERE = 4  /* lattice sites with |o| <= size spacings */
+};
+
+typedef struct p4a_body_spec {
+  int32_t shape;         /* enum p4a_body_shape */
+  double size;           /* lattice scenes: edge (cube) or radius (sphere) in
+                            spacings; presets use body_size for every body */
+  double density;        /* body density relative to the fluid; presets map
+                            0 or NaN to 0.5 (scene.js:169) */
+  double start_y;        /* presets: centre height / box y; 0 or NaN = 0.75 */
+  double centre[3];      /* lattice scenes only; presets place bodies */
+} p4a_body_spec;
+
+enum p4a_scene_kind {
+  P4A_SCENE_PRESET = 0,  /* buildScene(p) exactly as the reference */
+  P4A_SCENE_LATTICE = 1  /* jittered lattice block + explicit bodies */
+};
+
+typedef struct p4a_scene_spec {
+  int32_t kind;                  /* enum p4a_scene_kind */
+  /* preset scenes (scene.js:11-39, 159-315) */
+  int32_t target_particle_count;
+  double body_size;              /* 0: 0.12 * min(box) (scene.js:172) */
+  /* lattice scenes */
+  int32_t lattice[3];            /* sites per axis */
+  double origin[3];              /* block min corner; site = origin+(i+.5)d */
+  double jitter;                 /* uniform +-jitter*d per coordinate */
+  uint32_t seed;                 /* LCG seed (scene.js:52-56 idiom) */
+  int32_t calibrate_unjittered;  /* 1: m and denomRest from the unjittered
+                                    lattice; 0: as buildScene (jittered max) */
+  /* bodies (both kinds) */
+  int32_t n_bodies;
+  const p4a_body_spec* bodies;
+} p4a_scene_spec;
+
+/* A view of built scene arrays, laid out as buildScene's return value
+ * (src/scene.js:313-314).  Pointers stay owned by the p4a_scene_data. */
+typedef struct p4a_scene {
+  int32_t n, n_fluid, n_body;
+  const float* pos;        /* n*4, w = 0 */
+  const float* vel;        /* n*4 */
+  const float* rest;       /* n*4, body-local offset from the body centroid */
+  const uint32_t* phase;   /* n*4, x = 1-based body id, 0 = fluid */
+  int32_t n_boundary;
+  const float* boundary_pts;  /* nb*3 */
+  const float* boundary_psi;  /* nb */
+  int32_t n_bodies;
+  const double* body_centre;  /* nb*3 */
+  const double* body_density; /* per body, relative to the fluid */
+  const int32_t* body_count;  /* particles per body */
+  const int32_t* body_spec;   /* index into the spec's body list (bodies
+                                 that sample no particle are dropped,
+                                 scene.js:183) */
+  double mass, h, d, denom_rest;
+} p4a_scene;
+
+/* buildScene (scene.js:159-315) for P4A_SCENE_PRESET (params->box,
+ * spacing, rest_density and spec->target_particle_count, body list),
+ * or a jittered lattice scene for P4A_SCENE_LATTICE. */
+P4A_API int p4a_scene_build(const p4a_params* p, const p4a_scene_spec* spec,
+                            p4a_scene_data** out);
+P4A_API int p4a_scene_view(const p4a_scene_data* s, p4a_scene* view);
+P4A_API void p4a_scene_free(p4a_scene_data* s);
+
+/* boundaryParticles (scene.js:76-135): wall samples of the box shell and
+ * their Akinci volumes psi.  pts (nb*3) and psi (nb) are written when
+ * non-null; *count always.  Call once with nulls to size the arrays. */
+P4A_API int p4a_boundary_build(const double box[3], double d, double h,
+                               double rest_density, float* pts, float* psi,
+                               int32_t capacity, int32_t* count);
+
+/* ---------------------------------------------------------------------
+ * The Sim surface (src/sim.js)
+ * ------------------------------------------------------------------- */
+
+/* new Sim(device) (sim.js:10-18): bind to HIP device `device`. */
+P4A_API int p4a_create(int device, p4a_sim** out);
+/* Frees every device buffer of the handle. */
+P4A_API int p4a_destroy(p4a_sim* h);
+P4A_API const char* p4a_last_error(const p4a_sim* h);
+
+/* Sim.reset(params) (sim.js:66-221) with the scene already built (the
+ * reference calls buildScene inside reset; callers here pass its result).
+ * Allocates for cap = n + pour budget, sorts the wall samples
+ * (sortBoundary, sim.js:255-286), uploads the state and primes the grid
+ * (primeGrid, sim.js:223-253). */
+P4A_API int p4a_reset(p4a_sim* h, const p4a_params* p, const p4a_scene* scene);
+
+/* Re-read the live parameters that uploadParams takes each step
+ * (substeps, iterations, omega, xsph, scorr, cfm, tension, gravity;
+ * sim.js:371-413).  Geometry fields (box, spacing) are ignored here. */
+P4A_API int p4a_set_params(p4a_sim* h, const p4a_params* p);
+
+/* Sim.step(frameDt) (sim.js:415-656): time bank, then `sub` substeps of
+ * predict, grid build, bodies, I solver iterations and the finish kernels,
+ * then the stats reduction.  *substeps_run may be null. */
+P4A_API int p4a_step(p4a_sim* h, double frame_dt, int32_t* substeps_run);
+
+/* Run exactly `count` substeps of length dt (no time bank; for benches). */
+P4A_API int p4a_substeps(p4a_sim* h, double dt, int32_t count);
+
+/* Sim.resizeBox(newBox) (sim.js:687-772). */
+P4A_API int p4a_resize_box(p4a_sim* h, const double box[3]);
+
+/* Sim.appendFluid(pos, vel) (sim.js:774-809): pos3/vel3 are count*3
+ * floats; *appended receives the number actually added (0 when full). */
+P4A_API int p4a_append_fluid(p4a_sim* h, const float* pos3, const float* vel3,
+                             int32_t count, int32_t* appended);
+
+/* Sim.applyRayImpulse(origin, dir, impulse, radius, speedLimit)
+ * (sim.js:658-676). */
+P4A_API int p4a_apply_ray_impulse(p4a_sim* h, const double origin[3],
+                                  const double dir[3], const double impulse[3],
+                                  double radius, double speed_limit);
+
+/* Sim.holdBody / Sim.releaseBody (sim.js:678-685). */
+P4A_API int p4a_hold_body(p4a_sim* h, int32_t body_1based, const double target[3],
+                          double rate, double limit);
+P4A_API int p4a_release_body(p4a_sim* h);
+
+/* Buffers readable through p4a_read / p4a_device_ptr.  Particle arrays are
+ * in the live (cell-sorted) order of the last substep; P4A_BUF_PID maps a
+ * live slot to the particle's id (its index in the scene arrays or the
+ * append order), which the reference has no counterpart of. */
+enum p4a_buf {
+  P4A_BUF_POS = 0,         /* float4 n   (sim.buf.posA/B, livePos) */
+  P4A_BUF_VEL = 1,         /* float4 n   (liveVel) */
+  P4A_BUF_PRED = 2,        /* float4 n   (pred of the live parity) */
+  P4A_BUF_BODY = 3,        /* uint4 n, x = body id (liveBody) */
+  P4A_BUF_REST = 4,        /* float4 n   (restA/B) */
+  P4A_BUF_LAMBDA = 5,      /* float n */
+  P4A_BUF_DENSITY = 6,     /* float n */
+  P4A_BUF_SLOT = 7,        /* uint n: slot of the previous order's i (scatterSlot) */
+  P4A_BUF_CORR = 8,        /* float4 n */
+  P4A_BUF_NORMAL = 9,      /* float4 n */
+  P4A_BUF_CELL_START = 10, /* uint nCells+2 */
+  P4A_BUF_BPOS = 11,       /* float4 nb, sorted wall samples, w = psi */
+  P4A_BUF_BCELL_START = 12,/* uint nCells+2 */
+  P4A_BUF_BODY_CENTRE = 13,/* float4 nBodies (bodyCentre) */
+  P4A_BUF_BODY_ROT = 14,   /* float4 3*nBodies, rows of R (bodyRot) */
+  P4A_BUF_PID = 15,        /* uint n */
+  P4A_BUF_KEY = 16,        /* uint n: cell id of each particle, pre-sort order */
+  P4A_BUF_ORDER = 17,      /* uint n: pre-sort index that landed in each slot */
+  P4A_BUF_BPSI = 18,       /* float nb */
+  P4A_BUF_COUNT_
+};
+
+/* Copy `bytes` bytes at byte `offset` of buffer `id` to host `dst`
+ * (synchronous; the reference's __readBuf, main.js:810-820). */
+P4A_API int p4a_read(p4a_sim* h, int32_t id, void* dst, size_t offset, size_t bytes);
+/* Overwrite part of a state buffer from the host (test and restore use). */
+P4A_API int p4a_write(p4a_sim* h, int32_t id, const void* src, size_t offset, size_t bytes);
+/* Device address and size of a buffer for zero-copy interop. */
+P4A_API int p4a_device_ptr(p4a_sim* h, int32_t id, void** ptr, size_t* bytes);
+
+/* Scalars the reference exposes as Sim fields (sim.js:71-221, 415-429). */
+typedef struct p4a_info {
+  int32_t n, cap, n_fluid, n_body_parts, n_bodies, n_boundary, n_cells;
+  int32_t grid_dim[3];
+  int32_t parity, gen, last_substeps, pour_budget;
+  double h, mass, denom_rest, sim_time, last_advanced, time_bank;
+  double body_scale, cov_scale;
+} p4a_info;
+P4A_API int p4a_get_info(p4a_sim* h, p4a_info* out);
+
+/* Sim.stats after the last step (sim.js:619-633): avgRho, maxRho (both
+ * relative to rho0), maxSpeed, kinetic energy; reduced exactly in FP64. */
+P4A_API int p4a_stats(p4a_sim* h, double out[4]);
+
+/* Sim.bodyPose (sim.js:635-655): centre nBodies*3, rot nBodies*9
+ * (column-major, as the reference's p.rot). */
+P4A_API int p4a_body_pose(p4a_sim* h, float* centre, float* rot);
+
+/* Wait for all queued GPU work of the handle. */
+P4A_API int p4a_sync(p4a_sim* h);
+
+/* ---------------------------------------------------------------------
+ * Stage-level entry points: run one kernel group of a substep on the
+ * current state, so each can be checked against the oracle on identical
+ * inputs.  The stages follow the reference's pass order (sim.js:458-596).
+ * ------------------------------------------------------------------- */
+enum p4a_stage {
+  P4A_STAGE_PREDICT = 0,    /* bodyDrag (if held) + predict (wgsl.js:56-70) */
+  P4A_STAGE_GRID = 1,       /* count, scan, stable scatter (wgsl.js:352-475) */
+  P4A_STAGE_BODY_PRE = 2,   /* body centroids (wgsl.js:628-668, 174-185) */
+  P4A_STAGE_LAMBDA = 3,     /* wgsl.js:477-543 */
+  P4A_STAGE_DELTA = 4,      /* wgsl.js:545-612 */
+  P4A_STAGE_BODY_SOLVE = 5, /* shape matching (wgsl.js:628-783) */
+  P4A_STAGE_FINISH = 6,     /* velFromPos, xsph, normals, tension, finalize */
+  P4A_STAGE_STATS = 7       /* wgsl.js:785-832 */
+};
+P4A_API int p4a_run_stage(p4a_sim* h, int32_t stage, double dt);
+
+/* Per particle (live order): number of fluid neighbours with r^2 < h^2 in
+ * the 27-cell stencil of the current grid, and two checksums of their
+ * indices (sum of j, sum of j*j mod 2^32).  Integer outputs, n each. */
+P4A_API int p4a_neighbour_census(p4a_sim* h, uint32_t* count, uint32_t* sum_j,
+                                 uint32_t* sum_j2);
+
+/* Per-kernel timing of the last p4a_substeps call: names and mean ms per
+ * launch, measured with HIP events on the handle's stream.  `names` gets
+ * a newline-separated list.  Enabled by p4a_set_timing(h, 1). */
+P4A_API int p4a_set_timing(p4a_sim* h, int32_t on);
+P4A_API int p4a_timing(p4a_sim* h, char* names, size_t names_bytes, double* ms,
+                       int32_t* launches, int32_t capacity, int32_t* count);
+
+/* Library version string. */
+P4A_API const char* p4a_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P4A_H */

@@ -1,0 +1,568 @@
+// ORACLE - test infrastructure only.  Nothing in the product links,
+// imports or calls this file; tests/, __graft_entry__.smoke() and the CPU
+// baseline leg of bench.py use it as the checker.
+//
+// A CPU restatement of the Particles4All simulation step, following the
+// reference's WGSL kernels (src/wgsl.js) and Sim orchestration (src/sim.js)
+// operation by operation.  Every function runs in one of two modes:
+//   mode 0 (R = float):  one IEEE float operation per WGSL operation, in the
+//                        WGSL's order (no contraction; built with
+//                        -ffp-contract=off), constants rounded to float as
+//                        the reference's uniform buffer stores them;
+//   mode 1 (R = double): the same sequence in FP64 with FP64 constants -
+//                        the FP64 evaluation the GPU results are measured
+//                        against.
+// r^2 is fma(z,z, fma(y,y, x*x)) in both modes (the WGSL `dot` leaves the
+// contraction to the WebGPU compiler; this fixes one choice).
+// Rigid-body sums and the polar decomposition are FP64 in both modes; the
+// reference quantises those sums to i32 fixed point (wgsl.js:628-698), and
+// the difference is bounded by 0.5/bodyScale per summed term.
+//
+// Array I/O is double throughout (float values are exact in double).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+
+// Constants of the reference's Params uniform (wgsl.js:4-40), computed by
+// the caller from the scene and parameters (the oracle's restatement of
+// Sim.uploadParams, sim.js:371-413, lives in oracle.py).
+struct OrcParams {
+  double dt, h, h2, poly6, spiky, gravity, mass, rho0, invRho0;
+  double cfmEps, sCorrK, sCorrWq, xsphC, omega, invDt, volume, tension, cohes, cohesTerm;
+  double boxMin[3], clampMin[3], clampMax[3];
+  int32_t gridDim[3];
+  int32_t sorAverage;
+  int32_t nBoundary;
+  int32_t pad;
+};
+}
+
+namespace {
+
+template <class R>
+struct K {  // constants in mode R
+  R dt, h, h2, poly6, spiky, gravity, mass, rho0, invRho0, cfmEps, sCorrK, sCorrWq, xsphC,
+      omega, invDt, volume, tension, cohes, cohesTerm;
+  R boxMin[3], clampMin[3], clampMax[3];
+  int g[3];
+  int sor;
+  int nB;
+  explicit K(const OrcParams& p) {
+    dt = (R)p.dt; h = (R)p.h; h2 = (R)p.h2; poly6 = (R)p.poly6; spiky = (R)p.spiky;
+    gravity = (R)p.gravity; mass = (R)p.mass; rho0 = (R)p.rho0; invRho0 = (R)p.invRho0;
+    cfmEps = (R)p.cfmEps; sCorrK = (R)p.sCorrK; sCorrWq = (R)p.sCorrWq; xsphC = (R)p.xsphC;
+    omega = (R)p.omega; invDt = (R)p.invDt; volume = (R)p.volume; tension = (R)p.tension;
+    cohes = (R)p.cohes; cohesTerm = (R)p.cohesTerm;
+    for (int k = 0; k < 3; k++) {
+      boxMin[k] = (R)p.boxMin[k]; clampMin[k] = (R)p.clampMin[k]; clampMax[k] = (R)p.clampMax[k];
+      g[k] = p.gridDim[k];
+    }
+    sor = p.sorAverage;
+    nB = p.nBoundary;
+  }
+};
+
+inline float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
+inline double fma_r(double a, double b, double c) { return fma(a, b, c); }
+inline float sqrt_r(float a) { return sqrtf(a); }
+inline double sqrt_r(double a) { return sqrt(a); }
+
+template <class R>
+inline R r2_of(R x, R y, R z) { return fma_r(z, z, fma_r(y, y, x * x)); }
+
+// cellOf (wgsl.js:43-46): floor((p - boxMin)/h) clamped to the grid.
+template <class R>
+inline void cell_of(const K<R>& k, R x, R y, R z, int c[3]) {
+  const R p[3] = {x, y, z};
+  for (int a = 0; a < 3; a++) {
+    R f = std::floor((p[a] - k.boxMin[a]) / k.h);
+    f = std::fmin(std::fmax(f, (R)0), (R)(k.g[a] - 1));  // NaN -> 0
+    c[a] = (int)f;
+  }
+}
+template <class R>
+inline uint32_t cell_index(const K<R>& k, const int c[3]) {
+  return (uint32_t)((c[2] * k.g[1] + c[1]) * k.g[0] + c[0]);
+}
+
+// poly6 (wgsl.js:50-53) for r2 < h2
+template <class R>
+inline R poly6(const K<R>& k, R r2) {
+  R t = k.h2 - r2;
+  return k.poly6 * t * t * t;
+}
+
+// The reference's 27-cell stencil walk (e.g. wgsl.js:497-534): rows dz, dy;
+// per row the fluid span [cellStart[row+x0], cellStart[row+x1+1]) and then
+// the boundary span of the same cells.
+template <class R, class F, class B>
+inline void stencil(const K<R>& k, R px, R py, R pz, const uint32_t* cs, const uint32_t* bcs,
+                    F&& fluid, B&& bnd) {
+  int c[3];
+  cell_of(k, px, py, pz, c);
+  const int x0 = std::max(c[0] - 1, 0), x1 = std::min(c[0] + 1, k.g[0] - 1);
+  for (int dz = -1; dz <= 1; dz++) {
+    const int z = c[2] + dz;
+    if (z < 0 || z >= k.g[2]) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int y = c[1] + dy;
+      if (y < 0 || y >= k.g[1]) continue;
+      const uint32_t rb = (uint32_t)((z * k.g[1] + y) * k.g[0]);
+      for (uint32_t j = cs[rb + x0]; j < cs[rb + x1 + 1]; j++) fluid(j);
+      if (bcs && k.nB > 0)
+        for (uint32_t q = bcs[rb + x0]; q < bcs[rb + x1 + 1]; q++) bnd(q);
+    }
+  }
+}
+
+template <class R>
+void predict(const OrcParams& P, int n, const double* pos, const double* vel, double* pred,
+             double* velOut) {
+  K<R> k(P);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    // wgsl.js:66-68
+    const R gx = k.dt * (R)0, gy = k.dt * (-k.gravity), gz = k.dt * (R)0;
+    const R vx = (R)vel[i * 4] + gx, vy = (R)vel[i * 4 + 1] + gy, vz = (R)vel[i * 4 + 2] + gz;
+    pred[i * 4] = (R)pos[i * 4] + k.dt * vx;
+    pred[i * 4 + 1] = (R)pos[i * 4 + 1] + k.dt * vy;
+    pred[i * 4 + 2] = (R)pos[i * 4 + 2] + k.dt * vz;
+    pred[i * 4 + 3] = 1.0;
+    if (velOut) {
+      velOut[i * 4] = vx; velOut[i * 4 + 1] = vy; velOut[i * 4 + 2] = vz; velOut[i * 4 + 3] = 0;
+    }
+  }
+}
+
+template <class R>
+void lambda_k(const OrcParams& P, int n, const double* pred, const uint32_t* cs,
+              const double* bpos, const double* bpsi, const uint32_t* bcs, double* density,
+              double* lambda) {
+  K<R> k(P);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const R px = (R)pred[i * 4], py = (R)pred[i * 4 + 1], pz = (R)pred[i * 4 + 2];
+    R rho = k.mass * (k.poly6 * k.h2 * k.h2 * k.h2);  // mass * poly6(0)
+    R gx = 0, gy = 0, gz = 0, sg2 = 0;
+    stencil(k, px, py, pz, cs, bcs,
+      [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        const R rx = px - (R)pred[j * 4], ry = py - (R)pred[j * 4 + 1], rz = pz - (R)pred[j * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2) return;
+        rho = rho + k.mass * poly6(k, r2);
+        if (r2 > (R)1.0e-12) {
+          const R r = sqrt_r(r2);
+          const R hr = k.h - r;
+          const R s = k.volume * k.spiky * hr * hr / r;
+          const R ax = s * rx, ay = s * ry, az = s * rz;
+          gx = gx + ax; gy = gy + ay; gz = gz + az;
+          sg2 = sg2 + (ax * ax + ay * ay + az * az);
+        }
+      },
+      [&](uint32_t q) {
+        const R rx = px - (R)bpos[q * 4], ry = py - (R)bpos[q * 4 + 1], rz = pz - (R)bpos[q * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
+        const R psi = (R)bpsi[q];
+        rho = rho + psi * poly6(k, r2);
+        const R r = sqrt_r(r2);
+        const R hr = k.h - r;
+        const R s = psi * k.invRho0 * k.spiky * hr * hr / r;
+        gx = gx + s * rx; gy = gy + s * ry; gz = gz + s * rz;
+      });
+    density[i] = rho;
+    const R C = rho * k.invRho0 - (R)1;
+    R lam = 0;
+    if (C > (R)0) lam = -C / ((gx * gx + gy * gy + gz * gz) + sg2 + k.cfmEps + (R)1.0e-12);
+    lambda[i] = lam;
+  }
+}
+
+template <class R>
+void delta_k(const OrcParams& P, int n, const double* pred, const double* lambda,
+             const uint32_t* cs, const double* bpos, const double* bpsi, const uint32_t* bcs,
+             double* out) {
+  K<R> k(P);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const R px = (R)pred[i * 4], py = (R)pred[i * 4 + 1], pz = (R)pred[i * 4 + 2];
+    const R li = (R)lambda[i];
+    R dx = 0, dy = 0, dz = 0, bx = 0, by = 0, bz = 0, nc = 1;
+    stencil(k, px, py, pz, cs, bcs,
+      [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        const R rx = px - (R)pred[j * 4], ry = py - (R)pred[j * 4 + 1], rz = pz - (R)pred[j * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
+        const R r = sqrt_r(r2);
+        const R hr = k.h - r;
+        const R s = k.spiky * hr * hr / r;
+        const R t = k.h2 - r2;
+        const R ratio = k.poly6 * t * t * t * k.sCorrWq;
+        const R r2r = ratio * ratio;
+        const R sc = -k.sCorrK * r2r * r2r;
+        const R f = (li + (R)lambda[j] + sc) * s;
+        dx = dx + f * rx; dy = dy + f * ry; dz = dz + f * rz;
+        nc = nc + (R)1;
+      },
+      [&](uint32_t q) {
+        const R rx = px - (R)bpos[q * 4], ry = py - (R)bpos[q * 4 + 1], rz = pz - (R)bpos[q * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
+        const R r = sqrt_r(r2);
+        const R hr = k.h - r;
+        const R f = (li * (R)bpsi[q]) * (k.spiky * hr * hr / r);
+        bx = bx + f * rx; by = by + f * ry; bz = bz + f * rz;
+      });
+    const R ex = dx * k.volume + bx * k.invRho0;
+    const R ey = dy * k.volume + by * k.invRho0;
+    const R ez = dz * k.volume + bz * k.invRho0;
+    R scale = k.omega;
+    if (k.sor == 1) scale = k.omega / std::max(nc, (R)1);
+    R o[3] = {px + ex * scale, py + ey * scale, pz + ez * scale};
+    for (int a = 0; a < 3; a++) o[a] = std::min(std::max(o[a], k.clampMin[a]), k.clampMax[a]);
+    out[i * 4] = o[0]; out[i * 4 + 1] = o[1]; out[i * 4 + 2] = o[2]; out[i * 4 + 3] = 1.0;
+  }
+}
+
+// velFromPos (wgsl.js:72-83), xsph (85-123) and normals (250-288)
+template <class R>
+void xsph_k(const OrcParams& P, int n, const double* pred, const double* pos,
+            const double* density, const uint32_t* cs, int normals, double* vel, double* corr,
+            double* normal) {
+  K<R> k(P);
+  std::vector<R> v((size_t)n * 3);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++)
+      v[(size_t)i * 3 + a] = ((R)pred[i * 4 + a] - (R)pos[i * 4 + a]) * k.invDt;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const R px = (R)pred[i * 4], py = (R)pred[i * 4 + 1], pz = (R)pred[i * 4 + 2];
+    const R vx = v[(size_t)i * 3], vy = v[(size_t)i * 3 + 1], vz = v[(size_t)i * 3 + 2];
+    R ax = 0, ay = 0, az = 0, nx = 0, ny = 0, nz = 0;
+    stencil(k, px, py, pz, cs, (const uint32_t*)nullptr,
+      [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        const R rx = px - (R)pred[j * 4], ry = py - (R)pred[j * 4 + 1], rz = pz - (R)pred[j * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2) return;
+        const R den = std::max((R)density[j], (R)0.5 * k.rho0);
+        const R f = k.mass * poly6(k, r2) / den;
+        ax = ax + f * (v[(size_t)j * 3] - vx);
+        ay = ay + f * (v[(size_t)j * 3 + 1] - vy);
+        az = az + f * (v[(size_t)j * 3 + 2] - vz);
+        if (normals && r2 >= (R)1.0e-12) {
+          const R r = sqrt_r(r2);
+          const R hr = k.h - r;
+          const R g = (k.mass / den) * (k.spiky * hr * hr / r);
+          nx = nx + g * rx; ny = ny + g * ry; nz = nz + g * rz;
+        }
+      },
+      [&](uint32_t) {});
+    if (vel) {
+      vel[i * 4] = vx; vel[i * 4 + 1] = vy; vel[i * 4 + 2] = vz; vel[i * 4 + 3] = 0;
+    }
+    corr[i * 4] = k.xsphC * ax; corr[i * 4 + 1] = k.xsphC * ay; corr[i * 4 + 2] = k.xsphC * az;
+    corr[i * 4 + 3] = 0;
+    if (normals) {
+      normal[i * 4] = k.h * nx; normal[i * 4 + 1] = k.h * ny; normal[i * 4 + 2] = k.h * nz;
+      normal[i * 4 + 3] = 0;
+    }
+  }
+}
+
+// tension (wgsl.js:290-335); corr is updated in place
+template <class R>
+void tension_k(const OrcParams& P, int n, const double* pred, const double* density,
+               const double* normal, const uint32_t* cs, double* corr) {
+  K<R> k(P);
+  std::vector<R> add((size_t)n * 3);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const R px = (R)pred[i * 4], py = (R)pred[i * 4 + 1], pz = (R)pred[i * 4 + 2];
+    const R nix = (R)normal[i * 4], niy = (R)normal[i * 4 + 1], niz = (R)normal[i * 4 + 2];
+    const R rhoi = (R)density[i];
+    R ax = 0, ay = 0, az = 0;
+    stencil(k, px, py, pz, cs, (const uint32_t*)nullptr,
+      [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        const R rx = px - (R)pred[j * 4], ry = py - (R)pred[j * 4 + 1], rz = pz - (R)pred[j * 4 + 2];
+        const R r2 = r2_of(rx, ry, rz);
+        if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
+        const R r = sqrt_r(r2);
+        const R hrr = k.h - r;
+        const R kk = hrr * hrr * hrr * r * r * r;
+        R cw = k.cohes * ((R)2 * kk - k.cohesTerm);
+        if ((R)2 * r > k.h) cw = k.cohes * kk;
+        const R fc = -k.tension * k.mass * cw / r;
+        const R fx = fc * rx + (-k.tension * (nix - (R)normal[j * 4]));
+        const R fy = fc * ry + (-k.tension * (niy - (R)normal[j * 4 + 1]));
+        const R fz = fc * rz + (-k.tension * (niz - (R)normal[j * 4 + 2]));
+        const R KK = (R)2 * k.rho0 / (rhoi + (R)density[j]);
+        ax = ax + KK * fx; ay = ay + KK * fy; az = az + KK * fz;
+      },
+      [&](uint32_t) {});
+    add[(size_t)i * 3] = ax; add[(size_t)i * 3 + 1] = ay; add[(size_t)i * 3 + 2] = az;
+  }
+  for (int i = 0; i < n; i++)
+    for (int a = 0; a < 3; a++) corr[i * 4 + a] = (R)corr[i * 4 + a] + k.dt * add[(size_t)i * 3 + a];
+}
+
+// finalize (wgsl.js:337-350): vel += corr; pos = pred
+template <class R>
+void finalize_k(int n, const double* pred, const double* corr, double* vel, double* pos) {
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) vel[i * 4 + a] = (R)vel[i * 4 + a] + (R)corr[i * 4 + a];
+    vel[i * 4 + 3] = 0;
+    for (int a = 0; a < 4; a++) pos[i * 4 + a] = pred[i * 4 + a];
+  }
+}
+
+// FP64 polar factor of A (row-major) by the reference's iteration
+// (wgsl.js:727-760).
+void polar(double A[3][3], double R[3][3]) {
+  double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+               A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+               A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  if (std::fabs(det) < 1.0e-12) {
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) A[r][c] = r == c ? 1.0 : 0.0;
+    det = 1.0;
+  }
+  const double s = std::cbrt(std::fabs(det));
+  if (s > 1.0e-6)
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) A[r][c] *= 1.0 / s;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) R[r][c] = A[r][c];
+  for (int it = 0; it < 8; it++) {
+    // inverse by cross products of the columns (mat3x3fInverse)
+    double a[3] = {R[0][0], R[1][0], R[2][0]}, b[3] = {R[0][1], R[1][1], R[2][1]},
+           c[3] = {R[0][2], R[1][2], R[2][2]};
+    auto cross = [](const double* u, const double* w, double* o) {
+      o[0] = u[1] * w[2] - u[2] * w[1];
+      o[1] = u[2] * w[0] - u[0] * w[2];
+      o[2] = u[0] * w[1] - u[1] * w[0];
+    };
+    double r0[3], r1[3], r2[3];
+    cross(b, c, r0); cross(c, a, r1); cross(a, b, r2);
+    const double d = a[0] * r0[0] + a[1] * r0[1] + a[2] * r0[2];
+    const double invd = std::fabs(d) > 1.0e-20 ? 1.0 / d : 0.0;
+    // inverse rows are r0, r1, r2; so inverse^T has columns r0, r1, r2
+    double IT[3][3];
+    for (int q = 0; q < 3; q++) {
+      IT[q][0] = r0[q] * invd; IT[q][1] = r1[q] * invd; IT[q][2] = r2[q] * invd;
+    }
+    double diff = 0;
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 3; cc++) {
+        const double nx = (R[r][cc] + IT[r][cc]) * 0.5;
+        diff += (nx - R[r][cc]) * (nx - R[r][cc]);
+        R[r][cc] = nx;
+      }
+    if (diff < 1.0e-12) break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version() { return 1; }
+
+// cell id of every particle (wgsl.js:43-49, as count/scatterSlot use it)
+void orc_cell_ids(int mode, const OrcParams* P, int n, const double* pred, uint32_t* key) {
+  for (int i = 0; i < n; i++) {
+    int c[3];
+    if (mode == 0) {
+      K<float> k(*P);
+      cell_of(k, (float)pred[i * 4], (float)pred[i * 4 + 1], (float)pred[i * 4 + 2], c);
+      key[i] = cell_index(k, c);
+    } else {
+      K<double> k(*P);
+      cell_of(k, pred[i * 4], pred[i * 4 + 1], pred[i * 4 + 2], c);
+      key[i] = cell_index(k, c);
+    }
+  }
+}
+
+// Stable counting sort by cell (count, scan, scatter; wgsl.js:352-438 with
+// the within-cell order fixed to the input order).  cellStart has
+// nCells+1 entries; slot[i] = sorted position of input i; order[s] = input
+// index at sorted position s.
+void orc_grid(int n, int nCells, const uint32_t* key, uint32_t* cellStart, uint32_t* slot,
+              uint32_t* order) {
+  std::vector<uint32_t> cnt((size_t)nCells + 1, 0);
+  for (int i = 0; i < n; i++) cnt[key[i]]++;
+  uint32_t run = 0;
+  for (int c = 0; c < nCells; c++) {
+    cellStart[c] = run;
+    run += cnt[c];
+  }
+  cellStart[nCells] = run;
+  std::vector<uint32_t> cur(cellStart, cellStart + nCells);
+  for (int i = 0; i < n; i++) {
+    const uint32_t s = cur[key[i]]++;
+    slot[i] = s;
+    order[s] = (uint32_t)i;
+  }
+}
+
+void orc_predict(int mode, const OrcParams* P, int n, const double* pos, const double* vel,
+                 double* pred, double* velOut) {
+  if (mode == 0) predict<float>(*P, n, pos, vel, pred, velOut);
+  else predict<double>(*P, n, pos, vel, pred, velOut);
+}
+
+void orc_lambda(int mode, const OrcParams* P, int n, const double* pred, const uint32_t* cs,
+                const double* bpos, const double* bpsi, const uint32_t* bcs, double* density,
+                double* lambda) {
+  if (mode == 0) lambda_k<float>(*P, n, pred, cs, bpos, bpsi, bcs, density, lambda);
+  else lambda_k<double>(*P, n, pred, cs, bpos, bpsi, bcs, density, lambda);
+}
+
+void orc_delta(int mode, const OrcParams* P, int n, const double* pred, const double* lambda,
+               const uint32_t* cs, const double* bpos, const double* bpsi, const uint32_t* bcs,
+               double* out) {
+  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out);
+  else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out);
+}
+
+void orc_xsph(int mode, const OrcParams* P, int n, const double* pred, const double* pos,
+              const double* density, const uint32_t* cs, int normals, double* vel, double* corr,
+              double* normal) {
+  if (mode == 0) xsph_k<float>(*P, n, pred, pos, density, cs, normals, vel, corr, normal);
+  else xsph_k<double>(*P, n, pred, pos, density, cs, normals, vel, corr, normal);
+}
+
+void orc_tension(int mode, const OrcParams* P, int n, const double* pred, const double* density,
+                 const double* normal, const uint32_t* cs, double* corr) {
+  if (mode == 0) tension_k<float>(*P, n, pred, density, normal, cs, corr);
+  else tension_k<double>(*P, n, pred, density, normal, cs, corr);
+}
+
+void orc_finalize(int mode, int n, const double* pred, const double* corr, double* vel,
+                  double* pos) {
+  if (mode == 0) finalize_k<float>(n, pred, corr, vel, pos);
+  else finalize_k<double>(n, pred, corr, vel, pos);
+}
+
+// Neighbour census: fluid j != i with r^2 < h^2 in the 27-cell stencil,
+// count and index sums mod 2^32 (mode 0 arithmetic).
+void orc_census(const OrcParams* P, int n, const double* pred, const uint32_t* cs,
+                uint32_t* cnt, uint32_t* s1, uint32_t* s2) {
+  K<float> k(*P);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const float px = (float)pred[i * 4], py = (float)pred[i * 4 + 1], pz = (float)pred[i * 4 + 2];
+    uint32_t c = 0, a = 0, b = 0;
+    stencil(k, px, py, pz, cs, (const uint32_t*)nullptr,
+      [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        const float rx = px - (float)pred[j * 4], ry = py - (float)pred[j * 4 + 1],
+                    rz = pz - (float)pred[j * 4 + 2];
+        if (r2_of(rx, ry, rz) >= k.h2) return;
+        c += 1u; a += j; b += j * j;
+      },
+      [&](uint32_t) {});
+    cnt[i] = c; s1[i] = a; s2[i] = b;
+  }
+}
+
+// Body centroids before the solve (wgsl.js:628-668, 174-185): centre_b =
+// mean of pred over the body's particles.  bodyOf[i] is 1-based, 0 =
+// fluid.
+void orc_body_pre(int n, const double* pred, const uint32_t* bodyOf, int nBodies,
+                  double* centre) {
+  std::vector<double> s((size_t)nBodies * 3, 0.0), c(nBodies, 0.0);
+  for (int i = 0; i < n; i++) {
+    const uint32_t b = bodyOf[i];
+    if (!b) continue;
+    for (int a = 0; a < 3; a++) s[(b - 1) * 3 + a] += pred[i * 4 + a] - centre[(b - 1) * 3 + a];
+    c[b - 1] += 1;
+  }
+  for (int b = 0; b < nBodies; b++)
+    if (c[b] > 0)
+      for (int a = 0; a < 3; a++) centre[b * 3 + a] += s[b * 3 + a] / c[b];
+}
+
+// One shape-matching round (wgsl.js:628-783) in FP64: bodyCentre pass,
+// bodySeed (centroid), bodyCov about the centroid, bodyResolve (centre +=
+// mean shift * invRatio; polar factor of A), bodyProject in float:
+// pred = centre + R q (rows of R and centre rounded to float).
+// centre (3/body) and rot (9/body, row-major) are updated in place.
+void orc_body_solve(int n, double* pred, const uint32_t* bodyOf, const double* rest,
+                    int nBodies, const double* invRatio, double* centre, double* rot) {
+  std::vector<double> s((size_t)nBodies * 3, 0.0), cnt(nBodies, 0.0);
+  for (int i = 0; i < n; i++) {
+    const uint32_t b = bodyOf[i];
+    if (!b) continue;
+    for (int a = 0; a < 3; a++) s[(b - 1) * 3 + a] += pred[i * 4 + a] - centre[(b - 1) * 3 + a];
+    cnt[b - 1] += 1;
+  }
+  std::vector<double> ref((size_t)nBodies * 3), A((size_t)nBodies * 9, 0.0);
+  for (int b = 0; b < nBodies; b++)
+    for (int a = 0; a < 3; a++)
+      ref[b * 3 + a] = cnt[b] > 0 ? centre[b * 3 + a] + s[b * 3 + a] / cnt[b] : centre[b * 3 + a];
+  for (int i = 0; i < n; i++) {
+    const uint32_t b = bodyOf[i];
+    if (!b) continue;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++)
+        A[(b - 1) * 9 + r * 3 + c] += (pred[i * 4 + r] - ref[(b - 1) * 3 + r]) * rest[i * 4 + c];
+  }
+  std::vector<float> cf((size_t)nBodies * 3), rf((size_t)nBodies * 9);
+  for (int b = 0; b < nBodies; b++) {
+    if (cnt[b] <= 0) continue;
+    for (int a = 0; a < 3; a++) {
+      double d = s[b * 3 + a] / cnt[b];
+      if (invRatio[b] > 0) d *= invRatio[b];
+      centre[b * 3 + a] += d;
+    }
+    double M[3][3], R[3][3];
+    for (int k = 0; k < 9; k++) M[k / 3][k % 3] = A[b * 9 + k];
+    polar(M, R);
+    for (int k = 0; k < 9; k++) rot[b * 9 + k] = R[k / 3][k % 3];
+    for (int a = 0; a < 3; a++) cf[b * 3 + a] = (float)centre[b * 3 + a];
+    for (int k = 0; k < 9; k++) rf[b * 9 + k] = (float)R[k / 3][k % 3];
+  }
+  for (int i = 0; i < n; i++) {
+    const uint32_t b = bodyOf[i];
+    if (!b || cnt[b - 1] <= 0) continue;
+    const float* c = &cf[(b - 1) * 3];
+    const float* R = &rf[(b - 1) * 9];
+    const float qx = (float)rest[i * 4], qy = (float)rest[i * 4 + 1], qz = (float)rest[i * 4 + 2];
+    const float px = R[0] * qx + R[1] * qy + R[2] * qz;
+    const float py = R[3] * qx + R[4] * qy + R[5] * qz;
+    const float pz = R[6] * qx + R[7] * qy + R[8] * qz;
+    pred[i * 4] = c[0] + px; pred[i * 4 + 1] = c[1] + py; pred[i * 4 + 2] = c[2] + pz;
+    pred[i * 4 + 3] = 1.0;
+  }
+}
+
+// stats (wgsl.js:785-832) reduced exactly: sum rho/rho0 / n, max rho/rho0,
+// max |v|, sum 0.5 m |v|^2, per-particle terms in float.
+void orc_stats(int n, const double* density, const double* vel, double invRho0, double mass,
+               double* out) {
+  const float ir = (float)invRho0, m = (float)mass;
+  double sr = 0, mr = 0, mv = 0, ke = 0;
+  for (int i = 0; i < n; i++) {
+    const float rho = (float)density[i] * ir;
+    const float vx = (float)vel[i * 4], vy = (float)vel[i * 4 + 1], vz = (float)vel[i * 4 + 2];
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    sr += rho;
+    mr = std::max(mr, (double)rho);
+    mv = std::max(mv, (double)sqrtf(v2));
+    ke += (double)(0.5f * m * v2);
+  }
+  out[0] = sr / std::max(1, n);
+  out[1] = mr; out[2] = mv; out[3] = ke;
+}
+
+}  // extern "C"

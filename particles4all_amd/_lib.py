@@ -1,0 +1,368 @@
+"""ctypes binding of libp4a.so (include/p4a.h) for Python callers: the
+tests and bench.py.  The JavaScript `Sim` (node/sim.js) is the drop-in
+surface; this module is plumbing over the same C ABI.
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libp4a.so")
+
+P4A_OK = 0
+ERRORS = {1: "P4A_ERR_ARG", 2: "P4A_ERR_HIP", 3: "P4A_ERR_STATE", 4: "P4A_ERR_NOMEM",
+          5: "P4A_ERR_RANGE", 6: "P4A_ERR_NODEV"}
+
+PREC_FP32, PREC_FP16_OFFSET = 0, 1
+MATH_FAST, MATH_EXACT = 0, 1
+SHAPE_SPHERE, SHAPE_TORUS, SHAPE_BOX, SHAPE_LATTICE_CUBE, SHAPE_LATTICE_SPHERE = range(5)
+SCENE_PRESET, SCENE_LATTICE = 0, 1
+
+BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CORR=8,
+           NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
+           PID=15, KEY=16, ORDER=17, BPSI=18)
+STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
+
+
+class Params(C.Structure):
+    _fields_ = [("spacing", C.c_double), ("rest_density", C.c_double), ("gravity", C.c_double),
+                ("substeps", C.c_int32), ("iterations", C.c_int32),
+                ("cfm_epsilon_rel", C.c_double), ("scorr_k", C.c_double),
+                ("scorr_dq", C.c_double), ("xsph_c", C.c_double), ("omega", C.c_double),
+                ("sor_average", C.c_int32), ("surface_tension_k", C.c_double),
+                ("box", C.c_double * 3), ("no_boundary", C.c_int32),
+                ("pour_budget", C.c_int32), ("precision", C.c_int32), ("math", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+class BodySpec(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("size", C.c_double), ("density", C.c_double),
+                ("start_y", C.c_double), ("centre", C.c_double * 3)]
+
+
+class SceneSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("target_particle_count", C.c_int32),
+                ("body_size", C.c_double), ("lattice", C.c_int32 * 3),
+                ("origin", C.c_double * 3), ("jitter", C.c_double), ("seed", C.c_uint32),
+                ("calibrate_unjittered", C.c_int32), ("n_bodies", C.c_int32),
+                ("bodies", C.POINTER(BodySpec))]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("n", C.c_int32), ("n_fluid", C.c_int32), ("n_body", C.c_int32),
+                ("pos", C.POINTER(C.c_float)), ("vel", C.POINTER(C.c_float)),
+                ("rest", C.POINTER(C.c_float)), ("phase", C.POINTER(C.c_uint32)),
+                ("n_boundary", C.c_int32), ("boundary_pts", C.POINTER(C.c_float)),
+                ("boundary_psi", C.POINTER(C.c_float)), ("n_bodies", C.c_int32),
+                ("body_centre", C.POINTER(C.c_double)), ("body_density", C.POINTER(C.c_double)),
+                ("body_count", C.POINTER(C.c_int32)), ("body_spec", C.POINTER(C.c_int32)),
+                ("mass", C.c_double), ("h", C.c_double),
+                ("d", C.c_double), ("denom_rest", C.c_double)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n", C.c_int32), ("cap", C.c_int32), ("n_fluid", C.c_int32),
+                ("n_body_parts", C.c_int32), ("n_bodies", C.c_int32), ("n_boundary", C.c_int32),
+                ("n_cells", C.c_int32), ("grid_dim", C.c_int32 * 3), ("parity", C.c_int32),
+                ("gen", C.c_int32), ("last_substeps", C.c_int32), ("pour_budget", C.c_int32),
+                ("h", C.c_double), ("mass", C.c_double), ("denom_rest", C.c_double),
+                ("sim_time", C.c_double), ("last_advanced", C.c_double),
+                ("time_bank", C.c_double), ("body_scale", C.c_double), ("cov_scale", C.c_double)]
+
+
+_lib = None
+
+# Every symbol include/p4a.h declares.
+EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene_free",
+           "p4a_boundary_build", "p4a_create", "p4a_destroy", "p4a_last_error", "p4a_reset",
+           "p4a_set_params", "p4a_step", "p4a_substeps", "p4a_resize_box", "p4a_append_fluid",
+           "p4a_apply_ray_impulse", "p4a_hold_body", "p4a_release_body", "p4a_read",
+           "p4a_write", "p4a_device_ptr", "p4a_get_info", "p4a_stats", "p4a_body_pose",
+           "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
+           "p4a_timing", "p4a_version"]
+
+
+def lib():
+    """The loaded libp4a.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                           "(there is no fallback path)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.p4a_params_default.argtypes = [C.POINTER(Params)]
+    L.p4a_scene_build.argtypes = [C.POINTER(Params), C.POINTER(SceneSpec), C.POINTER(vp)]
+    L.p4a_scene_view.argtypes = [vp, C.POINTER(SceneView)]
+    L.p4a_scene_free.argtypes = [vp]
+    L.p4a_scene_free.restype = None
+    L.p4a_boundary_build.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double,
+                                     vp, vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.p4a_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.p4a_destroy.argtypes = [vp]
+    L.p4a_last_error.argtypes = [vp]
+    L.p4a_last_error.restype = C.c_char_p
+    L.p4a_reset.argtypes = [vp, C.POINTER(Params), C.POINTER(SceneView)]
+    L.p4a_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.p4a_step.argtypes = [vp, C.c_double, C.POINTER(C.c_int32)]
+    L.p4a_substeps.argtypes = [vp, C.c_double, C.c_int32]
+    L.p4a_resize_box.argtypes = [vp, C.POINTER(C.c_double)]
+    L.p4a_append_fluid.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.p4a_apply_ray_impulse.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.c_double, C.c_double]
+    L.p4a_hold_body.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.c_double, C.c_double]
+    L.p4a_release_body.argtypes = [vp]
+    L.p4a_read.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
+    L.p4a_write.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
+    L.p4a_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.p4a_get_info.argtypes = [vp, C.POINTER(Info)]
+    L.p4a_stats.argtypes = [vp, C.POINTER(C.c_double)]
+    L.p4a_body_pose.argtypes = [vp, vp, vp]
+    L.p4a_sync.argtypes = [vp]
+    L.p4a_run_stage.argtypes = [vp, C.c_int32, C.c_double]
+    L.p4a_neighbour_census.argtypes = [vp, vp, vp, vp]
+    L.p4a_set_timing.argtypes = [vp, C.c_int32]
+    L.p4a_timing.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
+                             C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
+    L.p4a_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dvec(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def default_params(**kw):
+    p = Params()
+    lib().p4a_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "box":
+            p.box = (C.c_double * 3)(*v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Scene:
+    """Scene arrays built by the library (buildScene or a lattice scene)."""
+
+    def __init__(self, params, spec, bodies=()):
+        L = lib()
+        self._bodies = (BodySpec * max(1, len(bodies)))(*bodies)
+        spec.n_bodies = len(bodies)
+        spec.bodies = C.cast(self._bodies, C.POINTER(BodySpec)) if bodies else None
+        h = C.c_void_p()
+        rc = L.p4a_scene_build(C.byref(params), C.byref(spec), C.byref(h))
+        if rc:
+            raise RuntimeError(f"p4a_scene_build failed: {ERRORS.get(rc, rc)}")
+        self.handle = h
+        self.view = SceneView()
+        L.p4a_scene_view(h, C.byref(self.view))
+        v = self.view
+        n, nb, nB = v.n, v.n_boundary, v.n_bodies
+        self.n, self.n_fluid, self.n_body = n, v.n_fluid, v.n_body
+        self.pos = np.ctypeslib.as_array(v.pos, (max(n, 1) * 4,))[: n * 4].reshape(n, 4)
+        self.vel = np.ctypeslib.as_array(v.vel, (max(n, 1) * 4,))[: n * 4].reshape(n, 4)
+        self.rest = np.ctypeslib.as_array(v.rest, (max(n, 1) * 4,))[: n * 4].reshape(n, 4)
+        self.phase = np.ctypeslib.as_array(v.phase, (max(n, 1) * 4,))[: n * 4].reshape(n, 4)
+        self.boundary_pts = np.ctypeslib.as_array(v.boundary_pts, (max(nb, 1) * 3,))[: nb * 3].reshape(nb, 3)
+        self.boundary_psi = np.ctypeslib.as_array(v.boundary_psi, (max(nb, 1),))[:nb]
+        self.n_bodies = nB
+        if nB:
+            self.body_centre = np.ctypeslib.as_array(v.body_centre, (nB * 3,)).reshape(nB, 3)
+            self.body_density = np.ctypeslib.as_array(v.body_density, (nB,))
+            self.body_count = np.ctypeslib.as_array(v.body_count, (nB,))
+        else:
+            self.body_centre = np.zeros((0, 3))
+            self.body_density = np.zeros(0)
+            self.body_count = np.zeros(0, np.int32)
+        self.mass, self.h, self.d, self.denom_rest = v.mass, v.h, v.d, v.denom_rest
+
+    def free(self):
+        if self.handle:
+            lib().p4a_scene_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def preset_scene(params, target, bodies=(), body_size=0.0):
+    """buildScene(p) for a preset-style parameter set.  `bodies` is a list
+    of (shape, density, startY) as the reference's 'shape:density:startY'
+    strings give them."""
+    shapes = {"sphere": SHAPE_SPHERE, "torus": SHAPE_TORUS}
+    spec = SceneSpec()
+    spec.kind = SCENE_PRESET
+    spec.target_particle_count = int(target)
+    spec.body_size = float(body_size)
+    bs = [BodySpec(shape=shapes.get(s, SHAPE_BOX), size=0.0, density=float(dn),
+                   start_y=float(sy), centre=(C.c_double * 3)(0, 0, 0))
+          for (s, dn, sy) in bodies]
+    return Scene(params, spec, bs)
+
+
+def lattice_scene(params, lattice, origin=(0.0, 0.0, 0.0), jitter=0.05, seed=1, bodies=(),
+                  calibrate_unjittered=True):
+    """Jittered lattice block (sites origin + (i+0.5)d, +-jitter*d) with
+    explicit bodies [(shape, size, density, (cx, cy, cz))]."""
+    spec = SceneSpec()
+    spec.kind = SCENE_LATTICE
+    spec.lattice = (C.c_int32 * 3)(*lattice)
+    spec.origin = _dvec(origin)
+    spec.jitter = float(jitter)
+    spec.seed = int(seed) & 0xFFFFFFFF
+    spec.calibrate_unjittered = 1 if calibrate_unjittered else 0
+    bs = [BodySpec(shape=sh, size=float(sz), density=float(dn), start_y=0.0, centre=_dvec(c))
+          for (sh, sz, dn, c) in bodies]
+    return Scene(params, spec, bs)
+
+
+class Sim:
+    """One scene on one GPU through the C ABI."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.p4a_create(device, C.byref(h))
+        if rc:
+            raise RuntimeError(f"p4a_create({device}) failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().p4a_last_error(self.h).decode()
+            raise RuntimeError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def close(self):
+        if self.h:
+            lib().p4a_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, params, scene):
+        self.params = params
+        self._check(lib().p4a_reset(self.h, C.byref(params), C.byref(scene.view)), "reset")
+
+    def set_params(self, params):
+        self.params = params
+        self._check(lib().p4a_set_params(self.h, C.byref(params)), "set_params")
+
+    def step(self, frame_dt):
+        sub = C.c_int32()
+        self._check(lib().p4a_step(self.h, frame_dt, C.byref(sub)), "step")
+        return sub.value
+
+    def substeps(self, dt, count):
+        self._check(lib().p4a_substeps(self.h, dt, count), "substeps")
+
+    def run_stage(self, stage, dt=0.0):
+        self._check(lib().p4a_run_stage(self.h, STAGE[stage], dt), f"stage {stage}")
+
+    def info(self):
+        i = Info()
+        self._check(lib().p4a_get_info(self.h, C.byref(i)), "info")
+        return i
+
+    def read(self, name, dtype=None, cols=None):
+        i = self.info()
+        n, nc = i.n, i.n_cells
+        spec = {"POS": (np.float32, n, 4), "VEL": (np.float32, n, 4), "PRED": (np.float32, n, 4),
+                "BODY": (np.uint32, n, 4), "REST": (np.float32, n, 4),
+                "LAMBDA": (np.float32, n, 1), "DENSITY": (np.float32, n, 1),
+                "SLOT": (np.uint32, n, 1), "CORR": (np.float32, n, 4),
+                "NORMAL": (np.float32, n, 4), "CELL_START": (np.uint32, nc + 2, 1),
+                "BPOS": (np.float32, i.n_boundary, 4),
+                "BCELL_START": (np.uint32, nc + 2, 1),
+                "BODY_CENTRE": (np.float32, i.n_bodies, 4),
+                "BODY_ROT": (np.float32, i.n_bodies * 3, 4), "PID": (np.uint32, n, 1),
+                "KEY": (np.uint32, n, 1), "ORDER": (np.uint32, n, 1),
+                "BPSI": (np.float32, i.n_boundary, 1)}[name]
+        dt, rows, c = spec
+        out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
+        self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")
+        return out
+
+    def write(self, name, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        self._check(lib().p4a_write(self.h, BUF[name], _ptr(a), offset, a.nbytes), f"write {name}")
+
+    def stats(self):
+        out = (C.c_double * 4)()
+        self._check(lib().p4a_stats(self.h, out), "stats")
+        return list(out)
+
+    def body_pose(self):
+        nB = self.info().n_bodies
+        c = np.zeros((nB, 3), np.float32)
+        r = np.zeros((nB, 9), np.float32)
+        self._check(lib().p4a_body_pose(self.h, _ptr(c), _ptr(r)), "body_pose")
+        return c, r
+
+    def census(self):
+        n = self.info().n
+        a = np.zeros(n, np.uint32)
+        b = np.zeros(n, np.uint32)
+        c = np.zeros(n, np.uint32)
+        self._check(lib().p4a_neighbour_census(self.h, _ptr(a), _ptr(b), _ptr(c)), "census")
+        return a, b, c
+
+    def resize_box(self, box):
+        self._check(lib().p4a_resize_box(self.h, _dvec(box)), "resize_box")
+
+    def append_fluid(self, pos3, vel3):
+        p = np.ascontiguousarray(pos3, np.float32).reshape(-1)
+        v = np.ascontiguousarray(vel3, np.float32).reshape(-1)
+        k = C.c_int32()
+        self._check(lib().p4a_append_fluid(self.h, _ptr(p), _ptr(v), min(p.size, v.size) // 3,
+                                           C.byref(k)), "append_fluid")
+        return k.value
+
+    def apply_ray_impulse(self, origin, direction, impulse, radius, speed_limit):
+        self._check(lib().p4a_apply_ray_impulse(self.h, _dvec(origin), _dvec(direction),
+                                                _dvec(impulse), radius, speed_limit), "impulse")
+
+    def hold_body(self, body, target, rate, limit):
+        self._check(lib().p4a_hold_body(self.h, body, _dvec(target), rate, limit), "hold_body")
+
+    def release_body(self):
+        self._check(lib().p4a_release_body(self.h), "release_body")
+
+    def sync(self):
+        self._check(lib().p4a_sync(self.h), "sync")
+
+    def set_timing(self, on):
+        self._check(lib().p4a_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def timing(self):
+        names = C.create_string_buffer(4096)
+        ms = (C.c_double * 64)()
+        cnt = (C.c_int32 * 64)()
+        k = C.c_int32()
+        self._check(lib().p4a_timing(self.h, names, 4096, ms, cnt, 64, C.byref(k)), "timing")
+        keys = names.value.decode().split("\n")[: k.value]
+        return {key: (ms[i], cnt[i]) for i, key in enumerate(keys)}
+
+    def device_ptr(self, name):
+        p = C.c_void_p()
+        b = C.c_size_t()
+        self._check(lib().p4a_device_ptr(self.h, BUF[name], C.byref(p), C.byref(b)), "device_ptr")
+        return p.value, b.value

@@ -1,0 +1,1189 @@
+// gfx950 kernels of the Particles4All simulation step.
+//
+// Reference kernels (src/wgsl.js) and their counterparts here:
+//   predict 56-70 + count 352-362        -> k_predict_count (fused)
+//   scanBlock/scanBlocks/scanAdd 364-423 -> k_scan_reduce + k_scan_add
+//   scatterSlot 425-438 (atomic slot)    -> k_scatter_tmp + k_sort_move
+//   scatterMove/scatterPhase 440-475        (stable order, fused move)
+//   lambda 477-543                       -> k_tile<LambdaOp>
+//   delta 545-612                        -> k_tile<DeltaOp>
+//   bodyCentre/bodySeed/bodyRefLoad      -> k_body_pre      (FP64)
+//   bodyCentre/Seed/Cov/Resolve/Project  -> k_body_solve    (FP64)
+//   velFromPos 72-83, xsph 85-123,
+//   normals 250-288                      -> k_tile<XsphNormalsOp> / <XsphOp>
+//   tension 290-335 + finalize 337-350   -> k_tile<TensionFinalizeOp>
+//   finalize 337-350                     -> k_finalize (no tension)
+//   stats 785-832                        -> k_stats_partial + k_stats_final
+//   bodyDrag 187-214, impulse 216-248, resize 125-157 -> k_body_drag,
+//                                           k_impulse, k_resize
+//
+// Neighbour kernels work on row tiles: a workgroup owns TX consecutive
+// cells of one grid row (y, z), stages the particle records of the 3x3
+// neighbouring rows (cells x0-1 .. x0+TX) in LDS once, and then every
+// particle sorted into the tile walks its 27-cell stencil out of LDS.  A
+// particle whose current cell has left the tile's row (the grid is stale
+// during the solver iterations) walks the same stencil out of global
+// memory instead, so both paths visit the same candidates in the same
+// order.
+//
+// Arithmetic: with EXACT the float operations are those of the reference
+// kernels, one IEEE-rounded operation each (the file is compiled with
+// -ffp-contract=off), so a float CPU evaluation of the same sequence gives
+// identical bits.  The one fixed choice is r^2 = fma(z,z, fma(y,y, x*x)),
+// used by every path.  Without EXACT, sqrt/division pairs become the
+// hardware reciprocal square root and constants are folded.
+#include "p4a_internal.h"
+
+namespace p4a {
+namespace {
+
+constexpr int WG = 256;
+constexpr int SCAN_CH = 2048;  // cells per scan block (256 threads x 8)
+
+__device__ __forceinline__ float r2_of(float x, float y, float z) {
+  return __fmaf_rn(z, z, __fmaf_rn(y, y, x * x));
+}
+
+// cellOf + cellIndex (wgsl.js:43-49); the clamp is done on the float so a
+// NaN position lands in cell 0 instead of an undefined integer.
+struct Cell { int x, y, z; };
+__device__ __forceinline__ Cell cell_of(const Uniform& U, float px, float py, float pz) {
+  float fx = floorf((px - U.boxMin[0]) / U.h);
+  float fy = floorf((py - U.boxMin[1]) / U.h);
+  float fz = floorf((pz - U.boxMin[2]) / U.h);
+  fx = fminf(fmaxf(fx, 0.0f), (float)(U.gridDim[0] - 1));
+  fy = fminf(fmaxf(fy, 0.0f), (float)(U.gridDim[1] - 1));
+  fz = fminf(fmaxf(fz, 0.0f), (float)(U.gridDim[2] - 1));
+  return Cell{(int)fx, (int)fy, (int)fz};
+}
+__device__ __forceinline__ uint32_t cell_index(const Uniform& U, Cell c) {
+  return (uint32_t)((c.z * U.gridDim[1] + c.y) * U.gridDim[0] + c.x);
+}
+
+// poly6(r2) of wgsl.js:50-53 for r2 < h2: ((poly6 * t) * t) * t.
+__device__ __forceinline__ float poly6_w(const Uniform& U, float r2) {
+  float t = U.h2 - r2;
+  return U.poly6 * t * t * t;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8).  Hand each XCD a contiguous range of tiles so neighbouring
+// tiles, which share halo rows, share one L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  int xcd = bid & 7, k = bid >> 3;
+  int per = nb >> 3, rem = nb & 7;
+  return xcd * per + min(xcd, rem) + k;
+}
+
+// ---------------------------------------------------------------------------
+// Wave helpers for runs of equal keys in a 64-lane wavefront.
+// Valid lanes form a prefix of the wave (i < n); each maximal run of equal
+// keys among adjacent valid lanes is handled by its first lane (the head).
+struct Run {
+  bool head;
+  int headLane;
+  int len;  // valid for heads
+};
+__device__ __forceinline__ Run wave_runs(uint32_t key, bool ok) {
+  const int lane = threadIdx.x & 63;
+  uint32_t prev = __shfl_up(key, 1, 64);
+  bool head = ok && (lane == 0 || prev != key);
+  unsigned long long heads = __ballot(head);
+  unsigned long long valid = __ballot(ok);
+  Run r;
+  r.head = head;
+  unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  unsigned long long hl = heads & le;
+  r.headLane = hl ? 63 - __builtin_clzll(hl) : 0;
+  unsigned long long gt = (lane == 63) ? 0ull : (~0ull << (lane + 1));
+  unsigned long long nx = heads & gt;
+  int end = nx ? __builtin_ctzll(nx) : __popcll(valid);
+  r.len = end - lane;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// predict (wgsl.js:56-70) fused with count (wgsl.js:352-362).
+// v = vel + dt*(0,-g,0); pred = (pos + dt*v, 1).  The updated v itself is not
+// stored: after the grid build the reference overwrites vel with
+// (pred - pos)/dt (velFromPos, wgsl.js:72-83) before anything reads it.
+__global__ void __launch_bounds__(WG)
+k_predict_count(Uniform U, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                float4* __restrict__ pred, uint32_t* __restrict__ key,
+                uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  const bool ok = i < U.n;
+  uint32_t c = 0xffffffffu;
+  if (ok) {
+    float4 p = pos[i], v = vel[i];
+    float gx = U.dt * 0.0f, gy = U.dt * (-U.gravity), gz = U.dt * 0.0f;
+    float vx = v.x + gx, vy = v.y + gy, vz = v.z + gz;
+    float qx = p.x + U.dt * vx, qy = p.y + U.dt * vy, qz = p.z + U.dt * vz;
+    pred[i] = make_float4(qx, qy, qz, 1.0f);
+    c = cell_index(U, cell_of(U, qx, qy, qz));
+    key[i] = c;
+  }
+  Run r = wave_runs(c, ok);
+  if (r.head) atomicAdd(&count[c], (uint32_t)r.len);
+}
+
+// Exclusive scan of cellCount into cellStart (wgsl.js:364-423), two passes:
+// block totals, then each block adds the totals before it and scans its own
+// 2048 cells.  Counts are zeroed as they are consumed; they are reused as
+// the scatter cursor.
+__global__ void __launch_bounds__(WG)
+k_scan_reduce(const uint32_t* __restrict__ count, uint32_t nCells, uint32_t* __restrict__ blockSum) {
+  __shared__ uint32_t red[WG / 64];
+  const uint32_t base = blockIdx.x * SCAN_CH + threadIdx.x * 8;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) s += (base + k < nCells) ? count[base + k] : 0u;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WG / 64; w++) t += red[w];
+    blockSum[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+k_scan_add(uint32_t* __restrict__ count, uint32_t nCells, const uint32_t* __restrict__ blockSum,
+           uint32_t* __restrict__ cellStart) {
+  __shared__ uint32_t red[WG / 64];
+  __shared__ uint32_t woff[WG / 64];
+  __shared__ uint32_t sPrefix;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Sum of the block totals before this block.
+  uint32_t pre = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += WG) pre += blockSum[b];
+  for (int off = 32; off > 0; off >>= 1) pre += __shfl_xor(pre, off, 64);
+  if (lane == 0) red[wave] = pre;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < WG / 64; w++) t += red[w];
+    sPrefix = t;
+  }
+  // Thread-local counts.
+  const uint32_t base = blockIdx.x * SCAN_CH + threadIdx.x * 8;
+  uint32_t v[8];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    v[k] = (base + k < nCells) ? count[base + k] : 0u;
+    s += v[k];
+  }
+  // Inclusive wave scan of thread sums.
+  uint32_t inc = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  __syncthreads();
+  if (lane == 63) woff[wave] = inc;
+  __syncthreads();
+  uint32_t wpre = 0;
+  for (int w = 0; w < wave; w++) wpre += woff[w];
+  uint32_t run = sPrefix + wpre + inc - s;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (base + k < nCells) {
+      cellStart[base + k] = run;
+      count[base + k] = 0u;
+    }
+    run += v[k];
+  }
+  if (base <= nCells && nCells < base + 8) {
+    // the thread owning the last cell writes cellStart[nCells] = total
+    cellStart[nCells] = run;
+  }
+}
+
+// Unstable placement by atomics (the reference's scatterSlot order), one
+// atomic per run of equal cells in a wave; k_sort_move then orders every
+// cell's segment by pre-sort index, which makes the result the stable
+// counting sort whatever order the atomics resolved in.
+__global__ void __launch_bounds__(WG)
+k_scatter_tmp(uint32_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ cellStart,
+              uint32_t* __restrict__ cursor, uint32_t* __restrict__ tmp) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  const bool ok = i < n;
+  const uint32_t c = ok ? key[i] : 0xffffffffu;
+  Run r = wave_runs(c, ok);
+  uint32_t base = 0;
+  if (r.head) base = atomicAdd(&cursor[c], (uint32_t)r.len);
+  base = __shfl(base, r.headLane, 64);
+  if (ok) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t d = cellStart[c] + base + (uint32_t)(lane - r.headLane);
+    if (d < n) tmp[d] = i;  // holds unless counts and keys disagree
+  }
+}
+
+// Stable rank inside each cell segment, then the moves of scatterMove and
+// scatterPhase (wgsl.js:440-475) as a gather into sorted slots.  Writes
+// slot (the reference's scatterSlot output) and the inverse map order.
+// Velocities move only when `vel` is given: inside a substep they are dead
+// until velFromPos rewrites them, so the hot path skips that traffic.
+__global__ void __launch_bounds__(WG)
+k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ key,
+            const uint32_t* __restrict__ cellStart, uint32_t* __restrict__ count,
+            const float4* __restrict__ pos, const float4* __restrict__ pred,
+            const uint32_t* __restrict__ pid, float4* __restrict__ posOut,
+            float4* __restrict__ predOut, uint32_t* __restrict__ pidOut,
+            uint32_t* __restrict__ order, uint32_t* __restrict__ slot,
+            uint32_t* __restrict__ slotOf, const float4* __restrict__ vel,
+            float4* __restrict__ velOut) {
+  const uint32_t s = blockIdx.x * WG + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t i = tmp[s];
+  const uint32_t c = key[i];
+  const uint32_t b = cellStart[c], e = min(cellStart[c + 1], n);
+  uint32_t rank = 0;
+  for (uint32_t t = b; t < e; t++) rank += (tmp[t] < i) ? 1u : 0u;
+  const uint32_t d = b + rank;
+  if (d >= n) return;  // holds unless counts and keys disagree
+  order[d] = i;
+  slot[i] = d;
+  posOut[d] = pos[i];
+  predOut[d] = pred[i];
+  const uint32_t q = pid[i];
+  pidOut[d] = q;
+  slotOf[q] = d;
+  if (vel) velOut[d] = vel[i];
+  if (s == b) count[c] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Row-tiled neighbour traversal.
+template <int NREC, int TX, int MAXREC>
+struct TileLds {
+  float4 rec[NREC][MAXREC];
+  uint32_t cs[9][TX + 3];   // cellStart of cells xa..xb of each row
+  int off[10];              // LDS offset of each row's first record
+  uint32_t base[9];         // global index of each row's first record
+  int xa[9];
+  int anyB;
+  int nonEmpty;
+};
+
+template <class Op, int TX, int NT, int MAXREC>
+__global__ void __launch_bounds__(NT)
+k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+       const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+       int tilesPerRow) {
+  __shared__ TileLds<Op::NREC, TX, MAXREC> S;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int txi = tile % tilesPerRow;
+  const int ty = (tile / tilesPerRow) % gy;
+  const int tz = tile / (tilesPerRow * gy);
+  const int x0 = txi * TX;
+  const int x1 = min(x0 + TX, gx);
+  const int xa = max(x0 - 1, 0), xb = min(x1 + 1, gx);
+  const int tid = threadIdx.x;
+
+  // Empty tiles leave at once.
+  const uint32_t rowC = (uint32_t)((tz * gy + ty) * gx);
+  if (tid == 0) S.nonEmpty = cellStart[rowC + x1] != cellStart[rowC + x0];
+  __syncthreads();
+  if (!S.nonEmpty) return;
+
+  // cellStart slices of the 9 neighbour rows.
+  const int W = xb - xa + 1;
+  for (int e = tid; e < 9 * (TX + 3); e += NT) {
+    const int r = e / (TX + 3), k = e - r * (TX + 3);
+    const int z = tz + r / 3 - 1, y = ty + r % 3 - 1;
+    if (k < W && z >= 0 && z < gz && y >= 0 && y < gy)
+      S.cs[r][k] = cellStart[(uint32_t)((z * gy + y) * gx + xa + k)];
+  }
+  if (tid < 9) {
+    S.xa[tid] = xa;
+    S.anyB = 0;
+  }
+  __syncthreads();
+  if (tid < 9) {
+    const int r = tid;
+    const int z = tz + r / 3 - 1, y = ty + r % 3 - 1;
+    const bool valid = z >= 0 && z < gz && y >= 0 && y < gy;
+    int cnt = valid ? (int)(S.cs[r][W - 1] - S.cs[r][0]) : 0;
+    S.base[r] = valid ? S.cs[r][0] : 0u;
+    // exclusive prefix over rows, done by each of the 9 threads
+    int o = 0;
+    for (int q = 0; q < r; q++) {
+      const int zq = tz + q / 3 - 1, yq = ty + q % 3 - 1;
+      if (zq >= 0 && zq < gz && yq >= 0 && yq < gy) o += (int)(S.cs[q][W - 1] - S.cs[q][0]);
+    }
+    S.off[r] = o;
+    if (r == 8) S.off[9] = o + cnt;
+    if (valid && U.nBoundary > 0) {
+      const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+      if (bcellStart[rb + xb] != bcellStart[rb + xa]) atomicOr(&S.anyB, 1);
+    }
+  }
+  __syncthreads();
+  const int total = S.off[9];
+  const bool ldsOk = total <= MAXREC;
+  if (ldsOk) {
+    for (int idx = tid; idx < total; idx += NT) {
+      int r = 0;
+#pragma unroll
+      for (int q = 1; q < 9; q++) r += (idx >= S.off[q]) ? 1 : 0;
+      const uint32_t j = S.base[r] + (uint32_t)(idx - S.off[r]);
+      float4 v[Op::NREC];
+      op.load(U, j, v);
+#pragma unroll
+      for (int k = 0; k < Op::NREC; k++) S.rec[k][idx] = v[k];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t T0 = S.cs[4][x0 - xa], T1 = S.cs[4][x1 - xa];
+  const bool anyB = S.anyB != 0;
+  for (uint32_t i = T0 + tid; i < T1; i += NT) {
+    typename Op::Target t;
+    op.target(U, i, t);
+    const Cell c = cell_of(U, t.px, t.py, t.pz);
+    const int xlo = max(c.x - 1, 0), xhi = min(c.x + 1, gx - 1);
+    const bool inTile = ldsOk && c.y == ty && c.z == tz && c.x >= x0 && c.x < x1;
+    const bool doB = U.nBoundary > 0 && (anyB || !inTile);
+    for (int dz = -1; dz <= 1; dz++) {
+      const int z = c.z + dz;
+      if (z < 0 || z >= gz) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int y = c.y + dy;
+        if (y < 0 || y >= gy) continue;
+        const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+        if (inTile) {
+          const int r = (dz + 1) * 3 + (dy + 1);
+          const int rel = S.off[r] - (int)S.base[r];
+          const int b = (int)S.cs[r][xlo - xa] + rel;
+          const int e = (int)S.cs[r][xhi + 1 - xa] + rel;
+          for (int q = b; q < e; q++) {
+            float4 v[Op::NREC];
+#pragma unroll
+            for (int k = 0; k < Op::NREC; k++) v[k] = S.rec[k][q];
+            op.pair(U, t, (uint32_t)(q - rel), v);
+          }
+        } else {
+          const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+          for (uint32_t j = b; j < e; j++) {
+            float4 v[Op::NREC];
+            op.load(U, j, v);
+            op.pair(U, t, j, v);
+          }
+        }
+        if (doB) {
+          const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+          for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+        }
+      }
+    }
+    op.finish(U, i, t);
+  }
+}
+
+// --- lambda (wgsl.js:477-543) ----------------------------------------------
+template <bool EXACT>
+struct LambdaOp {
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pred;
+  float* __restrict__ density;
+  float* __restrict__ lambda;
+  struct Target {
+    float px, py, pz;
+    uint32_t i;
+    float rho, gx, gy, gz, sg2;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
+    float4 p = pred[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z; t.i = i;
+    t.rho = U.mass * (U.poly6 * U.h2 * U.h2 * U.h2);  // P.mass * poly6(0.0)
+    t.gx = t.gy = t.gz = t.sg2 = 0.0f;
+  }
+  __device__ void pair(const Uniform& U, Target& t, uint32_t j, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (j == t.i || r2 >= U.h2) return;
+    if (EXACT) {
+      t.rho = t.rho + U.mass * poly6_w(U, r2);
+      if (r2 > 1.0e-12f) {
+        const float r = sqrtf(r2);
+        const float hr = U.h - r;
+        const float s = U.volume * U.spikyGrad * hr * hr / r;
+        const float gx = s * rx, gy = s * ry, gz = s * rz;
+        t.gx = t.gx + gx; t.gy = t.gy + gy; t.gz = t.gz + gz;
+        t.sg2 = t.sg2 + (gx * gx + gy * gy + gz * gz);
+      }
+    } else {
+      const float tt = U.h2 - r2;
+      t.rho = __fmaf_rn(U.mass * U.poly6, tt * tt * tt, t.rho);
+      if (r2 > 1.0e-12f) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float hr = __fmaf_rn(-r2, ir, U.h);
+        const float s = (U.volume * U.spikyGrad) * hr * hr * ir;
+        t.gx = __fmaf_rn(s, rx, t.gx);
+        t.gy = __fmaf_rn(s, ry, t.gy);
+        t.gz = __fmaf_rn(s, rz, t.gz);
+        t.sg2 = __fmaf_rn(s * s, r2, t.sg2);
+      }
+    }
+  }
+  __device__ void bpair(const Uniform& U, Target& t, float4 b) const {
+    const float rx = t.px - b.x, ry = t.py - b.y, rz = t.pz - b.z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    const float psi = b.w;
+    if (EXACT) {
+      t.rho = t.rho + psi * poly6_w(U, r2);
+      const float r = sqrtf(r2);
+      const float hr = U.h - r;
+      const float s = psi * U.invRho0 * U.spikyGrad * hr * hr / r;
+      t.gx = t.gx + s * rx; t.gy = t.gy + s * ry; t.gz = t.gz + s * rz;
+    } else {
+      const float tt = U.h2 - r2;
+      t.rho = __fmaf_rn(psi * U.poly6, tt * tt * tt, t.rho);
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float hr = __fmaf_rn(-r2, ir, U.h);
+      const float s = psi * (U.invRho0 * U.spikyGrad) * hr * hr * ir;
+      t.gx = __fmaf_rn(s, rx, t.gx);
+      t.gy = __fmaf_rn(s, ry, t.gy);
+      t.gz = __fmaf_rn(s, rz, t.gz);
+    }
+  }
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    density[i] = t.rho;
+    const float C = t.rho * U.invRho0 - 1.0f;
+    float lam = 0.0f;
+    if (C > 0.0f) {
+      const float gg = t.gx * t.gx + t.gy * t.gy + t.gz * t.gz;
+      lam = -C / (gg + t.sg2 + U.cfmEps + 1.0e-12f);
+    }
+    lambda[i] = lam;
+  }
+};
+
+// --- delta (wgsl.js:545-612) -----------------------------------------------
+template <bool EXACT>
+struct DeltaOp {
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pred;
+  const float* __restrict__ lambda;
+  float4* __restrict__ out;
+  struct Target {
+    float px, py, pz, li;
+    float dx, dy, dz, bx, by, bz, nc;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+    float4 p = pred[j];
+    v[0] = make_float4(p.x, p.y, p.z, lambda[j]);
+  }
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
+    float4 p = pred[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z; t.li = lambda[i];
+    t.dx = t.dy = t.dz = t.bx = t.by = t.bz = 0.0f;
+    t.nc = 1.0f;
+  }
+  __device__ void pair(const Uniform& U, Target& t, uint32_t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    if (EXACT) {
+      const float r = sqrtf(r2);
+      const float hr = U.h - r;
+      const float s = U.spikyGrad * hr * hr / r;
+      const float tt = U.h2 - r2;
+      const float ratio = U.poly6 * tt * tt * tt * U.sCorrWq;
+      const float r2r = ratio * ratio;
+      const float sc = -U.sCorrK * r2r * r2r;
+      const float f = (t.li + v[0].w + sc) * s;
+      t.dx = t.dx + f * rx; t.dy = t.dy + f * ry; t.dz = t.dz + f * rz;
+    } else {
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float hr = __fmaf_rn(-r2, ir, U.h);
+      const float s = U.spikyGrad * hr * hr * ir;
+      const float tt = U.h2 - r2;
+      const float ratio = (U.poly6 * U.sCorrWq) * (tt * tt * tt);
+      const float r2r = ratio * ratio;
+      const float sc = -U.sCorrK * r2r * r2r;
+      const float f = (t.li + v[0].w + sc) * s;
+      t.dx = __fmaf_rn(f, rx, t.dx);
+      t.dy = __fmaf_rn(f, ry, t.dy);
+      t.dz = __fmaf_rn(f, rz, t.dz);
+    }
+    t.nc = t.nc + 1.0f;
+  }
+  __device__ void bpair(const Uniform& U, Target& t, float4 b) const {
+    const float rx = t.px - b.x, ry = t.py - b.y, rz = t.pz - b.z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    if (EXACT) {
+      const float r = sqrtf(r2);
+      const float hr = U.h - r;
+      const float f = (t.li * b.w) * (U.spikyGrad * hr * hr / r);
+      t.bx = t.bx + f * rx; t.by = t.by + f * ry; t.bz = t.bz + f * rz;
+    } else {
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float hr = __fmaf_rn(-r2, ir, U.h);
+      const float f = (t.li * b.w) * (U.spikyGrad * hr * hr * ir);
+      t.bx = __fmaf_rn(f, rx, t.bx);
+      t.by = __fmaf_rn(f, ry, t.by);
+      t.bz = __fmaf_rn(f, rz, t.bz);
+    }
+  }
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float dx = t.dx * U.volume + t.bx * U.invRho0;
+    const float dy = t.dy * U.volume + t.by * U.invRho0;
+    const float dz = t.dz * U.volume + t.bz * U.invRho0;
+    float scale = U.omega;
+    if (U.sorAverage == 1u) scale = U.omega / fmaxf(t.nc, 1.0f);
+    float ox = t.px + dx * scale, oy = t.py + dy * scale, oz = t.pz + dz * scale;
+    ox = fminf(fmaxf(ox, U.clampMin[0]), U.clampMax[0]);
+    oy = fminf(fmaxf(oy, U.clampMin[1]), U.clampMax[1]);
+    oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
+    out[i] = make_float4(ox, oy, oz, 1.0f);
+  }
+};
+
+// --- velFromPos + xsph + normals (wgsl.js:72-123, 250-288) -----------------
+// Records: (pred.xyz, max(rho_j, rho0/2)), (vel_j.xyz, 0) with
+// vel_j = (pred_j - pos_j) * invDt, the value velFromPos writes.
+template <bool EXACT, bool NORMALS>
+struct XsphOp {
+  static constexpr int NREC = 2;
+  const float4* __restrict__ pred;
+  float4* __restrict__ pos;
+  const float* __restrict__ density;
+  float4* __restrict__ corr;
+  float4* __restrict__ normal;
+  struct Target {
+    float px, py, pz, vx, vy, vz;
+    float ax, ay, az, nx, ny, nz;
+  };
+  __device__ void load(const Uniform& U, uint32_t j, float4* v) const {
+    float4 p = pred[j], q = pos[j];
+    v[0] = make_float4(p.x, p.y, p.z, fmaxf(density[j], 0.5f * U.rho0));
+    v[1] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt, 0.0f);
+  }
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
+    float4 p = pred[i], q = pos[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.vx = (p.x - q.x) * U.invDt; t.vy = (p.y - q.y) * U.invDt; t.vz = (p.z - q.z) * U.invDt;
+    t.ax = t.ay = t.az = t.nx = t.ny = t.nz = 0.0f;
+  }
+  __device__ void pair(const Uniform& U, Target& t, uint32_t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2) return;
+    const float den = v[0].w;
+    if (EXACT) {
+      const float f = U.mass * poly6_w(U, r2) / den;
+      t.ax = t.ax + f * (v[1].x - t.vx);
+      t.ay = t.ay + f * (v[1].y - t.vy);
+      t.az = t.az + f * (v[1].z - t.vz);
+      if (NORMALS && r2 >= 1.0e-12f) {
+        const float r = sqrtf(r2);
+        const float hr = U.h - r;
+        const float g = (U.mass / den) * (U.spikyGrad * hr * hr / r);
+        t.nx = t.nx + g * rx; t.ny = t.ny + g * ry; t.nz = t.nz + g * rz;
+      }
+    } else {
+      const float inv = __builtin_amdgcn_rcpf(den);
+      const float tt = U.h2 - r2;
+      const float f = (U.mass * U.poly6) * (tt * tt * tt) * inv;
+      t.ax = __fmaf_rn(f, v[1].x - t.vx, t.ax);
+      t.ay = __fmaf_rn(f, v[1].y - t.vy, t.ay);
+      t.az = __fmaf_rn(f, v[1].z - t.vz, t.az);
+      if (NORMALS && r2 >= 1.0e-12f) {
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float hr = __fmaf_rn(-r2, ir, U.h);
+        const float g = (U.mass * U.spikyGrad) * inv * hr * hr * ir;
+        t.nx = __fmaf_rn(g, rx, t.nx);
+        t.ny = __fmaf_rn(g, ry, t.ny);
+        t.nz = __fmaf_rn(g, rz, t.nz);
+      }
+    }
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    corr[i] = make_float4(U.xsphC * t.ax, U.xsphC * t.ay, U.xsphC * t.az, 0.0f);
+    if (NORMALS) normal[i] = make_float4(U.h * t.nx, U.h * t.ny, U.h * t.nz, 0.0f);
+  }
+};
+
+// --- tension (wgsl.js:290-335) + velFromPos/finalize (72-83, 337-350) -------
+// Records: (pred.xyz, rho_j), (normal_j.xyz, 0).
+template <bool EXACT>
+struct TensionOp {
+  static constexpr int NREC = 2;
+  const float4* __restrict__ pred;
+  float4* __restrict__ pos;
+  const float* __restrict__ density;
+  const float4* __restrict__ normal;
+  float4* __restrict__ corr;
+  float4* __restrict__ vel;
+  struct Target {
+    float px, py, pz, nx, ny, nz, rho;
+    float ax, ay, az;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+    float4 p = pred[j];
+    v[0] = make_float4(p.x, p.y, p.z, density[j]);
+    v[1] = normal[j];
+  }
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    float4 p = pred[i], nn = normal[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
+    t.rho = density[i];
+    t.ax = t.ay = t.az = 0.0f;
+  }
+  __device__ void pair(const Uniform& U, Target& t, uint32_t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    if (EXACT) {
+      const float r = sqrtf(r2);
+      const float hrr = U.h - r;
+      const float k = hrr * hrr * hrr * r * r * r;
+      float cw = U.cohes * (2.0f * k - U.cohesTerm);
+      if (2.0f * r > U.h) cw = U.cohes * k;
+      const float fc = -U.tension * U.mass * cw / r;
+      const float fx = fc * rx + (-U.tension * (t.nx - v[1].x));
+      const float fy = fc * ry + (-U.tension * (t.ny - v[1].y));
+      const float fz = fc * rz + (-U.tension * (t.nz - v[1].z));
+      const float K = 2.0f * U.rho0 / (t.rho + v[0].w);
+      t.ax = t.ax + K * fx; t.ay = t.ay + K * fy; t.az = t.az + K * fz;
+    } else {
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * ir;
+      const float hrr = U.h - r;
+      const float k = hrr * hrr * hrr * r2 * r;
+      const float cw = (2.0f * r > U.h) ? U.cohes * k : U.cohes * (2.0f * k - U.cohesTerm);
+      const float fc = (-U.tension * U.mass) * cw * ir;
+      const float mg = -U.tension;
+      const float fx = __fmaf_rn(fc, rx, mg * (t.nx - v[1].x));
+      const float fy = __fmaf_rn(fc, ry, mg * (t.ny - v[1].y));
+      const float fz = __fmaf_rn(fc, rz, mg * (t.nz - v[1].z));
+      const float K = (2.0f * U.rho0) * __builtin_amdgcn_rcpf(t.rho + v[0].w);
+      t.ax = __fmaf_rn(K, fx, t.ax);
+      t.ay = __fmaf_rn(K, fy, t.ay);
+      t.az = __fmaf_rn(K, fz, t.az);
+    }
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float4 c = corr[i];
+    const float cx = c.x + U.dt * t.ax, cy = c.y + U.dt * t.ay, cz = c.z + U.dt * t.az;
+    const float4 q = pos[i];
+    const float vx = (t.px - q.x) * U.invDt, vy = (t.py - q.y) * U.invDt,
+                vz = (t.pz - q.z) * U.invDt;
+    corr[i] = make_float4(cx, cy, cz, 0.0f);
+    vel[i] = make_float4(vx + cx, vy + cy, vz + cz, 0.0f);
+    pos[i] = pred[i];
+  }
+};
+
+// --- neighbour census: |N(i)| and index checksums -------------------------
+struct CensusOp {
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pred;
+  uint32_t* __restrict__ cnt;
+  uint32_t* __restrict__ s1;
+  uint32_t* __restrict__ s2;
+  struct Target {
+    float px, py, pz;
+    uint32_t i, c, a, b;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    float4 p = pred[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z; t.i = i; t.c = t.a = t.b = 0u;
+  }
+  __device__ void pair(const Uniform& U, Target& t, uint32_t j, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (j == t.i || r2 >= U.h2) return;
+    t.c += 1u; t.a += j; t.b += j * j;
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform&, uint32_t i, Target& t) const {
+    cnt[i] = t.c; s1[i] = t.a; s2[i] = t.b;
+  }
+};
+
+// finalize (wgsl.js:337-350) with velFromPos (72-83) folded in, for the
+// no-tension path.
+__global__ void __launch_bounds__(WG)
+k_finalize(Uniform U, const float4* __restrict__ pred, float4* __restrict__ pos,
+           const float4* __restrict__ corr, float4* __restrict__ vel) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= U.n) return;
+  const float4 p = pred[i], q = pos[i], c = corr[i];
+  const float vx = (p.x - q.x) * U.invDt, vy = (p.y - q.y) * U.invDt, vz = (p.z - q.z) * U.invDt;
+  vel[i] = make_float4(vx + c.x, vy + c.y, vz + c.z, 0.0f);
+  pos[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// Rigid bodies, FP64.  One workgroup per body; the body's particles are the
+// static id list bodyPids[bodyFirst[b] .. bodyFirst[b+1]), located through
+// slotOf.
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int K>
+__device__ void block_sum_d(double (&v)[K], double (*sh)[K]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = wave_sum_d(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) sh[wave][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double t = 0;
+    for (int w = 0; w < WG / 64; w++) t += sh[w][k];
+    v[k] = t;
+  }
+}
+
+// bodyCentre + bodySeed + bodyRefLoad before the solve (wgsl.js:628-668,
+// 174-185; sim.js:496-516): bodyCentre becomes the centroid of pred.
+__global__ void __launch_bounds__(WG)
+k_body_pre(const float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
+           const uint32_t* __restrict__ bodyPids, const uint32_t* __restrict__ bodyFirst,
+           double* __restrict__ centreD, float4* __restrict__ centreF) {
+  __shared__ double sh[WG / 64][3];
+  const int b = blockIdx.x;
+  const uint32_t f = bodyFirst[b], l = bodyFirst[b + 1];
+  const double cx = centreD[b * 4], cy = centreD[b * 4 + 1], cz = centreD[b * 4 + 2];
+  double v[3] = {0, 0, 0};
+  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
+    const float4 p = pred[slotOf[bodyPids[k]]];
+    v[0] += (double)p.x - cx;
+    v[1] += (double)p.y - cy;
+    v[2] += (double)p.z - cz;
+  }
+  block_sum_d<3>(v, sh);
+  if (threadIdx.x == 0 && l > f) {
+    const double cnt = (double)(l - f);
+    const double nx = cx + v[0] / cnt, ny = cy + v[1] / cnt, nz = cz + v[2] / cnt;
+    centreD[b * 4] = nx; centreD[b * 4 + 1] = ny; centreD[b * 4 + 2] = nz;
+    centreF[b] = make_float4((float)nx, (float)ny, (float)nz, 0.0f);
+  }
+}
+
+// One shape-matching round (wgsl.js:628-783, sim.js:544-563): centroid and
+// A = sum (p - centroid) q^T in one pass, using
+// sum (p - c0 - m) q^T = sum (p - c0) q^T - m (sum q)^T with m the mean of
+// p - c0; centre moves by m * invRatio (buoyancy, wgsl.js:715-721); R is
+// the orthogonal polar factor of A / cbrt|det A| by at most 8 Newton steps
+// R <- (R + R^-T)/2 (wgsl.js:727-742); pred = centre + R q in float.
+__global__ void __launch_bounds__(WG)
+k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
+             const uint32_t* __restrict__ bodyPids, const uint32_t* __restrict__ bodyFirst,
+             const float4* __restrict__ restOfPid, const float* __restrict__ invRatio,
+             double* __restrict__ centreD, double* __restrict__ rotD,
+             float4* __restrict__ centreF, float4* __restrict__ rotF) {
+  __shared__ double sh[WG / 64][15];
+  __shared__ float sc[12];
+  const int b = blockIdx.x;
+  const uint32_t f = bodyFirst[b], l = bodyFirst[b + 1];
+  if (l <= f) return;
+  const double cx = centreD[b * 4], cy = centreD[b * 4 + 1], cz = centreD[b * 4 + 2];
+  double v[15];
+#pragma unroll
+  for (int k = 0; k < 15; k++) v[k] = 0;
+  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
+    const uint32_t q = bodyPids[k];
+    const float4 p = pred[slotOf[q]];
+    const float4 r = restOfPid[q];
+    const double px = (double)p.x - cx, py = (double)p.y - cy, pz = (double)p.z - cz;
+    const double qx = r.x, qy = r.y, qz = r.z;
+    v[0] += px; v[1] += py; v[2] += pz;
+    v[3] += qx; v[4] += qy; v[5] += qz;
+    v[6] += px * qx; v[7] += px * qy; v[8] += px * qz;
+    v[9] += py * qx; v[10] += py * qy; v[11] += py * qz;
+    v[12] += pz * qx; v[13] += pz * qy; v[14] += pz * qz;
+  }
+  block_sum_d<15>(v, sh);
+  if (threadIdx.x == 0) {
+    const double cnt = (double)(l - f);
+    const double m[3] = {v[0] / cnt, v[1] / cnt, v[2] / cnt};
+    double A[3][3];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) A[r][c] = v[6 + r * 3 + c] - m[r] * v[3 + c];
+    double ir = (double)invRatio[b];
+    double dm[3] = {m[0], m[1], m[2]};
+    if (ir > 0.0)
+      for (int k = 0; k < 3; k++) dm[k] *= ir;
+    const double nc[3] = {cx + dm[0], cy + dm[1], cz + dm[2]};
+    double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                 A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                 A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    if (fabs(det) < 1.0e-12) {
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) A[r][c] = (r == c) ? 1.0 : 0.0;
+      det = 1.0;
+    }
+    const double s = cbrt(fabs(det));
+    if (s > 1.0e-6)
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) A[r][c] *= 1.0 / s;
+    double R[3][3];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) R[r][c] = A[r][c];
+    for (int it = 0; it < 8; it++) {
+      // R^-T = cofactor(R) / det(R): column c of R^-T is the cross product
+      // of the other two columns of R.
+      double C[3][3];
+      for (int c = 0; c < 3; c++) {
+        const int a = (c + 1) % 3, e = (c + 2) % 3;
+        C[0][c] = R[1][a] * R[2][e] - R[2][a] * R[1][e];
+        C[1][c] = R[2][a] * R[0][e] - R[0][a] * R[2][e];
+        C[2][c] = R[0][a] * R[1][e] - R[1][a] * R[0][e];
+      }
+      const double d = R[0][0] * C[0][0] + R[1][0] * C[1][0] + R[2][0] * C[2][0];
+      const double invd = fabs(d) > 1.0e-20 ? 1.0 / d : 0.0;
+      double diff = 0;
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+          const double nx = (R[r][c] + C[r][c] * invd) * 0.5;
+          diff += (nx - R[r][c]) * (nx - R[r][c]);
+          R[r][c] = nx;
+        }
+      if (diff < 1.0e-12) break;
+    }
+    centreD[b * 4] = nc[0]; centreD[b * 4 + 1] = nc[1]; centreD[b * 4 + 2] = nc[2];
+    for (int k = 0; k < 9; k++) rotD[b * 9 + k] = R[k / 3][k % 3];
+    centreF[b] = make_float4((float)nc[0], (float)nc[1], (float)nc[2], 0.0f);
+    for (int r = 0; r < 3; r++)
+      rotF[b * 3 + r] = make_float4((float)R[r][0], (float)R[r][1], (float)R[r][2], 0.0f);
+    sc[0] = (float)nc[0]; sc[1] = (float)nc[1]; sc[2] = (float)nc[2];
+    for (int k = 0; k < 9; k++) sc[3 + k] = (float)R[k / 3][k % 3];
+  }
+  __syncthreads();
+  // bodyProject (wgsl.js:763-783)
+  const float ox = sc[0], oy = sc[1], oz = sc[2];
+  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
+    const uint32_t q = bodyPids[k];
+    const float4 r = restOfPid[q];
+    const float px = sc[3] * r.x + sc[4] * r.y + sc[5] * r.z;
+    const float py = sc[6] * r.x + sc[7] * r.y + sc[8] * r.z;
+    const float pz = sc[9] * r.x + sc[10] * r.y + sc[11] * r.z;
+    pred[slotOf[q]] = make_float4(ox + px, oy + py, oz + pz, 1.0f);
+  }
+}
+
+// bodyDrag (wgsl.js:187-214) over the held body's particles.
+__global__ void __launch_bounds__(WG)
+k_body_drag(DragArgs D, float4* __restrict__ vel, const uint32_t* __restrict__ slotOf,
+            const uint32_t* __restrict__ bodyPids, const uint32_t* __restrict__ bodyFirst,
+            const float4* __restrict__ centreF) {
+  const uint32_t b = D.held - 1u;
+  const uint32_t f = bodyFirst[b], l = bodyFirst[b + 1];
+  const uint32_t k = f + blockIdx.x * WG + threadIdx.x;
+  if (k >= l) return;
+  const uint32_t i = slotOf[bodyPids[k]];
+  const float4 c = centreF[b];
+  float wx = (D.goal[0] - c.x) * D.rate, wy = (D.goal[1] - c.y) * D.rate,
+        wz = (D.goal[2] - c.z) * D.rate;
+  const float sp = sqrtf(wx * wx + wy * wy + wz * wz);
+  if (sp > D.limit) {
+    const float s = D.limit / sp;
+    wx *= s; wy *= s; wz *= s;
+  }
+  const float4 v = vel[i];
+  vel[i] = make_float4(v.x + (wx - v.x) * D.blend, v.y + (wy - v.y) * D.blend,
+                       v.z + (wz - v.z) * D.blend, v.w);
+}
+
+// impulse (wgsl.js:216-248)
+__global__ void __launch_bounds__(WG)
+k_impulse(RayArgs R, const float4* __restrict__ pos, float4* __restrict__ vel) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= R.n) return;
+  const float4 p = pos[i];
+  const float tx = p.x - R.origin[0], ty = p.y - R.origin[1], tz = p.z - R.origin[2];
+  const float along = tx * R.dir[0] + ty * R.dir[1] + tz * R.dir[2];
+  if (along <= 0.0f) return;
+  const float ax = tx - along * R.dir[0], ay = ty - along * R.dir[1], az = tz - along * R.dir[2];
+  const float d2 = ax * ax + ay * ay + az * az;
+  if (d2 >= R.radius * R.radius) return;
+  const float x = 1.0f - sqrtf(d2) / R.radius;
+  const float4 v = vel[i];
+  const float old = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+  float nx = v.x + (x * x) * R.impulse[0], ny = v.y + (x * x) * R.impulse[1],
+        nz = v.z + (x * x) * R.impulse[2];
+  const float ns = sqrtf(nx * nx + ny * ny + nz * nz);
+  const float allowed = fmaxf(old, R.speedLimit);
+  if (ns > allowed) {
+    const float s = allowed / ns;
+    nx *= s; ny *= s; nz *= s;
+  }
+  vel[i] = make_float4(nx, ny, nz, 0.0f);
+}
+
+// resize (wgsl.js:125-157)
+__global__ void __launch_bounds__(WG)
+k_resize(ResizeArgs B, float4* __restrict__ pos) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= B.n) return;
+  float4 q = pos[i];
+  float p[3] = {q.x, q.y, q.z};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (B.newSize[a] < B.oldSize[a]) {
+      const float t = fminf(fmaxf((p[a] - B.oldMin[a]) / B.oldSize[a], 0.0f), 1.0f);
+      p[a] = B.newMin[a] + t * B.newSize[a];
+    }
+    p[a] = fminf(fmaxf(p[a], B.clampMin[a]), B.clampMax[a]);
+  }
+  pos[i] = make_float4(p[0], p[1], p[2], q.w);
+}
+
+// stats (wgsl.js:785-832), reduced exactly: per-block FP64 partials, then
+// one block folds the partials in a fixed order.
+__global__ void __launch_bounds__(WG)
+k_stats_partial(Uniform U, const float* __restrict__ density, const float4* __restrict__ vel,
+                double* __restrict__ part) {
+  __shared__ double sh[WG / 64][4];
+  const uint32_t i0 = blockIdx.x * (WG * 8) + threadIdx.x;
+  double sr = 0, mr = 0, mv = 0, ke = 0;
+  for (int k = 0; k < 8; k++) {
+    const uint32_t i = i0 + k * WG;
+    if (i < U.n) {
+      const float rho = density[i] * U.invRho0;
+      const float4 v = vel[i];
+      const float v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+      sr += rho;
+      mr = fmax(mr, (double)rho);
+      mv = fmax(mv, (double)sqrtf(v2));
+      ke += (double)(0.5f * U.mass * v2);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    ke += __shfl_xor(ke, off, 64);
+    mr = fmax(mr, __shfl_xor(mr, off, 64));
+    mv = fmax(mv, __shfl_xor(mv, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sh[wave][0] = sr; sh[wave][1] = mr; sh[wave][2] = mv; sh[wave][3] = ke; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int w = 0; w < WG / 64; w++) {
+      a += sh[w][0]; b = fmax(b, sh[w][1]); c = fmax(c, sh[w][2]); d += sh[w][3];
+    }
+    part[blockIdx.x * 4 + 0] = a; part[blockIdx.x * 4 + 1] = b;
+    part[blockIdx.x * 4 + 2] = c; part[blockIdx.x * 4 + 3] = d;
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+k_stats_final(const double* __restrict__ part, int nb, uint32_t n, double* __restrict__ out) {
+  __shared__ double sh[WG / 64][4];
+  double sr = 0, mr = 0, mv = 0, ke = 0;
+  for (int b = threadIdx.x; b < nb; b += WG) {
+    sr += part[b * 4]; mr = fmax(mr, part[b * 4 + 1]);
+    mv = fmax(mv, part[b * 4 + 2]); ke += part[b * 4 + 3];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    ke += __shfl_xor(ke, off, 64);
+    mr = fmax(mr, __shfl_xor(mr, off, 64));
+    mv = fmax(mv, __shfl_xor(mv, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sh[wave][0] = sr; sh[wave][1] = mr; sh[wave][2] = mv; sh[wave][3] = ke; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int w = 0; w < WG / 64; w++) {
+      a += sh[w][0]; b = fmax(b, sh[w][1]); c = fmax(c, sh[w][2]); d += sh[w][3];
+    }
+    out[0] = a / (double)(n > 0 ? n : 1u);
+    out[1] = b; out[2] = c; out[3] = d;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline unsigned blocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
+
+template <class Op, int TX, int NT, int MAXREC>
+hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
+  const int gx = U.gridDim[0];
+  const int tpr = (gx + TX - 1) / TX;
+  const long nt = (long)tpr * U.gridDim[1] * U.gridDim[2];
+  if (nt <= 0 || U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_tile<Op, TX, NT, MAXREC>), dim3((unsigned)nt), dim3(NT), 0, s, U, op,
+                     A.cellStart, A.bcellStart, A.bpos, tpr);
+  return hipGetLastError();
+}
+
+// Tile shapes: NREC=1 kernels stage up to 3072 records (48 KiB) at TX=32,
+// NREC=2 kernels up to 1536 (48 KiB) at TX=16.  A tile whose halo holds
+// more falls back to global memory for all its particles.
+template <class Op>
+hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
+                     hipStream_t s) {
+  if (t.tx == 16) return run_tile<Op, 16, 128, 2048>(U, A, op, s);
+  return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+}
+template <class Op>
+hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
+                     hipStream_t s) {
+  if (t.tx == 32) return run_tile<Op, 32, 256, 1920>(U, A, op, s);
+  return run_tile<Op, 16, 128, 1536>(U, A, op, s);
+}
+
+}  // namespace
+
+int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
+
+hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_predict_count, dim3(blocks(U.n)), dim3(WG), 0, s, U, A.pos[par],
+                     A.vel[par], A.pred[par], A.key, A.cellCount);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s) {
+  // nCells+1 entries are written (cellStart[nCells] = total), so the blocks
+  // cover indices 0..nCells inclusive.
+  const unsigned nb = (unsigned)((U.nCells + SCAN_CH) / SCAN_CH);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, A.cellCount, U.nCells, A.blockSum);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(WG), 0, s, A.cellCount, U.nCells, A.blockSum,
+                     A.cellStart);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
+                          hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_tmp, dim3(blocks(U.n)), dim3(WG), 0, s, U.n, A.key, A.cellStart,
+                     A.cellCount, A.tmp);
+  hipLaunchKernelGGL(k_sort_move, dim3(blocks(U.n)), dim3(WG), 0, s, U.n, A.tmp, A.key,
+                     A.cellStart, A.cellCount, A.pos[par], A.pred[par], A.pid[par],
+                     A.pos[par ^ 1], A.pred[par ^ 1], A.pid[par ^ 1], A.order, A.slot,
+                     A.slotOf, moveVel ? A.vel[par] : nullptr,
+                     moveVel ? A.vel[par ^ 1] : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, bool exact,
+                         const TileCfg& t, hipStream_t s) {
+  if (exact) {
+    LambdaOp<true> op{pred, A.density, A.lambda};
+    return run_tile1(U, A, op, t, s);
+  }
+  LambdaOp<false> op{pred, A.density, A.lambda};
+  return run_tile1(U, A, op, t, s);
+}
+
+hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
+                        bool exact, const TileCfg& t, hipStream_t s) {
+  if (exact) {
+    DeltaOp<true> op{pred, A.lambda, out};
+    return run_tile1(U, A, op, t, s);
+  }
+  DeltaOp<false> op{pred, A.lambda, out};
+  return run_tile1(U, A, op, t, s);
+}
+
+hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
+                           hipStream_t s) {
+  if (U.nBodies == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_body_pre, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
+                     A.bodyFirst, A.bodyCentreD, A.bodyCentre);
+  return hipGetLastError();
+}
+
+hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, hipStream_t s) {
+  if (U.nBodies == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_body_solve, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
+                     A.bodyFirst, A.restOfPid, A.bodyInvRatio, A.bodyCentreD, A.bodyRotD,
+                     A.bodyCentre, A.bodyRot);
+  return hipGetLastError();
+}
+
+hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
+                       bool exact, bool normals, const TileCfg& t, hipStream_t s) {
+  float4* p = const_cast<float4*>(pos);
+  if (normals) {
+    if (exact) return run_tile2(U, A, XsphOp<true, true>{pred, p, A.density, A.corr, A.normal}, t, s);
+    return run_tile2(U, A, XsphOp<false, true>{pred, p, A.density, A.corr, A.normal}, t, s);
+  }
+  if (exact) return run_tile2(U, A, XsphOp<true, false>{pred, p, A.density, A.corr, A.normal}, t, s);
+  return run_tile2(U, A, XsphOp<false, false>{pred, p, A.density, A.corr, A.normal}, t, s);
+}
+
+hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
+                          float4* vel, bool exact, const TileCfg& t, hipStream_t s) {
+  if (exact) return run_tile2(U, A, TensionOp<true>{pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+  return run_tile2(U, A, TensionOp<false>{pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+}
+
+hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
+                           float4* vel, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(blocks(U.n)), dim3(WG), 0, s, U, pred, pos, A.corr, vel);
+  return hipGetLastError();
+}
+
+hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s) {
+  const int nb = stats_blocks(U.n);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_stats_partial, dim3(nb), dim3(WG), 0, s, U, A.density, vel,
+                       A.statsPart);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(WG), 0, s, A.statsPart, nb, U.n, A.statsOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,
+                            hipStream_t s) {
+  if (U.nBodies == 0 || d.held < 1 || d.held > U.nBodies) return hipSuccess;
+  // the grid covers the largest body; extra threads exit
+  hipLaunchKernelGGL(k_body_drag, dim3(blocks(U.nBodyParts)), dim3(WG), 0, s, d, vel,
+                     A.slotOf, A.bodyPids, A.bodyFirst, A.bodyCentre);
+  return hipGetLastError();
+}
+
+hipError_t launch_impulse(const float4* pos, float4* vel, const RayArgs& r, hipStream_t s) {
+  if (r.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_impulse, dim3(blocks(r.n)), dim3(WG), 0, s, r, pos, vel);
+  return hipGetLastError();
+}
+
+hipError_t launch_resize(const ResizeArgs& r, float4* pos, hipStream_t s) {
+  if (r.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_resize, dim3(blocks(r.n)), dim3(WG), 0, s, r, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_census(const Uniform& U, const Arrays& A, const float4* pred,
+                         const TileCfg& t, hipStream_t s) {
+  CensusOp op{pred, A.census[0], A.census[1], A.census[2]};
+  return run_tile1(U, A, op, t, s);
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n)), dim3(WG), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace p4a

@@ -1,0 +1,134 @@
+// Internal definitions shared by the host orchestration (sim.hip) and the
+// gfx950 kernels (kernels.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace p4a {
+
+// The reference's `Params` uniform (wgsl.js:4-40) with the values
+// Sim.uploadParams derives (sim.js:371-413).  Passed to every kernel by
+// value, so a captured graph carries its own copy.
+struct Uniform {
+  float boxMin[3];
+  float dt;
+  float boxMax[3];
+  float h;
+  int gridDim[3];
+  float h2;
+  float clampMin[3];
+  float poly6;
+  float clampMax[3];
+  float spikyGrad;
+  float gravity, mass, rho0, invRho0;
+  float cfmEps, sCorrK, sCorrWq, xsphC;
+  uint32_t n, nCells, nBoundary, nBodyParts, nBodies;
+  float bodyScale, covScale, omega;
+  uint32_t sorAverage;
+  float invDt, volume, tension, cohes, cohesTerm;
+};
+
+// Device arrays of one scene.  Particle arrays are in the live
+// (cell-sorted) order; `pid` names each slot's particle.
+struct Arrays {
+  float4* pos[2];       // live positions, ping-pong across the grid build
+  float4* pred[2];      // predict output (pre-sort) / sorted copy
+  float4* predX;        // solver ping-pong partner
+  float4* vel[2];       // velocities, same parity as pos
+  uint32_t* pid[2];     // particle id per slot
+  uint32_t* key;        // cell id per pre-sort index
+  uint32_t* tmp;        // unstable scatter target (pre-sort index per slot)
+  uint32_t* order;      // pre-sort index per sorted slot
+  uint32_t* slot;       // sorted slot per pre-sort index
+  uint32_t* slotOf;     // live slot per particle id
+  float* density;
+  float* lambda;
+  float4* corr;
+  float4* normal;
+  uint32_t* cellCount;  // nCells+1, zero between grid builds
+  uint32_t* cellStart;  // nCells+2
+  uint32_t* blockSum;
+  float4* bpos;         // sorted wall samples, w = psi
+  float* bpsi;
+  uint32_t* bcellStart; // nCells+2
+  // bodies (indexed by particle id or body)
+  uint32_t* bodyOfPid;  // 1-based body id per particle id, 0 = fluid
+  float4* restOfPid;    // body-local rest offset per particle id
+  uint32_t* bodyPids;   // particle ids of body particles, grouped by body
+  uint32_t* bodyFirst;  // nBodies+1 offsets into bodyPids
+  double* bodyCentreD;  // 4 per body
+  double* bodyRotD;     // 9 per body, row-major R
+  float* bodyInvRatio;  // 1/max(1e-6, density) as the reference's f32 info.x
+  float4* bodyCentre;   // reference layout (f32), bodyCentre buffer
+  float4* bodyRot;      // reference layout: 3 rows per body
+  // stats
+  double* statsPart;    // 4 per block
+  double* statsOut;     // 4
+  // census
+  uint32_t* census[3];
+};
+
+// Launch configuration of the tiled neighbour kernels.
+struct TileCfg {
+  int tx;       // cells per tile along x
+  int threads;  // threads per workgroup
+};
+
+// Drag of a held body (wgsl.js:187-214).
+struct DragArgs {
+  float goal[3];
+  float rate;
+  uint32_t held;   // 1-based body id
+  float blend;
+  float limit;
+};
+
+// Ray impulse (wgsl.js:216-248).
+struct RayArgs {
+  float origin[3];
+  float radius;
+  float dir[3];
+  float speedLimit;
+  float impulse[3];
+  uint32_t n;
+};
+
+// Box-wall move (wgsl.js:125-157).
+struct ResizeArgs {
+  float oldMin[3], oldSize[3], newMin[3], newSize[3], clampMin[3], clampMax[3];
+  uint32_t n;
+};
+
+// Kernel launchers (kernels.hip).  All enqueue on `s` and return the
+// hipError_t of the launch.
+hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s);
+hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s);
+hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
+                          hipStream_t s);
+hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
+                         bool exact, const TileCfg& t, hipStream_t s);
+hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
+                        bool exact, const TileCfg& t, hipStream_t s);
+hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
+                           hipStream_t s);
+hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred,
+                             hipStream_t s);
+hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
+                       bool exact, bool normals, const TileCfg& t, hipStream_t s);
+hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
+                          float4* vel, bool exact, const TileCfg& t, hipStream_t s);
+hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
+                           float4* vel, hipStream_t s);
+hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s);
+hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,
+                            hipStream_t s);
+hipError_t launch_impulse(const float4* pos, float4* vel, const RayArgs& r, hipStream_t s);
+hipError_t launch_resize(const ResizeArgs& r, float4* pos, hipStream_t s);
+hipError_t launch_census(const Uniform& U, const Arrays& A, const float4* pred,
+                         const TileCfg& t, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+int stats_blocks(uint32_t n);
+
+}  // namespace p4a

@@ -1,0 +1,1064 @@
+// Host orchestration of libp4a: the reference's `class Sim` (src/sim.js)
+// over HIP.  One handle = one scene on one GPU, with its own stream.
+//
+// Substep (sim.js:458-596), as launched here:
+//   [bodyDrag] predict+count, scan, scatter (stable), ->parity flip,
+//   [body centroids], I x (lambda, delta, [shape matching]),
+//   xsph(+normals), tension+finalize (or finalize)
+// The whole substep is captured once per parity as a hipGraph and
+// replayed; the graph is rebuilt when anything it bakes in changes
+// (particle count, grid, parameters, iteration count).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/p4a.h"
+#include "p4a_internal.h"
+
+using namespace p4a;
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+struct TimedLaunch {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct p4a_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool ready = false;
+  p4a_params params{};
+  // scene
+  double h = 0, d = 0, mass = 0, denomRest = 0;
+  int gridDim[3] = {1, 1, 1};
+  int nCells = 1;
+  int n = 0, cap = 0, nFluid = 0, nBodyParts = 0, nBodies = 0, nBoundary = 0, pourBudget = 0;
+  double bodyScale = 0, covScale = 0;
+  // state
+  int parity = 0, gen = 0, lastSubsteps = 0;
+  double simTime = 0, timeBank = 0, lastAdvanced = 0;
+  float4* predCur = nullptr;
+  Arrays A{};
+  std::vector<void*> devAllocs;       // particle/body/stat arrays
+  std::vector<void*> cellAllocs;      // grid arrays (depend on nCells)
+  std::vector<void*> bndAllocs;       // wall arrays
+  size_t bposCount = 0;
+  // host mirrors for reads in the reference layout
+  std::vector<uint32_t> bodyOfPidH;
+  std::vector<float> restOfPidH;
+  std::vector<int32_t> bodyCountH;
+  // interaction
+  int heldBody = -1;
+  double heldTarget[3] = {0, 0, 0};
+  double heldRate = 10, heldLimit = 4;
+  // graphs
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  std::string graphKey[2];
+  // stats
+  double* statsHost = nullptr;
+  bool statsValid = false;
+  // timing
+  bool timing = false;
+  std::vector<TimedLaunch> timed;
+  std::vector<hipEvent_t> eventPool;
+  size_t eventNext = 0;
+  // tile shapes
+  TileCfg tile1{32, 256}, tile2{16, 128};
+};
+
+namespace {
+
+#define HIP_TRY(sim, expr)                                                      \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      (sim)->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+      return P4A_ERR_HIP;                                                       \
+    }                                                                           \
+  } while (0)
+
+int fail(p4a_sim* s, int code, const std::string& m) {
+  if (s) s->err = m;
+  return code;
+}
+
+template <class T>
+int dalloc(p4a_sim* s, std::vector<void*>& owner, T** p, size_t count) {
+  size_t bytes = std::max<size_t>(16, count * sizeof(T));
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    s->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+    return P4A_ERR_NOMEM;
+  }
+  e = hipMemsetAsync(q, 0, bytes, s->stream);
+  if (e != hipSuccess) {
+    hipFree(q);
+    s->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
+    return P4A_ERR_HIP;
+  }
+  owner.push_back(q);
+  *p = (T*)q;
+  return P4A_OK;
+}
+
+void free_list(std::vector<void*>& v) {
+  for (void* q : v) hipFree(q);
+  v.clear();
+}
+
+void drop_graphs(p4a_sim* s) {
+  for (int k = 0; k < 2; k++) {
+    if (s->graph[k]) hipGraphExecDestroy(s->graph[k]);
+    s->graph[k] = nullptr;
+    s->graphKey[k].clear();
+  }
+}
+
+// Sim.uploadParams (sim.js:371-413): FP64 host arithmetic, then f32.
+Uniform make_uniform(const p4a_sim* s, double dt) {
+  const p4a_params& p = s->params;
+  const double h = s->h, d = p.spacing;
+  Uniform U;
+  std::memset(&U, 0, sizeof(U));
+  for (int k = 0; k < 3; k++) {
+    U.boxMin[k] = 0.0f;
+    U.boxMax[k] = (float)p.box[k];
+    U.gridDim[k] = s->gridDim[k];
+    U.clampMin[k] = (float)(0.5 * d);
+    U.clampMax[k] = (float)(p.box[k] - 0.5 * d);
+  }
+  U.dt = (float)dt;
+  U.h = (float)h;
+  U.h2 = (float)(h * h);
+  const double poly6 = 315.0 / (64.0 * kPi * std::pow(h, 9));
+  U.poly6 = (float)poly6;
+  U.spikyGrad = (float)(-45.0 / (kPi * std::pow(h, 6)));
+  U.gravity = (float)p.gravity;
+  U.mass = (float)s->mass;
+  U.rho0 = (float)p.rest_density;
+  U.invRho0 = (float)(1.0 / p.rest_density);
+  U.cfmEps = (float)std::max(1e-9, p.cfm_epsilon_rel * s->denomRest);
+  U.sCorrK = (float)(p.scorr_k / (s->denomRest * std::max(1, p.iterations)));
+  const double rq = p.scorr_dq * h;
+  const double tq = h * h - rq * rq;
+  const double wq = poly6 * tq * tq * tq;
+  U.sCorrWq = (float)(wq > 0 ? 1.0 / wq : 0.0);
+  U.xsphC = (float)p.xsph_c;
+  U.n = (uint32_t)s->n;
+  U.nCells = (uint32_t)s->nCells;
+  U.nBoundary = p.no_boundary ? 0u : (uint32_t)s->nBoundary;
+  U.nBodyParts = (uint32_t)s->nBodyParts;
+  U.nBodies = (uint32_t)s->nBodies;
+  U.bodyScale = (float)s->bodyScale;
+  U.covScale = (float)s->covScale;
+  U.omega = (float)p.omega;
+  U.sorAverage = p.sor_average ? 1u : 0u;
+  U.invDt = (float)(dt > 0 ? 1.0 / dt : 0.0);
+  U.volume = (float)(s->mass / p.rest_density);
+  U.tension = (float)p.surface_tension_k;
+  U.cohes = (float)(32.0 / (kPi * std::pow(h, 9)));
+  U.cohesTerm = (float)(std::pow(h, 6) / 64.0);
+  return U;
+}
+
+hipEvent_t take_event(p4a_sim* s) {
+  if (s->eventNext == s->eventPool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    s->eventPool.push_back(e);
+  }
+  return s->eventPool[s->eventNext++];
+}
+
+// Launch one kernel group, bracketed by events when timing is on.
+template <class F>
+int launch(p4a_sim* s, const char* name, F&& f) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (s->timing) {
+    a = take_event(s);
+    b = take_event(s);
+    if (a) hipEventRecord(a, s->stream);
+  }
+  hipError_t e = f();
+  if (e != hipSuccess) {
+    s->err = std::string(name) + ": " + hipGetErrorString(e);
+    return P4A_ERR_HIP;
+  }
+  if (s->timing && a && b) {
+    hipEventRecord(b, s->stream);
+    s->timed.push_back({name, a, b});
+  }
+  return P4A_OK;
+}
+
+#define LAUNCH(s, name, call)                          \
+  do {                                                 \
+    int rc_ = launch((s), (name), [&]() { return (call); }); \
+    if (rc_) return rc_;                               \
+  } while (0)
+
+bool exact_math(const p4a_sim* s) { return s->params.math == P4A_MATH_EXACT; }
+
+float4* other_pred(p4a_sim* s, int par) {
+  return s->predCur == s->A.pred[par] ? s->A.predX : s->A.pred[par];
+}
+
+// Grid build on the current parity: count is done by predict; scan and
+// the stable scatter follow, then the parity flips (sim.js:475-492).
+int run_grid(p4a_sim* s, const Uniform& U, bool moveVel) {
+  const int par = s->parity;
+  LAUNCH(s, "scan", launch_scan(U, s->A, s->stream));
+  LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, s->stream));
+  s->parity ^= 1;
+  s->predCur = s->A.pred[s->parity];
+  return P4A_OK;
+}
+
+int run_finish(p4a_sim* s, const Uniform& U) {
+  const int par = s->parity;
+  const bool ex = exact_math(s);
+  const bool tension = s->params.surface_tension_k > 0;
+  LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, s->stream));
+  if (tension)
+    LAUNCH(s, "tension", launch_tension(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par], ex,
+                                        s->tile2, s->stream));
+  else
+    LAUNCH(s, "finalize", launch_finalize(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par],
+                                          s->stream));
+  return P4A_OK;
+}
+
+// One substep after the optional drag: predict .. finalize.
+int run_substep_body(p4a_sim* s, const Uniform& U) {
+  LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
+  int rc = run_grid(s, U, false);
+  if (rc) return rc;
+  const int par = s->parity;
+  const bool ex = exact_math(s);
+  if (s->nBodies > 0) LAUNCH(s, "body_pre", launch_body_pre(U, s->A, s->predCur, s->stream));
+  const int iters = std::max(1, s->params.iterations);
+  for (int it = 0; it < iters; it++) {
+    LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, ex, s->tile1, s->stream));
+    float4* out = other_pred(s, par);
+    LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, ex, s->tile1, s->stream));
+    s->predCur = out;
+    if (s->nBodies > 0) LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur, s->stream));
+  }
+  return run_finish(s, U);
+}
+
+std::string graph_key(const p4a_sim* s, const Uniform& U) {
+  std::string k((const char*)&U, sizeof(U));
+  k += std::to_string(s->params.iterations) + "/" + std::to_string(s->params.math) + "/" +
+       std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.tx) +
+       "/" + std::to_string(s->tile2.tx);
+  return k;
+}
+
+int run_substep(p4a_sim* s, double dt) {
+  const Uniform U = make_uniform(s, dt);
+  if (s->heldBody >= 1 && s->nBodyParts > 0) {
+    DragArgs D;
+    for (int k = 0; k < 3; k++) D.goal[k] = (float)s->heldTarget[k];
+    D.rate = (float)s->heldRate;
+    D.held = (uint32_t)s->heldBody;
+    D.blend = (float)(1.0 - std::exp(-std::max(dt, 0.0) / 0.02));
+    D.limit = (float)s->heldLimit;
+    LAUNCH(s, "body_drag", launch_body_drag(U, s->A, s->A.vel[s->parity], D, s->stream));
+  }
+  const int par0 = s->parity;
+  if (!s->params.use_graph || s->timing) {
+    int rc = run_substep_body(s, U);
+    if (rc) return rc;
+  } else {
+    const std::string key = graph_key(s, U);
+    if (!s->graph[par0] || s->graphKey[par0] != key) {
+      if (s->graph[par0]) {
+        hipGraphExecDestroy(s->graph[par0]);
+        s->graph[par0] = nullptr;
+      }
+      HIP_TRY(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      int rc = run_substep_body(s, U);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(s, P4A_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&s->graph[par0], g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(s, P4A_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      s->graphKey[par0] = key;
+      // capture only recorded the work; the host-side state advanced as if
+      // it ran, so rewind it and replay the graph below.
+      s->parity = par0;
+    }
+    HIP_TRY(s, hipGraphLaunch(s->graph[par0], s->stream));
+    // host-side bookkeeping of what the graph does
+    s->parity = par0 ^ 1;
+    const int iters = std::max(1, s->params.iterations);
+    s->predCur = (iters % 2 == 0) ? s->A.pred[s->parity] : s->A.predX;
+  }
+  s->simTime += dt;
+  return P4A_OK;
+}
+
+int run_stats(p4a_sim* s) {
+  const Uniform U = make_uniform(s, 0.0);
+  LAUNCH(s, "stats", launch_stats(U, s->A, s->A.vel[s->parity], s->stream));
+  HIP_TRY(s, hipMemcpyAsync(s->statsHost, s->A.statsOut, 4 * sizeof(double),
+                            hipMemcpyDeviceToHost, s->stream));
+  s->statsValid = true;
+  return P4A_OK;
+}
+
+// sortBoundary (sim.js:255-286): stable counting sort of the wall samples
+// into the simulation grid, in host FP64 arithmetic on the float samples.
+void sort_boundary(const p4a_sim* s, const float* pts, const float* psi, int nb,
+                   std::vector<float>& sorted, std::vector<float>& sortedPsi,
+                   std::vector<uint32_t>& start) {
+  const int* dim = s->gridDim;
+  const double h = s->h;
+  const int nCells = s->nCells;
+  std::vector<uint32_t> counts(nCells + 2, 0), cell(std::max(nb, 1));
+  for (int i = 0; i < nb; i++) {
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+      double f = std::floor((double)pts[i * 3 + k] / h);
+      c[k] = (int)std::min((double)dim[k] - 1, std::max(0.0, f));
+    }
+    cell[i] = (uint32_t)((c[2] * dim[1] + c[1]) * dim[0] + c[0]);
+    counts[cell[i]]++;
+  }
+  start.assign(nCells + 2, 0);
+  uint32_t run = 0;
+  for (int c = 0; c <= nCells; c++) {
+    start[c] = run;
+    run += counts[c];
+  }
+  start[nCells + 1] = run;
+  std::vector<uint32_t> cursor(start);
+  sorted.assign((size_t)std::max(1, nb) * 4, 0.0f);
+  sortedPsi.assign((size_t)std::max(1, nb), 0.0f);
+  for (int i = 0; i < nb; i++) {
+    const uint32_t k = cursor[cell[i]]++;
+    sorted[k * 4 + 0] = pts[i * 3 + 0];
+    sorted[k * 4 + 1] = pts[i * 3 + 1];
+    sorted[k * 4 + 2] = pts[i * 3 + 2];
+    sorted[k * 4 + 3] = psi[i];
+    sortedPsi[k] = psi[i];
+  }
+}
+
+int set_grid(p4a_sim* s) {
+  for (int k = 0; k < 3; k++)
+    s->gridDim[k] = std::max(1, (int)std::floor(s->params.box[k] / s->h));
+  const long cells = (long)s->gridDim[0] * s->gridDim[1] * s->gridDim[2];
+  if (cells > (long)INT32_MAX - 4) return fail(s, P4A_ERR_RANGE, "grid too large");
+  s->nCells = (int)cells;
+  return P4A_OK;
+}
+
+int alloc_cells(p4a_sim* s) {
+  free_list(s->cellAllocs);
+  const size_t nc = (size_t)s->nCells;
+  int rc;
+  if ((rc = dalloc(s, s->cellAllocs, &s->A.cellCount, nc + 1))) return rc;
+  if ((rc = dalloc(s, s->cellAllocs, &s->A.cellStart, nc + 2))) return rc;
+  if ((rc = dalloc(s, s->cellAllocs, &s->A.blockSum, nc / 2048 + 2))) return rc;
+  if ((rc = dalloc(s, s->cellAllocs, &s->A.bcellStart, nc + 2))) return rc;
+  return P4A_OK;
+}
+
+int upload_boundary(p4a_sim* s, const float* pts, const float* psi, int nb) {
+  std::vector<float> sorted, sortedPsi;
+  std::vector<uint32_t> start;
+  sort_boundary(s, pts, psi, nb, sorted, sortedPsi, start);
+  free_list(s->bndAllocs);
+  int rc;
+  if ((rc = dalloc(s, s->bndAllocs, &s->A.bpos, (size_t)std::max(1, nb)))) return rc;
+  if ((rc = dalloc(s, s->bndAllocs, &s->A.bpsi, (size_t)std::max(1, nb)))) return rc;
+  s->bposCount = (size_t)nb;
+  s->nBoundary = nb;
+  HIP_TRY(s, hipMemcpyAsync(s->A.bpos, sorted.data(), sorted.size() * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(s->A.bpsi, sortedPsi.data(), sortedPsi.size() * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(s->A.bcellStart, start.data(), start.size() * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return P4A_OK;
+}
+
+struct BufView {
+  void* ptr;
+  size_t bytes;
+};
+
+BufView buf_view(p4a_sim* s, int id) {
+  const size_t n = (size_t)s->n, cap = (size_t)s->cap, nc = (size_t)s->nCells;
+  const int par = s->parity;
+  (void)cap;
+  switch (id) {
+    case P4A_BUF_POS: return {s->A.pos[par], n * 16};
+    case P4A_BUF_VEL: return {s->A.vel[par], n * 16};
+    case P4A_BUF_PRED: return {s->predCur, n * 16};
+    case P4A_BUF_LAMBDA: return {s->A.lambda, n * 4};
+    case P4A_BUF_DENSITY: return {s->A.density, n * 4};
+    case P4A_BUF_SLOT: return {s->A.slot, n * 4};
+    case P4A_BUF_CORR: return {s->A.corr, n * 16};
+    case P4A_BUF_NORMAL: return {s->A.normal, n * 16};
+    case P4A_BUF_CELL_START: return {s->A.cellStart, (nc + 2) * 4};
+    case P4A_BUF_BPOS: return {s->A.bpos, s->bposCount * 16};
+    case P4A_BUF_BCELL_START: return {s->A.bcellStart, (nc + 2) * 4};
+    case P4A_BUF_BODY_CENTRE: return {s->A.bodyCentre, (size_t)s->nBodies * 16};
+    case P4A_BUF_BODY_ROT: return {s->A.bodyRot, (size_t)s->nBodies * 48};
+    case P4A_BUF_PID: return {s->A.pid[par], n * 4};
+    case P4A_BUF_KEY: return {s->A.key, n * 4};
+    case P4A_BUF_ORDER: return {s->A.order, n * 4};
+    case P4A_BUF_BPSI: return {s->A.bpsi, s->bposCount * 4};
+    default: return {nullptr, 0};
+  }
+}
+
+int check(p4a_sim* s) {
+  if (!s) return P4A_ERR_ARG;
+  if (!s->ready) return fail(s, P4A_ERR_STATE, "reset() has not been called");
+  return P4A_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+P4A_API const char* p4a_version(void) { return "p4a 0.1 (gfx950)"; }
+
+P4A_API int p4a_create(int device, p4a_sim** out) {
+  if (!out) return P4A_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return P4A_ERR_NODEV;
+  p4a_sim* s = new (std::nothrow) p4a_sim();
+  if (!s) return P4A_ERR_NOMEM;
+  s->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&s->stream) != hipSuccess) {
+    delete s;
+    return P4A_ERR_HIP;
+  }
+  if (hipHostMalloc((void**)&s->statsHost, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(s->stream);
+    delete s;
+    return P4A_ERR_NOMEM;
+  }
+  for (int k = 0; k < 4; k++) s->statsHost[k] = 0;
+  s->statsHost[0] = 1;
+  s->statsHost[1] = 1;
+  p4a_params_default(&s->params);
+  if (const char* e = std::getenv("P4A_TILE1")) s->tile1.tx = std::atoi(e) == 16 ? 16 : 32;
+  if (const char* e = std::getenv("P4A_TILE2")) s->tile2.tx = std::atoi(e) == 32 ? 32 : 16;
+  *out = s;
+  return P4A_OK;
+}
+
+P4A_API int p4a_destroy(p4a_sim* s) {
+  if (!s) return P4A_ERR_ARG;
+  hipSetDevice(s->device);
+  hipStreamSynchronize(s->stream);
+  drop_graphs(s);
+  free_list(s->devAllocs);
+  free_list(s->cellAllocs);
+  free_list(s->bndAllocs);
+  for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
+  if (s->statsHost) hipHostFree(s->statsHost);
+  hipStreamDestroy(s->stream);
+  delete s;
+  return P4A_OK;
+}
+
+P4A_API const char* p4a_last_error(const p4a_sim* s) {
+  return s ? s->err.c_str() : "null handle";
+}
+
+P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
+  if (!s || !p || !sc) return P4A_ERR_ARG;
+  if (sc->n < 0 || (sc->n > 0 && (!sc->pos || !sc->vel || !sc->rest || !sc->phase)))
+    return fail(s, P4A_ERR_ARG, "scene arrays missing");
+  if (sc->n_boundary < 0 || (sc->n_boundary > 0 && (!sc->boundary_pts || !sc->boundary_psi)))
+    return fail(s, P4A_ERR_ARG, "boundary arrays missing");
+  if (sc->n_bodies < 0 || (sc->n_bodies > 0 && (!sc->body_centre || !sc->body_density)))
+    return fail(s, P4A_ERR_ARG, "body arrays missing");
+  if (!(sc->h > 0) || !(p->spacing > 0)) return fail(s, P4A_ERR_ARG, "h and spacing must be > 0");
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  drop_graphs(s);
+  free_list(s->devAllocs);
+  free_list(s->cellAllocs);
+  free_list(s->bndAllocs);
+  s->ready = false;
+  s->params = *p;
+  s->h = sc->h;
+  s->d = sc->d;
+  s->mass = sc->mass;
+  s->denomRest = sc->denom_rest;
+  s->simTime = 0;
+  s->gen += 1;
+  int rc = set_grid(s);
+  if (rc) return rc;
+
+  const int n = sc->n;
+  s->n = n;
+  s->nFluid = sc->n_fluid;
+  s->pourBudget = p->pour_budget < 0 ? sc->n_fluid : p->pour_budget;
+  const long capL = (long)n + s->pourBudget;
+  if (capL > (long)INT32_MAX / 16) return fail(s, P4A_ERR_RANGE, "capacity too large");
+  s->cap = (int)capL;
+  const size_t cap = (size_t)std::max(1, s->cap);
+
+  Arrays& A = s->A;
+  std::memset(&A, 0, sizeof(A));
+#define ALLOC(ptr, count) if ((rc = dalloc(s, s->devAllocs, &(ptr), (count)))) return rc
+  for (int k = 0; k < 2; k++) {
+    ALLOC(A.pos[k], cap);
+    ALLOC(A.pred[k], cap);
+    ALLOC(A.vel[k], cap);
+    ALLOC(A.pid[k], cap);
+  }
+  ALLOC(A.predX, cap);
+  ALLOC(A.key, cap);
+  ALLOC(A.tmp, cap);
+  ALLOC(A.order, cap);
+  ALLOC(A.slot, cap);
+  ALLOC(A.slotOf, cap);
+  ALLOC(A.density, cap);
+  ALLOC(A.lambda, cap);
+  ALLOC(A.corr, cap);
+  ALLOC(A.normal, cap);
+  ALLOC(A.bodyOfPid, cap);
+  ALLOC(A.restOfPid, cap);
+  for (int k = 0; k < 3; k++) ALLOC(A.census[k], cap);
+  ALLOC(A.statsPart, (size_t)stats_blocks((uint32_t)cap) * 4 + 4);
+  ALLOC(A.statsOut, 4);
+
+  // Bodies: particle ids grouped by body, in ascending id order.
+  const int nB = sc->n_bodies;
+  s->nBodies = nB;
+  s->bodyOfPidH.assign(cap, 0u);
+  s->restOfPidH.assign(cap * 4, 0.0f);
+  std::vector<std::vector<uint32_t>> members(nB);
+  int nBodyParts = 0;
+  for (int i = 0; i < n; i++) {
+    const uint32_t b = sc->phase[(size_t)i * 4];
+    if (b == 0) continue;
+    if ((int)b > nB) return fail(s, P4A_ERR_ARG, "phase names a body that does not exist");
+    members[b - 1].push_back((uint32_t)i);
+    s->bodyOfPidH[i] = b;
+    for (int k = 0; k < 3; k++) s->restOfPidH[(size_t)i * 4 + k] = sc->rest[(size_t)i * 4 + k];
+    nBodyParts++;
+  }
+  s->nBodyParts = nBodyParts;
+  std::vector<uint32_t> pids, first(nB + 1, 0);
+  s->bodyCountH.assign(nB, 0);
+  for (int b = 0; b < nB; b++) {
+    first[b] = (uint32_t)pids.size();
+    pids.insert(pids.end(), members[b].begin(), members[b].end());
+    s->bodyCountH[b] = (int32_t)members[b].size();
+  }
+  first[nB] = (uint32_t)pids.size();
+  ALLOC(A.bodyPids, std::max<size_t>(1, pids.size()));
+  ALLOC(A.bodyFirst, (size_t)nB + 1);
+  ALLOC(A.bodyCentreD, (size_t)std::max(1, nB) * 4);
+  ALLOC(A.bodyRotD, (size_t)std::max(1, nB) * 9);
+  ALLOC(A.bodyInvRatio, (size_t)std::max(1, nB));
+  ALLOC(A.bodyCentre, (size_t)std::max(1, nB));
+  ALLOC(A.bodyRot, (size_t)std::max(1, nB) * 3);
+#undef ALLOC
+  // Fixed-point scales of the reference (sim.js:163-171), reported only:
+  // the body sums here are FP64.
+  {
+    double maxCount = 1, maxRest = 1e-3;
+    for (int b = 0; b < nB; b++) maxCount = std::max(maxCount, (double)s->bodyCountH[b]);
+    for (int i = 0; i < n; i++) {
+      const double r = std::hypot(std::hypot((double)sc->rest[i * 4], (double)sc->rest[i * 4 + 1]),
+                                  (double)sc->rest[i * 4 + 2]);
+      if (r > maxRest) maxRest = r;
+    }
+    const double extent = std::max(p->box[0], std::max(p->box[1], p->box[2]));
+    s->bodyScale = 1e9 / (extent * maxCount);
+    s->covScale = 1e9 / (extent * maxRest * maxCount);
+  }
+  if (nB > 0) {
+    std::vector<double> cD((size_t)nB * 4, 0.0), rD((size_t)nB * 9, 0.0);
+    std::vector<float> cF((size_t)nB * 4, 0.0f), rF((size_t)nB * 12, 0.0f), inv(nB);
+    for (int b = 0; b < nB; b++) {
+      for (int k = 0; k < 3; k++) {
+        cD[b * 4 + k] = sc->body_centre[b * 3 + k];
+        cF[b * 4 + k] = (float)sc->body_centre[b * 3 + k];
+        rD[b * 9 + k * 4] = 1.0;
+        rF[b * 12 + k * 4 + k] = 1.0f;
+      }
+      inv[b] = (float)(1.0 / std::max(1e-6, sc->body_density[b]));
+    }
+    HIP_TRY(s, hipMemcpyAsync(A.bodyPids, pids.data(), pids.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyFirst, first.data(), first.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyCentreD, cD.data(), cD.size() * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyRotD, rD.data(), rD.size() * 8, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyCentre, cF.data(), cF.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyRot, rF.data(), rF.size() * 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(A.bodyInvRatio, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s->stream));
+  }
+  HIP_TRY(s, hipMemcpyAsync(A.bodyOfPid, s->bodyOfPidH.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(A.restOfPid, s->restOfPidH.data(), (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+
+  // Particle state (sim.js:191-195).
+  std::vector<uint32_t> iota(cap);
+  for (size_t i = 0; i < cap; i++) iota[i] = (uint32_t)i;
+  HIP_TRY(s, hipMemcpyAsync(A.pos[0], sc->pos, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(A.vel[0], sc->vel, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(A.pred[0], sc->pos, (size_t)n * 16, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(A.pid[0], iota.data(), cap * 4, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(s, hipMemcpyAsync(A.slotOf, iota.data(), cap * 4, hipMemcpyHostToDevice, s->stream));
+
+  if ((rc = alloc_cells(s))) return rc;
+  if ((rc = upload_boundary(s, sc->boundary_pts, sc->boundary_psi, sc->n_boundary))) return rc;
+
+  s->parity = 0;
+  s->predCur = A.pred[0];
+  s->heldBody = -1;
+  s->heldTarget[0] = s->heldTarget[1] = s->heldTarget[2] = 0;
+  s->heldRate = 10;
+  s->heldLimit = 4;
+  s->statsValid = false;
+  s->ready = true;
+
+  // primeGrid (sim.js:223-253): predict with dt = 0 and one grid build.
+  const Uniform U0 = make_uniform(s, 0.0);
+  const bool timing = s->timing;
+  s->timing = false;
+  rc = launch(s, "predict", [&]() { return launch_predict_count(U0, s->A, s->parity, s->stream); });
+  if (!rc) rc = run_grid(s, U0, true);
+  s->timing = timing;
+  if (rc) return rc;
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_set_params(p4a_sim* s, const p4a_params* p) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!p) return P4A_ERR_ARG;
+  p4a_params& q = s->params;
+  q.substeps = p->substeps;
+  q.iterations = p->iterations;
+  q.cfm_epsilon_rel = p->cfm_epsilon_rel;
+  q.scorr_k = p->scorr_k;
+  q.scorr_dq = p->scorr_dq;
+  q.xsph_c = p->xsph_c;
+  q.omega = p->omega;
+  q.sor_average = p->sor_average;
+  q.surface_tension_k = p->surface_tension_k;
+  q.gravity = p->gravity;
+  q.no_boundary = p->no_boundary;
+  q.math = p->math;
+  q.precision = p->precision;
+  q.use_graph = p->use_graph;
+  return P4A_OK;
+}
+
+P4A_API int p4a_substeps(p4a_sim* s, double dt, int32_t count) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (count < 0) return P4A_ERR_ARG;
+  HIP_TRY(s, hipSetDevice(s->device));
+  if (s->timing) {
+    s->timed.clear();
+    s->eventNext = 0;
+  }
+  for (int k = 0; k < count; k++)
+    if ((rc = run_substep(s, dt))) return rc;
+  if (count > 0) return run_stats(s);
+  return P4A_OK;
+}
+
+P4A_API int p4a_step(p4a_sim* s, double frame_dt, int32_t* substeps_run) {
+  int rc = check(s);
+  if (rc) return rc;
+  // time bank (sim.js:415-429)
+  const int base = std::max(1, s->params.substeps);
+  const double dtTarget = (1.0 / 60.0) / base;
+  s->timeBank += frame_dt;
+  int sub = (int)std::floor(s->timeBank / dtTarget + 1e-4);
+  const int workCap = base * 8;
+  if (sub > workCap) {
+    sub = workCap;
+    s->timeBank = 0;
+  } else {
+    s->timeBank -= sub * dtTarget;
+  }
+  s->lastSubsteps = sub;
+  s->lastAdvanced = sub * dtTarget;
+  if (substeps_run) *substeps_run = std::max(sub, 0);
+  if (sub < 1) {
+    s->lastAdvanced = 0;
+    return P4A_OK;
+  }
+  return p4a_substeps(s, dtTarget, sub);
+}
+
+P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
+  int rc = check(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipSetDevice(s->device));
+  const Uniform U = make_uniform(s, dt);
+  const bool ex = exact_math(s);
+  switch (stage) {
+    case P4A_STAGE_PREDICT:
+      if (s->heldBody >= 1 && s->nBodyParts > 0) {
+        DragArgs D;
+        for (int k = 0; k < 3; k++) D.goal[k] = (float)s->heldTarget[k];
+        D.rate = (float)s->heldRate;
+        D.held = (uint32_t)s->heldBody;
+        D.blend = (float)(1.0 - std::exp(-std::max(dt, 0.0) / 0.02));
+        D.limit = (float)s->heldLimit;
+        LAUNCH(s, "body_drag", launch_body_drag(U, s->A, s->A.vel[s->parity], D, s->stream));
+      }
+      LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
+      break;
+    case P4A_STAGE_GRID:
+      if ((rc = run_grid(s, U, true))) return rc;
+      break;
+    case P4A_STAGE_BODY_PRE:
+      LAUNCH(s, "body_pre", launch_body_pre(U, s->A, s->predCur, s->stream));
+      break;
+    case P4A_STAGE_LAMBDA:
+      LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, ex, s->tile1, s->stream));
+      break;
+    case P4A_STAGE_DELTA: {
+      float4* out = other_pred(s, s->parity);
+      LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, ex, s->tile1, s->stream));
+      s->predCur = out;
+      break;
+    }
+    case P4A_STAGE_BODY_SOLVE:
+      LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur, s->stream));
+      break;
+    case P4A_STAGE_FINISH:
+      if ((rc = run_finish(s, U))) return rc;
+      s->simTime += dt;
+      break;
+    case P4A_STAGE_STATS:
+      if ((rc = run_stats(s))) return rc;
+      break;
+    default:
+      return fail(s, P4A_ERR_ARG, "unknown stage");
+  }
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!box) return P4A_ERR_ARG;
+  const double oldBox[3] = {s->params.box[0], s->params.box[1], s->params.box[2]};
+  bool moved = false;
+  for (int a = 0; a < 3; a++)
+    if (std::fabs(box[a] - oldBox[a]) > 1e-9) moved = true;
+  if (!moved) return P4A_OK;
+  for (int a = 0; a < 3; a++)
+    if (!(box[a] > 0)) return fail(s, P4A_ERR_ARG, "box sizes must be > 0");
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  drop_graphs(s);
+  for (int a = 0; a < 3; a++) s->params.box[a] = box[a];
+  const int oldCells = s->nCells;
+  if ((rc = set_grid(s))) return rc;
+  if (s->nCells != oldCells && (rc = alloc_cells(s))) return rc;
+  int32_t nb = 0;
+  if ((rc = p4a_boundary_build(box, s->params.spacing, s->h, s->params.rest_density, nullptr,
+                               nullptr, 0, &nb)))
+    return fail(s, rc, "boundary build failed");
+  std::vector<float> pts((size_t)std::max(1, nb) * 3), psi((size_t)std::max(1, nb));
+  if ((rc = p4a_boundary_build(box, s->params.spacing, s->h, s->params.rest_density, pts.data(),
+                               psi.data(), nb, &nb)))
+    return fail(s, rc, "boundary build failed");
+  if ((rc = upload_boundary(s, pts.data(), psi.data(), nb))) return rc;
+  s->gen += 1;
+  ResizeArgs R;
+  const double halfD = 0.5 * s->params.spacing;
+  for (int a = 0; a < 3; a++) {
+    R.oldMin[a] = 0.0f;
+    R.oldSize[a] = (float)oldBox[a];
+    R.newMin[a] = 0.0f;
+    R.newSize[a] = (float)box[a];
+    R.clampMin[a] = (float)halfD;
+    R.clampMax[a] = (float)(box[a] - halfD);
+  }
+  R.n = (uint32_t)s->n;
+  LAUNCH(s, "resize", launch_resize(R, s->A.pos[s->parity], s->stream));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, int32_t count,
+                             int32_t* appended) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (appended) *appended = 0;
+  if (count <= 0) return P4A_OK;
+  if (!pos3 || !vel3) return P4A_ERR_ARG;
+  const int room = std::max(0, s->cap - s->n);
+  const int k = std::min(count, room);
+  if (k <= 0) return P4A_OK;
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  drop_graphs(s);
+  const size_t off = (size_t)s->n;
+  std::vector<float> p4((size_t)k * 4), v4((size_t)k * 4, 0.0f), zero4((size_t)k * 4, 0.0f),
+      zero1(k, 0.0f), rho(k, (float)s->params.rest_density);
+  std::vector<uint32_t> ids(k);
+  for (int i = 0; i < k; i++) {
+    p4[i * 4 + 0] = pos3[i * 3 + 0];
+    p4[i * 4 + 1] = pos3[i * 3 + 1];
+    p4[i * 4 + 2] = pos3[i * 3 + 2];
+    p4[i * 4 + 3] = 1.0f;
+    v4[i * 4 + 0] = vel3[i * 3 + 0];
+    v4[i * 4 + 1] = vel3[i * 3 + 1];
+    v4[i * 4 + 2] = vel3[i * 3 + 2];
+    ids[i] = (uint32_t)(off + i);
+  }
+  const int par = s->parity;
+  HIP_TRY(s, hipMemcpy(s->A.pos[par] + off, p4.data(), p4.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.vel[par] + off, v4.data(), v4.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->predCur + off, p4.data(), p4.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.density + off, rho.data(), rho.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.lambda + off, zero1.data(), zero1.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.corr + off, zero4.data(), zero4.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.normal + off, zero4.data(), zero4.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.pid[par] + off, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.slotOf + off, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  s->n += k;
+  if (appended) *appended = k;
+  return P4A_OK;
+}
+
+P4A_API int p4a_apply_ray_impulse(p4a_sim* s, const double origin[3], const double dir[3],
+                                  const double impulse[3], double radius, double speed_limit) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!origin || !dir || !impulse) return P4A_ERR_ARG;
+  if (!s->n || radius <= 0) return P4A_OK;
+  const double len = std::sqrt(impulse[0] * impulse[0] + impulse[1] * impulse[1] + impulse[2] * impulse[2]);
+  if (len <= 0) return P4A_OK;
+  double dl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (!(dl != 0)) dl = 1;  // `|| 1`
+  RayArgs R;
+  for (int k = 0; k < 3; k++) {
+    R.origin[k] = (float)origin[k];
+    R.dir[k] = (float)(dir[k] / dl);
+    R.impulse[k] = (float)impulse[k];
+  }
+  R.radius = (float)radius;
+  R.speedLimit = (float)speed_limit;
+  R.n = (uint32_t)s->n;
+  HIP_TRY(s, hipSetDevice(s->device));
+  LAUNCH(s, "impulse", launch_impulse(s->A.pos[s->parity], s->A.vel[s->parity], R, s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_hold_body(p4a_sim* s, int32_t body, const double target[3], double rate,
+                          double limit) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!target) return P4A_ERR_ARG;
+  s->heldBody = (body >= 1 && body <= s->nBodies) ? body : -1;
+  for (int k = 0; k < 3; k++) s->heldTarget[k] = target[k];
+  s->heldRate = rate;
+  s->heldLimit = limit;
+  return P4A_OK;
+}
+
+P4A_API int p4a_release_body(p4a_sim* s) {
+  int rc = check(s);
+  if (rc) return rc;
+  s->heldBody = -1;
+  return P4A_OK;
+}
+
+P4A_API int p4a_read(p4a_sim* s, int32_t id, void* dst, size_t offset, size_t bytes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!dst && bytes) return P4A_ERR_ARG;
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  if (id == P4A_BUF_BODY || id == P4A_BUF_REST) {
+    // reference layout (uint4 / float4 per live slot) from the id maps
+    const size_t total = (size_t)s->n * 16;
+    if (offset > total || bytes > total - offset) return fail(s, P4A_ERR_RANGE, "read past end");
+    std::vector<uint32_t> pid(s->n);
+    HIP_TRY(s, hipMemcpy(pid.data(), s->A.pid[s->parity], (size_t)s->n * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> out((size_t)s->n * 4, 0u);
+    for (int i = 0; i < s->n; i++) {
+      const uint32_t q = pid[i];
+      if (id == P4A_BUF_BODY) out[(size_t)i * 4] = q < s->bodyOfPidH.size() ? s->bodyOfPidH[q] : 0u;
+      else if ((size_t)q * 4 + 3 < s->restOfPidH.size())
+        std::memcpy(&out[(size_t)i * 4], &s->restOfPidH[(size_t)q * 4], 16);
+    }
+    std::memcpy(dst, (const char*)out.data() + offset, bytes);
+    return P4A_OK;
+  }
+  BufView v = buf_view(s, id);
+  if (!v.ptr) return fail(s, P4A_ERR_ARG, "unknown buffer id");
+  if (offset > v.bytes || bytes > v.bytes - offset) return fail(s, P4A_ERR_RANGE, "read past end");
+  if (bytes) HIP_TRY(s, hipMemcpy(dst, (const char*)v.ptr + offset, bytes, hipMemcpyDeviceToHost));
+  return P4A_OK;
+}
+
+P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, size_t bytes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!src && bytes) return P4A_ERR_ARG;
+  if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
+      id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
+    return fail(s, P4A_ERR_ARG, "buffer is not writable");
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  BufView v = buf_view(s, id);
+  if (!v.ptr) return fail(s, P4A_ERR_ARG, "unknown buffer id");
+  if (offset > v.bytes || bytes > v.bytes - offset) return fail(s, P4A_ERR_RANGE, "write past end");
+  if (bytes) HIP_TRY(s, hipMemcpy((char*)v.ptr + offset, src, bytes, hipMemcpyHostToDevice));
+  drop_graphs(s);
+  return P4A_OK;
+}
+
+P4A_API int p4a_device_ptr(p4a_sim* s, int32_t id, void** ptr, size_t* bytes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!ptr) return P4A_ERR_ARG;
+  BufView v = buf_view(s, id);
+  if (!v.ptr) return fail(s, P4A_ERR_ARG, "no device buffer with that id");
+  *ptr = v.ptr;
+  if (bytes) *bytes = v.bytes;
+  return P4A_OK;
+}
+
+P4A_API int p4a_get_info(p4a_sim* s, p4a_info* o) {
+  if (!s || !o) return P4A_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->n = s->n; o->cap = s->cap; o->n_fluid = s->nFluid; o->n_body_parts = s->nBodyParts;
+  o->n_bodies = s->nBodies; o->n_boundary = s->nBoundary; o->n_cells = s->nCells;
+  for (int k = 0; k < 3; k++) o->grid_dim[k] = s->gridDim[k];
+  o->parity = s->parity; o->gen = s->gen; o->last_substeps = s->lastSubsteps;
+  o->pour_budget = s->pourBudget;
+  o->h = s->h; o->mass = s->mass; o->denom_rest = s->denomRest; o->sim_time = s->simTime;
+  o->last_advanced = s->lastAdvanced; o->time_bank = s->timeBank;
+  o->body_scale = s->bodyScale; o->cov_scale = s->covScale;
+  return P4A_OK;
+}
+
+P4A_API int p4a_stats(p4a_sim* s, double out[4]) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!out) return P4A_ERR_ARG;
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 4; k++) out[k] = s->statsHost[k];
+  return P4A_OK;
+}
+
+P4A_API int p4a_body_pose(p4a_sim* s, float* centre, float* rot) {
+  int rc = check(s);
+  if (rc) return rc;
+  const int nB = s->nBodies;
+  if (nB == 0) return P4A_OK;
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  std::vector<float> c((size_t)nB * 4), r((size_t)nB * 12);
+  HIP_TRY(s, hipMemcpy(c.data(), s->A.bodyCentre, c.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(s, hipMemcpy(r.data(), s->A.bodyRot, r.size() * 4, hipMemcpyDeviceToHost));
+  for (int b = 0; b < nB; b++) {
+    if (centre)
+      for (int k = 0; k < 3; k++) centre[b * 3 + k] = c[b * 4 + k];
+    if (rot)
+      for (int row = 0; row < 3; row++)
+        for (int col = 0; col < 3; col++) rot[b * 9 + col * 3 + row] = r[(b * 3 + row) * 4 + col];
+  }
+  return P4A_OK;
+}
+
+P4A_API int p4a_sync(p4a_sim* s) {
+  if (!s) return P4A_ERR_ARG;
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_neighbour_census(p4a_sim* s, uint32_t* count, uint32_t* sum_j, uint32_t* sum_j2) {
+  int rc = check(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipSetDevice(s->device));
+  const Uniform U = make_uniform(s, 0.0);
+  LAUNCH(s, "census", launch_census(U, s->A, s->predCur, s->tile1, s->stream));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  const size_t b = (size_t)s->n * 4;
+  if (count) HIP_TRY(s, hipMemcpy(count, s->A.census[0], b, hipMemcpyDeviceToHost));
+  if (sum_j) HIP_TRY(s, hipMemcpy(sum_j, s->A.census[1], b, hipMemcpyDeviceToHost));
+  if (sum_j2) HIP_TRY(s, hipMemcpy(sum_j2, s->A.census[2], b, hipMemcpyDeviceToHost));
+  return P4A_OK;
+}
+
+P4A_API int p4a_set_timing(p4a_sim* s, int32_t on) {
+  if (!s) return P4A_ERR_ARG;
+  s->timing = on != 0;
+  s->timed.clear();
+  s->eventNext = 0;
+  return P4A_OK;
+}
+
+P4A_API int p4a_timing(p4a_sim* s, char* names, size_t names_bytes, double* ms,
+                       int32_t* launches, int32_t capacity, int32_t* count) {
+  if (!s || !count) return P4A_ERR_ARG;
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  std::vector<std::string> keys;
+  std::vector<double> tot;
+  std::vector<int> cnt;
+  for (const TimedLaunch& t : s->timed) {
+    float e = 0;
+    if (hipEventElapsedTime(&e, t.a, t.b) != hipSuccess) continue;
+    size_t k = 0;
+    while (k < keys.size() && keys[k] != t.name) k++;
+    if (k == keys.size()) {
+      keys.push_back(t.name);
+      tot.push_back(0);
+      cnt.push_back(0);
+    }
+    tot[k] += e;
+    cnt[k] += 1;
+  }
+  *count = (int32_t)keys.size();
+  std::string all;
+  for (size_t k = 0; k < keys.size(); k++) {
+    if ((int32_t)k < capacity) {
+      if (ms) ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
+      if (launches) launches[k] = cnt[k];
+    }
+    all += keys[k];
+    all += "\n";
+  }
+  if (names && names_bytes) {
+    std::strncpy(names, all.c_str(), names_bytes - 1);
+    names[names_bytes - 1] = 0;
+  }
+  return P4A_OK;
+}
+
+}  // extern "C"

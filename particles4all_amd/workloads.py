@@ -1,0 +1,74 @@
+"""Synthetic workloads in lattice units (spacing d = 1, h = 2d, rho0 = 1000).
+
+  dam_break_32     32^3 jittered block in a 64^3 box, two floating 8^3 cubes
+                   of density 0.5; frame 1/60 s, 2 substeps, 4 iterations.
+  block_1m         128^3 block filling half of a 256x128x128 box; substep
+                   1/120 s, 4 iterations, XSPH 0.01, surface tension 0.5.
+  block_4m_bodies  256x128x128 block in a 512x192x128 box plus 64 bodies
+                   (32 lattice cubes, edges 6-12; 32 lattice spheres, radii
+                   4-8; densities log-uniform in [0.3, 2.0]) in the space
+                   above the fluid; settings as block_1m.
+
+Jitter is uniform +-0.05d per coordinate from the reference's LCG idiom
+(scene.js:52-56); mass and denomRest are calibrated on the unjittered
+lattice.  `scale` shrinks the lattice for tests (box shrinks with it).
+"""
+import numpy as np
+
+from ._lib import (SHAPE_LATTICE_CUBE, SHAPE_LATTICE_SPHERE, default_params, lattice_scene)
+
+
+def _params(box, substeps, iterations, xsph, tension, gravity=9.8, **kw):
+    p = default_params(spacing=1.0, rest_density=1000.0, gravity=gravity, substeps=substeps,
+                       iterations=iterations, xsph_c=xsph, surface_tension_k=tension,
+                       box=box, pour_budget=0, **kw)
+    return p
+
+
+def dam_break_32(scale=1.0, seed=1, bodies=True, **kw):
+    L = max(4, int(round(32 * scale)))
+    B = 2 * L
+    p = _params([B, B, B], substeps=2, iterations=4, xsph=0.05, tension=0.0, **kw)
+    bl = []
+    if bodies:
+        e = max(2, int(round(8 * scale)))
+        y = 1.5 * L
+        bl = [(SHAPE_LATTICE_CUBE, e, 0.5, (0.5 * L, y, 0.5 * L)),
+              (SHAPE_LATTICE_CUBE, e, 0.5, (1.5 * L, y, 0.5 * L))]
+    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed, bodies=bl)
+    return p, sc, 1.0 / 60.0
+
+
+def block_1m(scale=1.0, seed=2, **kw):
+    L = max(4, int(round(128 * scale)))
+    p = _params([2 * L, L, L], substeps=2, iterations=4, xsph=0.01, tension=0.5, **kw)
+    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed)
+    return p, sc, 1.0 / 120.0
+
+
+def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
+    nx, ny, nz = (max(4, int(round(v * scale))) for v in (256, 128, 128))
+    box = [2 * nx, ny * 1.5, nz]
+    p = _params(box, substeps=2, iterations=4, xsph=0.01, tension=0.5, **kw)
+    rng = np.random.default_rng(seed)
+    bl = []
+    cols = max(1, n_bodies // 4)
+    pitch_x, pitch_z = box[0] / cols, box[2] / 4
+    y = ny + 0.25 * ny
+    s = max(scale, 0.05)
+    for k in range(n_bodies):
+        cx = (k % cols + 0.5) * pitch_x
+        cz = (k // cols % 4 + 0.5) * pitch_z
+        dens = float(np.exp(rng.uniform(np.log(0.3), np.log(2.0))))
+        if k % 2 == 0:
+            edge = int(rng.integers(6, 13))
+            bl.append((SHAPE_LATTICE_CUBE, max(2, round(edge * s)), dens, (cx, y, cz)))
+        else:
+            rad = int(rng.integers(4, 9))
+            bl.append((SHAPE_LATTICE_SPHERE, max(1, round(rad * s)), dens, (cx, y, cz)))
+    sc = lattice_scene(p, (nx, ny, nz), jitter=0.05, seed=seed, bodies=bl)
+    return p, sc, 1.0 / 120.0
+
+
+WORKLOADS = {"dam_break_32": dam_break_32, "block_1m": block_1m,
+             "block_4m_bodies": block_4m_bodies}

@@ -1,0 +1,335 @@
+"""Pins the oracle (oracle/oracle.cpp) against closed forms, library
+routines, invariants and brute force, so that the GPU parity tests compare
+against something that is itself checked.  CPU only.
+
+What each test pins:
+  cell ids           IEEE float division + floor + clamp, written in numpy
+  grid               numpy's stable argsort and bincount
+  census             brute-force O(n^2) neighbour sets
+  lattice density    rho = m * sum_o W(|o| d) on an unjittered lattice
+  two-body forms     lambda / delta / xsph / normals / tension for a pair,
+                     derived from the formulas of the reference kernels
+  momentum           sum of delta-p and of the tension impulses vanish
+                     (antisymmetric pair terms), uniform velocity -> xsph 0
+  polar              a rigidly rotated body gives back its rotation
+  modes              float mode and FP64 mode agree to float precision
+  kernels            poly6 and spiky are unit-normalised (radial quadrature)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def make_P(box=(16.0, 12.0, 10.0), d=1.0, mass=1.0, denom=0.5, dt=1 / 120, tension=0.5,
+           xsph=0.05, sor=False, nb=0, iters=4):
+    return O.constants(list(box), d, 2 * d, mass, denom, 1000.0, 9.8, iters, 0.01, 0.1, 0.3,
+                       xsph, 1.0, sor, tension, nb, dt)
+
+
+def float4(p):
+    out = np.zeros((len(p), 4))
+    out[:, :3] = np.asarray(p, np.float32)
+    return out
+
+
+def fresh_grid(P, pred, mode=O.F32):
+    key = O.cell_ids(mode, P, pred)
+    cs, slot, order = O.grid(P, key)
+    return key, cs, slot, order
+
+
+def test_kernel_normalisation():
+    h = 2.0
+    r = np.linspace(0, h, 200001)
+    poly6 = 315 / (64 * math.pi * h ** 9) * (h * h - r * r) ** 3
+    spiky = 15 / (math.pi * h ** 6) * (h - r) ** 3
+    assert abs(np.trapezoid(4 * math.pi * r * r * poly6, r) - 1) < 1e-8
+    assert abs(np.trapezoid(4 * math.pi * r * r * spiky, r) - 1) < 1e-8
+    P = make_P()
+    # spiky gradient coefficient is d/dr of the unit spiky kernel
+    assert P.spiky == pytest.approx(-3 * 15 / (math.pi * h ** 6), rel=1e-15)
+
+
+def test_cell_ids_closed_form():
+    rng = np.random.default_rng(0)
+    P = make_P()
+    g = [int(x) for x in P.gridDim]
+    p = rng.uniform(-3, 20, (5000, 3)).astype(np.float32)
+    p[:100] = np.round(p[:100])  # exact cell boundaries
+    p[100] = np.nan
+    pred = float4(p)
+    key = O.cell_ids(O.F32, P, pred)
+    h32 = np.float32(P.h)
+    with np.errstate(invalid="ignore"):
+        c = np.floor(p / h32).astype(np.float64)
+    c = np.nan_to_num(c, nan=0.0)
+    for a in range(3):
+        c[:, a] = np.clip(c[:, a], 0, g[a] - 1)
+    c = c.astype(np.int64)
+    want = (c[:, 2] * g[1] + c[:, 1]) * g[0] + c[:, 0]
+    assert np.array_equal(key, want.astype(np.uint32))
+
+
+def test_grid_is_stable_counting_sort():
+    rng = np.random.default_rng(1)
+    P = make_P()
+    nc = O.n_cells(P)
+    key = rng.integers(0, nc, 20000).astype(np.uint32)
+    key[:5000] = rng.integers(0, 7, 5000)  # heavy cells
+    cs, slot, order = O.grid(P, key)
+    perm = np.argsort(key, kind="stable")
+    assert np.array_equal(order, perm)
+    assert np.array_equal(slot[perm], np.arange(len(key)))
+    counts = np.bincount(key, minlength=nc)
+    assert np.array_equal(cs[1:], np.cumsum(counts))
+    assert cs[0] == 0 and cs[-1] == len(key)
+
+
+def _r2_f32(a, b):
+    """fma(z,z, fma(y,y, x*x)) of float32 differences, evaluated in FP64
+    (exact except in rare double-rounding cases, which the callers treat as
+    borderline)."""
+    d = (a.astype(np.float32) - b.astype(np.float32)).astype(np.float64)
+    xx = np.float32(d[..., 0] * d[..., 0]).astype(np.float64)
+    t = np.float32(d[..., 1] * d[..., 1] + xx).astype(np.float64)
+    return np.float32(d[..., 2] * d[..., 2] + t).astype(np.float64)
+
+
+def test_census_matches_brute_force():
+    rng = np.random.default_rng(2)
+    P = make_P(box=(14.0, 10.0, 8.0))
+    p = rng.uniform(0, [14, 10, 8], (1500, 3)).astype(np.float32)
+    key, cs, slot, order = fresh_grid(P, float4(p))
+    ps = p[order]
+    cnt, s1, s2 = O.census(P, float4(ps), cs)
+    r2 = _r2_f32(ps[:, None, :], ps[None, :, :])
+    h2 = np.float32(P.h2)
+    near = r2 < h2
+    np.fill_diagonal(near, False)
+    border = np.abs(r2 - h2) <= 4 * np.spacing(h2)
+    ok_rows = ~border.any(axis=1)
+    assert ok_rows.mean() > 0.99
+    j = np.arange(len(ps), dtype=np.uint64)
+    want_cnt = near.sum(axis=1)
+    want_s1 = (near * j[None, :]).sum(axis=1) % (1 << 32)
+    want_s2 = (near * (j * j % (1 << 32))[None, :]).sum(axis=1) % (1 << 32)
+    assert np.array_equal(cnt[ok_rows], want_cnt[ok_rows])
+    assert np.array_equal(s1[ok_rows], want_s1[ok_rows].astype(np.uint32))
+    assert np.array_equal(s2[ok_rows], want_s2[ok_rows].astype(np.uint32))
+
+
+def lattice_mass(d, h, rho0):
+    coef = 315 / (64 * math.pi * h ** 9)
+    s = 0.0
+    R = int(math.ceil(h / d)) + 1
+    for a in range(-R, R + 1):
+        for b in range(-R, R + 1):
+            for c in range(-R, R + 1):
+                r2 = (a * a + b * b + c * c) * d * d
+                if r2 < h * h:
+                    s += coef * (h * h - r2) ** 3
+    return rho0 / s
+
+
+def test_lattice_density_equals_rest_density():
+    m = lattice_mass(1.0, 2.0, 1000.0)
+    P = make_P(box=(12.0, 12.0, 12.0), mass=m)
+    g = np.arange(12) + 0.5
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    key, cs, slot, order = fresh_grid(P, float4(pts), O.F64)
+    ps = float4(pts[order])
+    den, lam = O.lambda_(O.F64, P, ps, cs)
+    interior = np.all((ps[:, :3] > 2.5) & (ps[:, :3] < 9.5), axis=1)
+    assert interior.sum() > 100
+    assert np.max(np.abs(den[interior] / 1000.0 - 1)) < 1e-12
+    assert np.all(lam <= 0)  # lambda is never positive
+    # fewer neighbours at the surface: density below rest, lambda zero
+    assert np.all(lam[~interior & (den < 1000.0)] == 0)
+
+
+def pair_P(r, mass=3000.0, **kw):
+    P = make_P(box=(20.0, 20.0, 20.0), mass=mass, **kw)
+    return P
+
+
+def pair_state(r, u=(0.6, 0.48, 0.64)):
+    u = np.asarray(u) / np.linalg.norm(u)
+    a = np.array([10.1, 10.2, 10.3])
+    return float4([a, a + r * u]), u
+
+
+@pytest.mark.parametrize("r", [0.3, 1.0, 1.7])
+def test_pair_lambda_delta_closed_form(r):
+    P = pair_P(r)
+    pred, u = pair_state(r)
+    key, cs, slot, order = fresh_grid(P, pred, O.F64)
+    ps = pred[order]
+    den, lam = O.lambda_(O.F64, P, ps, cs)
+    h, h2 = P.h, P.h2
+    r = float(np.linalg.norm(ps[1, :3] - ps[0, :3]))
+    W = lambda q: P.poly6 * (h2 - q * q) ** 3
+    rho = P.mass * (W(0.0) + W(r))
+    s = P.volume * P.spiky * (h - r) ** 2 / r  # |grad| / r
+    g2 = (s * r) ** 2
+    C = rho / P.rho0 - 1
+    want = -C / (g2 + g2 + P.cfmEps + 1e-12) if C > 0 else 0.0
+    assert den == pytest.approx([rho, rho], rel=1e-13)
+    assert lam == pytest.approx([want, want], rel=1e-12)
+    out = O.delta(O.F64, P, ps, lam, cs)
+    sc = -P.sCorrK * (W(r) * P.sCorrWq) ** 4
+    rij = ps[0, :3] - ps[1, :3]
+    dp = (lam[0] + lam[1] + sc) * (P.spiky * (h - r) ** 2 / r) * rij * P.volume
+    np.testing.assert_allclose(out[0, :3], ps[0, :3] + dp, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[1, :3], ps[1, :3] - dp, rtol=0, atol=1e-12)
+
+
+def test_pair_xsph_normals_tension_closed_form():
+    r = 1.2
+    P = pair_P(r, mass=900.0)
+    pred, u = pair_state(r)
+    key, cs, slot, order = fresh_grid(P, pred, O.F64)
+    ps = pred[order]
+    pos = ps.copy()
+    v = np.array([[0.5, -1.0, 0.25], [-0.75, 0.5, 1.5]])
+    pos[:, :3] = ps[:, :3] - v * P.dt  # so that (pred - pos)/dt = v
+    vel_true = (ps[:, :3] - pos[:, :3]) * P.invDt
+    den = np.array([1020.0, 480.0])
+    vel, corr, normal = O.xsph(O.F64, P, ps, pos, den, cs, True)
+    h, h2 = P.h, P.h2
+    dh = np.maximum(den, 0.5 * P.rho0)
+    rij = ps[0, :3] - ps[1, :3]
+    rr = np.linalg.norm(rij)  # r after rounding the positions to float
+    assert rr == pytest.approx(r, rel=1e-6)
+    W = P.poly6 * (h2 - rr * rr) ** 3
+    want0 = P.xsphC * P.mass * W / dh[1] * (vel_true[1] - vel_true[0])
+    want1 = P.xsphC * P.mass * W / dh[0] * (vel_true[0] - vel_true[1])
+    np.testing.assert_allclose(corr[0, :3], want0, atol=1e-12)
+    np.testing.assert_allclose(corr[1, :3], want1, atol=1e-12)
+    g = P.spiky * (h - rr) ** 2 / rr
+    np.testing.assert_allclose(normal[0, :3], h * P.mass / dh[1] * g * rij, atol=1e-12)
+    np.testing.assert_allclose(normal[1, :3], -h * P.mass / dh[0] * g * rij, atol=1e-12)
+    # Akinci cohesion + curvature for the pair
+    c2 = O.tension(O.F64, P, ps, den, normal, cs, corr)
+    k = (h - rr) ** 3 * rr ** 3
+    cw = P.cohes * k if 2 * rr > h else P.cohes * (2 * k - P.cohesTerm)
+    K = 2 * P.rho0 / (den[0] + den[1])
+    f0 = K * (-P.tension * P.mass * cw / rr * rij - P.tension * (normal[0, :3] - normal[1, :3]))
+    np.testing.assert_allclose(c2[0, :3] - corr[0, :3], P.dt * f0, atol=1e-12)
+    np.testing.assert_allclose(c2[1, :3] - corr[1, :3], -P.dt * f0, atol=1e-12)
+
+
+def random_cloud(n=3000, seed=3, box=(24.0, 20.0, 20.0)):
+    rng = np.random.default_rng(seed)
+    lo = np.array([4.0, 4.0, 4.0])
+    hi = np.array(box) - 4.0
+    return rng.uniform(lo, hi, (n, 3))
+
+
+def test_delta_momentum_and_tension_momentum_vanish():
+    m = lattice_mass(1.0, 2.0, 1000.0) * 1.3
+    P = make_P(box=(24.0, 20.0, 20.0), mass=m)
+    P.clampMin = type(P.clampMin)(-1e9, -1e9, -1e9)
+    P.clampMax = type(P.clampMax)(1e9, 1e9, 1e9)
+    p = random_cloud()
+    key, cs, slot, order = fresh_grid(P, float4(p), O.F64)
+    ps = float4(p[order])
+    den, lam = O.lambda_(O.F64, P, ps, cs)
+    assert (lam < 0).sum() > 100
+    out = O.delta(O.F64, P, ps, lam, cs)
+    dp = out[:, :3] - ps[:, :3]
+    assert np.abs(dp).max() > 1e-3
+    assert np.abs(dp.sum(axis=0)).max() < 1e-9 * np.abs(dp).sum()
+    pos = ps.copy()
+    vel, corr, normal = O.xsph(O.F64, P, ps, pos, den, cs, True)
+    c2 = O.tension(O.F64, P, ps, den, normal, cs, corr)
+    imp = c2[:, :3] - corr[:, :3]
+    assert np.abs(imp).max() > 0
+    assert np.abs(imp.sum(axis=0)).max() < 1e-9 * np.abs(imp).sum()
+
+
+def test_uniform_velocity_gives_no_xsph():
+    P = make_P(box=(24.0, 20.0, 20.0))
+    p = random_cloud()
+    key, cs, slot, order = fresh_grid(P, float4(p), O.F64)
+    ps = float4(p[order])
+    pos = ps.copy()
+    pos[:, :3] -= np.array([0.3, -0.2, 0.1]) * P.dt
+    den = np.full(len(ps), 1000.0)
+    vel, corr, normal = O.xsph(O.F64, P, ps, pos, den, cs, False)
+    assert np.abs(corr[:, :3]).max() < 1e-10
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("inv_ratio", [1.0, 2.0])
+def test_polar_recovers_rotation_and_centroid(inv_ratio):
+    rng = np.random.default_rng(4)
+    q = rng.uniform(-3, 3, (200, 3))
+    q -= q.mean(axis=0)
+    R = random_rotation(rng)
+    t = np.array([30.0, 20.0, 10.0])
+    pred = float4(q @ R.T + t)
+    pred[:, :3] = (q @ R.T + t)  # keep FP64 positions exactly
+    rest = float4(q)
+    rest[:, :3] = q
+    body = np.ones(len(q), np.uint32)
+    c0 = np.array([[29.0, 21.0, 10.5]])
+    out, centre, rot = O.body_solve(pred, body, rest, [inv_ratio], c0, np.eye(3).reshape(1, 9))
+    np.testing.assert_allclose(rot.reshape(3, 3), R, atol=1e-12)
+    np.testing.assert_allclose(centre[0], c0[0] + inv_ratio * (t - c0[0]), atol=1e-12)
+    cen = c0 = O.body_pre(pred, body, np.array([[0.0, 0.0, 0.0]]))
+    np.testing.assert_allclose(cen[0], t, atol=1e-12)
+
+
+def test_float_and_fp64_modes_agree():
+    m = lattice_mass(1.0, 2.0, 1000.0)
+    P = make_P(box=(24.0, 20.0, 20.0), mass=m)
+    p = random_cloud()
+    key, cs, slot, order = fresh_grid(P, float4(p))
+    ps = float4(p[order])
+    d32, l32 = O.lambda_(O.F32, P, ps, cs)
+    d64, l64 = O.lambda_(O.F64, P, ps, cs)
+    np.testing.assert_allclose(d32, d64, rtol=2e-6)
+    np.testing.assert_allclose(l32, l64, atol=2e-6 * np.abs(l64).max())
+    o32 = O.delta(O.F32, P, ps, l64, cs)
+    o64 = O.delta(O.F64, P, ps, l64, cs)
+    np.testing.assert_allclose(o32[:, :3], o64[:, :3], atol=4e-6)
+
+
+def test_stats_closed_form():
+    rng = np.random.default_rng(5)
+    den = rng.uniform(900, 1100, 1000)
+    vel = np.zeros((1000, 4))
+    vel[:, :3] = rng.normal(size=(1000, 3))
+    s = O.stats(den, vel, 1 / 1000.0, 2.0)
+    v = np.float32(vel[:, :3]).astype(np.float64)
+    assert s[0] == pytest.approx(np.mean(den / 1000.0), rel=1e-6)
+    assert s[1] == pytest.approx(den.max() / 1000.0, rel=1e-6)
+    assert s[2] == pytest.approx(np.sqrt((v ** 2).sum(1)).max(), rel=1e-6)
+    assert s[3] == pytest.approx(0.5 * 2.0 * (v ** 2).sum(), rel=1e-5)
+
+
+def test_boundary_sort_is_stable_counting_sort():
+    from oracle import scene_ref
+    P = make_P(box=(8.0, 6.0, 6.0))
+    pts, psi = scene_ref.boundary_particles([8.0, 6.0, 6.0], 1.0, 2.0, 1000.0)
+    pts = np.array(pts, np.float32)
+    rec, start = O.sort_boundary(P, pts, psi)
+    assert len(pts) == 9 * 7 * 7 - 7 * 5 * 5
+    assert start[-1] == len(pts) and np.all(np.diff(start.astype(np.int64)) >= 0)
+    g = [int(x) for x in P.gridDim]
+    for c in range(O.n_cells(P)):
+        seg = rec[start[c]:start[c + 1], :3]
+        cc = np.clip(np.floor(seg / P.h), 0, np.array(g) - 1).astype(int)
+        assert np.all((cc[:, 2] * g[1] + cc[:, 1]) * g[0] + cc[:, 0] == c)

@@ -242,6 +242,8 @@ enum p4a_buf {
   P4A_BUF_KEY = 16,        /* uint n: cell id of each particle, pre-sort order */
   P4A_BUF_ORDER = 17,      /* uint n: pre-sort index that landed in each slot */
   P4A_BUF_BPSI = 18,       /* float nb */
+  P4A_BUF_SMOOTH_POS = 19, /* float4 n: smoothed centres of the last p4a_aniso */
+  P4A_BUF_ANISO_G = 20,    /* float4 2n: packed G of the last p4a_aniso */
   P4A_BUF_COUNT_
 };
 
@@ -270,6 +272,35 @@ P4A_API int p4a_stats(p4a_sim* h, double out[4]);
 /* Sim.bodyPose (sim.js:635-655): centre nBodies*3, rot nBodies*9
  * (column-major, as the reference's p.rot). */
 P4A_API int p4a_body_pose(p4a_sim* h, float* centre, float* rot);
+
+/* ---------------------------------------------------------------------
+ * Anisotropic-kernel pass (src/aniso_wgsl.js:3-159, parameters as
+ * SurfaceMesh.buildAnisotropy fills them, src/mesh.js:207-261): per live
+ * particle, the weighted neighbour mean and covariance within
+ * R = radius_scale*h over the current grid, Laplacian-smoothed centre, a 3x3
+ * Jacobi eigensolve and the clamped ellipsoid matrix G.  Output buffers:
+ * P4A_BUF_SMOOTH_POS (xyz, 0) and P4A_BUF_ANISO_G, two float4 per particle:
+ * (G00, G01, G02, G11), (G12, G22, det G, neighbour count incl. self).
+ * ------------------------------------------------------------------- */
+typedef struct p4a_aniso_params {
+  double radius_scale;   /* anisoRadiusScale: R = radius_scale * h, <= 2 */
+  double lambda;         /* anisoLambda: centre smoothing */
+  double kr;             /* anisoRatio: largest/smallest axis ratio cap */
+  double kn;             /* anisoLonely: isotropic scale of lonely particles */
+  double ks;             /* anisoKs, divided by 0.15 R^2 as mesh.js:226 */
+  double stretch;        /* anisoStretch: cap on the largest extent */
+  double min_extent;     /* lower bound of each extent (0 = none) */
+  int32_t neps;          /* anisoMinNeighbours: blend window centre */
+  int32_t fp64_fallback; /* 1: re-solve in FP64 when the eigenvalue gap is
+                            below gap_tol * max|eigenvalue| */
+  double gap_tol;        /* 1e-4 */
+} p4a_aniso_params;
+
+/* SurfaceMesh defaults (mesh.js:81-87) with the FP64 fallback on. */
+P4A_API int p4a_aniso_params_default(p4a_aniso_params* a);
+/* Run the pass on the live positions; *fp64_solves (may be null) receives
+ * the number of particles whose eigensolve took the FP64 path. */
+P4A_API int p4a_aniso(p4a_sim* h, const p4a_aniso_params* a, int32_t* fp64_solves);
 
 /* Wait for all queued GPU work of the handle. */
 P4A_API int p4a_sync(p4a_sim* h);

@@ -370,11 +370,158 @@ void polar(double A[3][3], double R[3][3]) {
   }
 }
 
+// Anisotropic kernel (aniso_wgsl.js:3-159).  ap = {lambda, support, radius,
+// cells, kr, kn, ks (already divided by 0.15 R^2), stretch, minExtent, neps}.
+template <class R>
+void jacobi_ref(R a[3][3], R v[3][3]) {  // aniso_wgsl.js:30-55, row-major
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) v[i][j] = i == j ? (R)1 : (R)0;
+  for (int sweep = 0; sweep < 6; sweep++)
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        const R apq = a[p][q];
+        if (std::fabs(apq) < (R)1.0e-12) continue;
+        const R theta = (a[q][q] - a[p][p]) / ((R)2 * apq);
+        const R sg = theta > 0 ? (R)1 : (theta < 0 ? (R)-1 : (R)0);
+        R t = sg / (std::fabs(theta) + sqrt_r(theta * theta + (R)1));
+        if (theta == (R)0) t = (R)1;
+        const R c = (R)1 / sqrt_r(t * t + (R)1), s = t * c;
+        R r[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        r[p][p] = c; r[q][q] = c; r[p][q] = s; r[q][p] = -s;
+        R m[3][3], nn[3][3];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) m[i][j] = r[0][i] * a[0][j] + r[1][i] * a[1][j] + r[2][i] * a[2][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) a[i][j] = m[i][0] * r[0][j] + m[i][1] * r[1][j] + m[i][2] * r[2][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) nn[i][j] = v[i][0] * r[0][j] + v[i][1] * r[1][j] + v[i][2] * r[2][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) v[i][j] = nn[i][j];
+      }
+}
+
+template <class R>
+void aniso_k(const OrcParams& P, int n, const double* pos, const uint32_t* cs, const double* ap,
+             double* smooth, double* g) {
+  K<R> k(P);
+  const R lambda = (R)ap[0], support = (R)ap[1], radius = (R)ap[2];
+  const int cells = (int)ap[3];
+  const R kr = (R)ap[4], kn = (R)ap[5], ks = (R)ap[6], stretch = (R)ap[7], minExt = (R)ap[8];
+  const int neps = (int)ap[9];
+  const R invSupport = (R)(1.0 / ap[1]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int i = 0; i < n; i++) {
+    const R xi[3] = {(R)pos[i * 4], (R)pos[i * 4 + 1], (R)pos[i * 4 + 2]};
+    int c[3];
+    for (int a = 0; a < 3; a++) {
+      R f = std::floor((xi[a] - k.boxMin[a]) * invSupport);
+      c[a] = (int)std::fmin(std::fmax(f, (R)0), (R)(k.g[a] - 1));
+    }
+    const int x0 = std::max(c[0] - cells, 0), x1 = std::min(c[0] + cells, k.g[0] - 1);
+    const R r2max = radius * radius, invR = (R)1 / radius;
+    const R rel[3] = {xi[0] - k.boxMin[0], xi[1] - k.boxMin[1], xi[2] - k.boxMin[2]};
+    R sd[3] = {0, 0, 0}, m2[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, wsum = 0;
+    int cnt = 0;
+    for (int dz = -cells; dz <= cells; dz++) {
+      const int z = c[2] + dz;
+      if (z < 0 || z >= k.g[2]) continue;
+      const R dzm = std::fmax(std::fmax((R)z * support - rel[2], rel[2] - (R)(z + 1) * support), (R)0);
+      const R dz2 = dzm * dzm;
+      if (dz2 >= r2max) continue;
+      for (int dy = -cells; dy <= cells; dy++) {
+        const int y = c[1] + dy;
+        if (y < 0 || y >= k.g[1]) continue;
+        const R dym = std::fmax(std::fmax((R)y * support - rel[1], rel[1] - (R)(y + 1) * support), (R)0);
+        if (dz2 + dym * dym >= r2max) continue;
+        const uint32_t rb = (uint32_t)((z * k.g[1] + y) * k.g[0]);
+        for (uint32_t j = cs[rb + x0]; j < cs[rb + x1 + 1]; j++) {
+          const R d[3] = {(R)pos[j * 4] - xi[0], (R)pos[j * 4 + 1] - xi[1], (R)pos[j * 4 + 2] - xi[2]};
+          const R r2 = r2_of(d[0], d[1], d[2]);
+          if (r2 >= r2max) continue;
+          const R t = sqrt_r(r2) * invR;
+          const R w = (R)1 - t * t * t;
+          for (int a = 0; a < 3; a++) sd[a] = sd[a] + d[a] * w;
+          for (int col = 0; col < 3; col++)
+            for (int row = 0; row < 3; row++) m2[row][col] = m2[row][col] + d[row] * (d[col] * w);
+          wsum = wsum + w;
+          cnt++;
+        }
+      }
+    }
+    if (wsum <= (R)0) {
+      const R iso = kn / support;
+      for (int a = 0; a < 3; a++) smooth[i * 4 + a] = xi[a];
+      smooth[i * 4 + 3] = 0;
+      g[i * 8 + 0] = iso; g[i * 8 + 1] = 0; g[i * 8 + 2] = 0; g[i * 8 + 3] = iso;
+      g[i * 8 + 4] = 0; g[i * 8 + 5] = iso; g[i * 8 + 6] = iso * iso * iso; g[i * 8 + 7] = 0;
+      continue;
+    }
+    R db[3];
+    for (int a = 0; a < 3; a++) {
+      db[a] = sd[a] / wsum;
+      smooth[i * 4 + a] = xi[a] + db[a] * lambda;
+    }
+    smooth[i * 4 + 3] = 0;
+    R A[3][3], V[3][3];
+    for (int row = 0; row < 3; row++)
+      for (int col = 0; col < 3; col++) A[row][col] = m2[row][col] / wsum - db[row] * db[col];
+    jacobi_ref(A, V);
+    const R sig[3] = {A[0][0], A[1][1], A[2][2]};
+    const R s1 = std::fmax(sig[0], std::fmax(sig[1], sig[2]));
+    R ext[3], inv[3];
+    for (int a = 0; a < 3; a++) ext[a] = ks * std::fmax(sig[a], s1 / kr);
+    const R dv = std::fmax((R)1, std::fmax(ext[0], std::fmax(ext[1], ext[2])) / stretch);
+    for (int a = 0; a < 3; a++) {
+      ext[a] = std::fmax(ext[a] / dv, minExt);
+      inv[a] = (R)1 / (support * ext[a]);
+    }
+    const R e0 = (R)neps * (R)0.4, e1 = (R)neps * (R)1.6;
+    R w = std::fmin(std::fmax(((R)cnt - e0) / (e1 - e0), (R)0), (R)1);
+    w = w * w * ((R)3 - (R)2 * w);
+    const R iso = kn / support;
+    R G[3][3];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        const R an = V[a][0] * inv[0] * V[b][0] + V[a][1] * inv[1] * V[b][1] + V[a][2] * inv[2] * V[b][2];
+        G[a][b] = (a == b ? iso : (R)0) * ((R)1 - w) + an * w;
+      }
+    const R det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) -
+                  G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+                  G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    g[i * 8 + 0] = G[0][0]; g[i * 8 + 1] = G[0][1]; g[i * 8 + 2] = G[0][2]; g[i * 8 + 3] = G[1][1];
+    g[i * 8 + 4] = G[1][2]; g[i * 8 + 5] = G[2][2]; g[i * 8 + 6] = det; g[i * 8 + 7] = cnt;
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
 int orc_version() { return 1; }
+
+void orc_aniso(int mode, const OrcParams* P, int n, const double* pos, const uint32_t* cs,
+               const double* ap, double* smooth, double* g) {
+  if (mode == 0) aniso_k<float>(*P, n, pos, cs, ap, smooth, g);
+  else aniso_k<double>(*P, n, pos, cs, ap, smooth, g);
+}
+
+// The Jacobi eigensolve of the anisotropy pass on one symmetric matrix
+// (row-major a[9]); eigenvalues to w[3], eigenvectors as columns of v[9].
+void orc_jacobi(int mode, const double* a9, double* w, double* v9) {
+  if (mode == 0) {
+    float a[3][3], v[3][3];
+    for (int k = 0; k < 9; k++) a[k / 3][k % 3] = (float)a9[k];
+    jacobi_ref(a, v);
+    for (int k = 0; k < 3; k++) w[k] = a[k][k];
+    for (int k = 0; k < 9; k++) v9[k] = v[k / 3][k % 3];
+  } else {
+    double a[3][3], v[3][3];
+    for (int k = 0; k < 9; k++) a[k / 3][k % 3] = a9[k];
+    jacobi_ref(a, v);
+    for (int k = 0; k < 3; k++) w[k] = a[k][k];
+    for (int k = 0; k < 9; k++) v9[k] = v[k / 3][k % 3];
+  }
+}
 
 // cell id of every particle (wgsl.js:43-49, as count/scatterSlot use it)
 void orc_cell_ids(int mode, const OrcParams* P, int n, const double* pred, uint32_t* key) {

@@ -255,8 +255,35 @@ def stats(density, vel, inv_rho0, mass):
     return out
 
 
+def aniso(mode, P, pos, cs, radius_scale=2.0, lam=0.9, kr=4.0, kn=1.0, ks=1.0, stretch=2.0,
+          min_extent=0.0, neps=25):
+    """Anisotropy pass (aniso_wgsl.js:3-159) with the parameters of
+    SurfaceMesh.buildAnisotropy (mesh.js:221-242)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    cs = np.ascontiguousarray(cs, np.uint32)
+    h = P.h
+    radius = radius_scale * h
+    ap = np.array([lam, h, radius, math.ceil(radius_scale), kr, kn, ks / (0.15 * radius * radius),
+                   stretch, min_extent, neps], np.float64)
+    n = len(pos)
+    smooth = np.zeros((n, 4))
+    g = np.zeros((n, 8))
+    lib().orc_aniso(mode, C.byref(P), n, _p(pos), _p(cs), _p(ap), _p(smooth), _p(g))
+    return smooth, g.reshape(n * 2, 4)
+
+
+def jacobi(mode, a):
+    a = np.ascontiguousarray(a, np.float64).reshape(9)
+    w = np.zeros(3)
+    v = np.zeros(9)
+    lib().orc_jacobi(mode, _p(a), _p(w), _p(v))
+    return w, v.reshape(3, 3)
+
+
 def _setup(L):
     vp = C.c_void_p
+    L.orc_aniso.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
+    L.orc_jacobi.argtypes = [C.c_int, vp, vp, vp]
     L.orc_cell_ids.argtypes = [C.c_int, C.c_void_p, C.c_int, vp, vp]
     L.orc_grid.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
     L.orc_predict.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]

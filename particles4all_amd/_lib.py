@@ -24,7 +24,7 @@ SCENE_PRESET, SCENE_LATTICE = 0, 1
 
 BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CORR=8,
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
-           PID=15, KEY=16, ORDER=17, BPSI=18)
+           PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -64,6 +64,13 @@ class SceneView(C.Structure):
                 ("d", C.c_double), ("denom_rest", C.c_double)]
 
 
+class AnisoParams(C.Structure):
+    _fields_ = [("radius_scale", C.c_double), ("lambda_", C.c_double), ("kr", C.c_double),
+                ("kn", C.c_double), ("ks", C.c_double), ("stretch", C.c_double),
+                ("min_extent", C.c_double), ("neps", C.c_int32), ("fp64_fallback", C.c_int32),
+                ("gap_tol", C.c_double)]
+
+
 class Info(C.Structure):
     _fields_ = [("n", C.c_int32), ("cap", C.c_int32), ("n_fluid", C.c_int32),
                 ("n_body_parts", C.c_int32), ("n_bodies", C.c_int32), ("n_boundary", C.c_int32),
@@ -83,7 +90,7 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_apply_ray_impulse", "p4a_hold_body", "p4a_release_body", "p4a_read",
            "p4a_write", "p4a_device_ptr", "p4a_get_info", "p4a_stats", "p4a_body_pose",
            "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
-           "p4a_timing", "p4a_version"]
+           "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso"]
 
 
 def lib():
@@ -130,6 +137,8 @@ def lib():
     L.p4a_timing.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
                              C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     L.p4a_version.restype = C.c_char_p
+    L.p4a_aniso_params_default.argtypes = [C.POINTER(AnisoParams)]
+    L.p4a_aniso.argtypes = [vp, C.POINTER(AnisoParams), C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -295,7 +304,8 @@ class Sim:
                 "BODY_CENTRE": (np.float32, i.n_bodies, 4),
                 "BODY_ROT": (np.float32, i.n_bodies * 3, 4), "PID": (np.uint32, n, 1),
                 "KEY": (np.uint32, n, 1), "ORDER": (np.uint32, n, 1),
-                "BPSI": (np.float32, i.n_boundary, 1)}[name]
+                "BPSI": (np.float32, i.n_boundary, 1), "SMOOTH_POS": (np.float32, n, 4),
+                "ANISO_G": (np.float32, 2 * n, 4)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")
@@ -360,6 +370,15 @@ class Sim:
         self._check(lib().p4a_timing(self.h, names, 4096, ms, cnt, 64, C.byref(k)), "timing")
         keys = names.value.decode().split("\n")[: k.value]
         return {key: (ms[i], cnt[i]) for i, key in enumerate(keys)}
+
+    def aniso(self, **kw):
+        a = AnisoParams()
+        lib().p4a_aniso_params_default(C.byref(a))
+        for k, v in kw.items():
+            setattr(a, "lambda_" if k == "lambda" else k, v)
+        cnt = C.c_int32()
+        self._check(lib().p4a_aniso(self.h, C.byref(a), C.byref(cnt)), "aniso")
+        return cnt.value
 
     def device_ptr(self, name):
         p = C.c_void_p()

@@ -257,24 +257,37 @@ k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Row-tiled neighbour traversal.
-template <int NREC, int TX, int MAXREC>
+// Row-tiled neighbour traversal over a (2CR+1)^3-cell stencil (CR = 1 for
+// the solver kernels, 2 for the anisotropy pass).
+template <int NREC, int TX, int MAXREC, int CR>
 struct TileLds {
+  static constexpr int ROWS = (2 * CR + 1) * (2 * CR + 1);
+  static constexpr int W = TX + 2 * CR + 1;
   float4 rec[NREC][MAXREC];
-  uint32_t cs[9][TX + 3];   // cellStart of cells xa..xb of each row
-  int off[10];              // LDS offset of each row's first record
-  uint32_t base[9];         // global index of each row's first record
-  int xa[9];
+  uint32_t cs[ROWS][W];     // cellStart of cells xa..xb of each row
+  int off[ROWS + 1];        // LDS offset of each row's first record
+  uint32_t base[ROWS];      // global index of each row's first record
   int anyB;
   int nonEmpty;
 };
 
-template <class Op, int TX, int NT, int MAXREC>
+// Default hooks: the cell of a target is cellOf(pred) (wgsl.js:43-46) and
+// no row is skipped.
+struct OpBase {
+  template <class T>
+  __device__ Cell cell(const Uniform& U, const T& t) const { return cell_of(U, t.px, t.py, t.pz); }
+  template <class T>
+  __device__ bool skip_row(const Uniform&, const T&, int, int) const { return false; }
+};
+
+template <class Op, int TX, int NT, int MAXREC, int CR>
 __global__ void __launch_bounds__(NT)
 k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
        const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
        int tilesPerRow) {
-  __shared__ TileLds<Op::NREC, TX, MAXREC> S;
+  using L = TileLds<Op::NREC, TX, MAXREC, CR>;
+  constexpr int ROWS = L::ROWS, SIDE = 2 * CR + 1, CW = L::W;
+  __shared__ L S;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int txi = tile % tilesPerRow;
@@ -282,7 +295,7 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const int tz = tile / (tilesPerRow * gy);
   const int x0 = txi * TX;
   const int x1 = min(x0 + TX, gx);
-  const int xa = max(x0 - 1, 0), xb = min(x1 + 1, gx);
+  const int xa = max(x0 - CR, 0), xb = min(x1 + CR, gx);
   const int tid = threadIdx.x;
 
   // Empty tiles leave at once.
@@ -291,46 +304,43 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   __syncthreads();
   if (!S.nonEmpty) return;
 
-  // cellStart slices of the 9 neighbour rows.
+  // cellStart slices of the neighbour rows.
   const int W = xb - xa + 1;
-  for (int e = tid; e < 9 * (TX + 3); e += NT) {
-    const int r = e / (TX + 3), k = e - r * (TX + 3);
-    const int z = tz + r / 3 - 1, y = ty + r % 3 - 1;
+  for (int e = tid; e < ROWS * CW; e += NT) {
+    const int r = e / CW, k = e - r * CW;
+    const int z = tz + r / SIDE - CR, y = ty + r % SIDE - CR;
     if (k < W && z >= 0 && z < gz && y >= 0 && y < gy)
       S.cs[r][k] = cellStart[(uint32_t)((z * gy + y) * gx + xa + k)];
   }
-  if (tid < 9) {
-    S.xa[tid] = xa;
-    S.anyB = 0;
-  }
+  if (tid == 0) S.anyB = 0;
   __syncthreads();
-  if (tid < 9) {
+  if (tid < ROWS) {
     const int r = tid;
-    const int z = tz + r / 3 - 1, y = ty + r % 3 - 1;
+    const int z = tz + r / SIDE - CR, y = ty + r % SIDE - CR;
     const bool valid = z >= 0 && z < gz && y >= 0 && y < gy;
-    int cnt = valid ? (int)(S.cs[r][W - 1] - S.cs[r][0]) : 0;
+    const int cnt = valid ? (int)(S.cs[r][W - 1] - S.cs[r][0]) : 0;
     S.base[r] = valid ? S.cs[r][0] : 0u;
-    // exclusive prefix over rows, done by each of the 9 threads
+    // exclusive prefix over rows, done by each row's thread
     int o = 0;
     for (int q = 0; q < r; q++) {
-      const int zq = tz + q / 3 - 1, yq = ty + q % 3 - 1;
+      const int zq = tz + q / SIDE - CR, yq = ty + q % SIDE - CR;
       if (zq >= 0 && zq < gz && yq >= 0 && yq < gy) o += (int)(S.cs[q][W - 1] - S.cs[q][0]);
     }
     S.off[r] = o;
-    if (r == 8) S.off[9] = o + cnt;
-    if (valid && U.nBoundary > 0) {
+    if (r == ROWS - 1) S.off[ROWS] = o + cnt;
+    if (valid && U.nBoundary > 0 && Op::BOUNDARY) {
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       if (bcellStart[rb + xb] != bcellStart[rb + xa]) atomicOr(&S.anyB, 1);
     }
   }
   __syncthreads();
-  const int total = S.off[9];
+  const int total = S.off[ROWS];
   const bool ldsOk = total <= MAXREC;
   if (ldsOk) {
     for (int idx = tid; idx < total; idx += NT) {
       int r = 0;
 #pragma unroll
-      for (int q = 1; q < 9; q++) r += (idx >= S.off[q]) ? 1 : 0;
+      for (int q = 1; q < ROWS; q++) r += (idx >= S.off[q]) ? 1 : 0;
       const uint32_t j = S.base[r] + (uint32_t)(idx - S.off[r]);
       float4 v[Op::NREC];
       op.load(U, j, v);
@@ -340,24 +350,26 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   }
   __syncthreads();
 
-  const uint32_t T0 = S.cs[4][x0 - xa], T1 = S.cs[4][x1 - xa];
+  constexpr int RC = CR * SIDE + CR;  // the tile's own row
+  const uint32_t T0 = S.cs[RC][x0 - xa], T1 = S.cs[RC][x1 - xa];
   const bool anyB = S.anyB != 0;
   for (uint32_t i = T0 + tid; i < T1; i += NT) {
     typename Op::Target t;
     op.target(U, i, t);
-    const Cell c = cell_of(U, t.px, t.py, t.pz);
-    const int xlo = max(c.x - 1, 0), xhi = min(c.x + 1, gx - 1);
+    const Cell c = op.cell(U, t);
+    const int xlo = max(c.x - CR, 0), xhi = min(c.x + CR, gx - 1);
     const bool inTile = ldsOk && c.y == ty && c.z == tz && c.x >= x0 && c.x < x1;
-    const bool doB = U.nBoundary > 0 && (anyB || !inTile);
-    for (int dz = -1; dz <= 1; dz++) {
+    const bool doB = Op::BOUNDARY && U.nBoundary > 0 && (anyB || !inTile);
+    for (int dz = -CR; dz <= CR; dz++) {
       const int z = c.z + dz;
       if (z < 0 || z >= gz) continue;
-      for (int dy = -1; dy <= 1; dy++) {
+      for (int dy = -CR; dy <= CR; dy++) {
         const int y = c.y + dy;
         if (y < 0 || y >= gy) continue;
+        if (op.skip_row(U, t, z, y)) continue;
         const uint32_t rb = (uint32_t)((z * gy + y) * gx);
         if (inTile) {
-          const int r = (dz + 1) * 3 + (dy + 1);
+          const int r = (dz + CR) * SIDE + (dy + CR);
           const int rel = S.off[r] - (int)S.base[r];
           const int b = (int)S.cs[r][xlo - xa] + rel;
           const int e = (int)S.cs[r][xhi + 1 - xa] + rel;
@@ -387,7 +399,8 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
 template <bool EXACT>
-struct LambdaOp {
+struct LambdaOp : OpBase {
+  static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
   float* __restrict__ density;
@@ -468,7 +481,8 @@ struct LambdaOp {
 
 // --- delta (wgsl.js:545-612) -----------------------------------------------
 template <bool EXACT>
-struct DeltaOp {
+struct DeltaOp : OpBase {
+  static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
   const float* __restrict__ lambda;
@@ -552,7 +566,8 @@ struct DeltaOp {
 // Records: (pred.xyz, max(rho_j, rho0/2)), (vel_j.xyz, 0) with
 // vel_j = (pred_j - pos_j) * invDt, the value velFromPos writes.
 template <bool EXACT, bool NORMALS>
-struct XsphOp {
+struct XsphOp : OpBase {
+  static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
   float4* __restrict__ pos;
@@ -617,7 +632,8 @@ struct XsphOp {
 // --- tension (wgsl.js:290-335) + velFromPos/finalize (72-83, 337-350) -------
 // Records: (pred.xyz, rho_j), (normal_j.xyz, 0).
 template <bool EXACT>
-struct TensionOp {
+struct TensionOp : OpBase {
+  static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
   float4* __restrict__ pos;
@@ -688,7 +704,8 @@ struct TensionOp {
 };
 
 // --- neighbour census: |N(i)| and index checksums -------------------------
-struct CensusOp {
+struct CensusOp : OpBase {
+  static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
   uint32_t* __restrict__ cnt;
@@ -712,6 +729,176 @@ struct CensusOp {
   __device__ void bpair(const Uniform&, Target&, float4) const {}
   __device__ void finish(const Uniform&, uint32_t i, Target& t) const {
     cnt[i] = t.c; s1[i] = t.a; s2[i] = t.b;
+  }
+};
+
+// --- anisotropic kernel (aniso_wgsl.js:3-159) -----------------------------
+// Symmetric 3x3 eigensolve by cyclic Jacobi rotations (aniso_wgsl.js:30-55):
+// a <- r^T a r, v <- v r for the pairs (0,1), (0,2), (1,2).  Matrices are
+// row-major here (a[row][col]); the WGSL's a[q][p] (column q, row p) is
+// a[p][q].
+template <class R>
+__device__ void jacobi3(R a[3][3], R v[3][3], int sweeps, R tiny) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) v[i][j] = (i == j) ? (R)1 : (R)0;
+  for (int sweep = 0; sweep < sweeps; sweep++) {
+    for (int p = 0; p < 2; p++) {
+      for (int q = p + 1; q < 3; q++) {
+        const R apq = a[p][q];
+        if (fabs(apq) < tiny) continue;
+        const R theta = (a[q][q] - a[p][p]) / ((R)2 * apq);
+        const R sg = theta > (R)0 ? (R)1 : (theta < (R)0 ? (R)-1 : (R)0);
+        R t = sg / (fabs(theta) + sqrt(theta * theta + (R)1));
+        if (theta == (R)0) t = (R)1;
+        const R c = (R)1 / sqrt(t * t + (R)1);
+        const R s = t * c;
+        R r[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        r[p][p] = c; r[q][q] = c;
+        r[p][q] = s; r[q][p] = -s;
+        R m[3][3], n[3][3];
+        for (int i = 0; i < 3; i++)  // m = r^T a
+          for (int j = 0; j < 3; j++) m[i][j] = r[0][i] * a[0][j] + r[1][i] * a[1][j] + r[2][i] * a[2][j];
+        for (int i = 0; i < 3; i++)  // a = m r
+          for (int j = 0; j < 3; j++) a[i][j] = m[i][0] * r[0][j] + m[i][1] * r[1][j] + m[i][2] * r[2][j];
+        for (int i = 0; i < 3; i++)  // v = v r
+          for (int j = 0; j < 3; j++) n[i][j] = v[i][0] * r[0][j] + v[i][1] * r[1][j] + v[i][2] * r[2][j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) v[i][j] = n[i][j];
+      }
+    }
+  }
+}
+
+template <class R>
+__device__ R det3(const R m[3][3]) {
+  return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+         m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+
+// G from eigenvectors v (columns) and eigenvalues sig (aniso_wgsl.js:133-153).
+template <class R>
+__device__ void aniso_g(const AnisoArgs& A, const R v[3][3], const R sig[3], int n, R G[3][3]) {
+  const R s1 = fmax(sig[0], fmax(sig[1], sig[2]));
+  R ext[3];
+  for (int k = 0; k < 3; k++) ext[k] = (R)A.ks * fmax(sig[k], s1 / (R)A.kr);
+  const R mx = fmax(ext[0], fmax(ext[1], ext[2]));
+  const R dv = fmax((R)1, mx / (R)A.stretch);
+  R inv[3];
+  for (int k = 0; k < 3; k++) {
+    ext[k] = fmax(ext[k] / dv, (R)A.minExtent);
+    inv[k] = (R)1 / ((R)A.support * ext[k]);
+  }
+  const R e0 = (R)A.neps * (R)0.4, e1 = (R)A.neps * (R)1.6;
+  R w = fmin(fmax(((R)n - e0) / (e1 - e0), (R)0), (R)1);
+  w = w * w * ((R)3 - (R)2 * w);
+  const R iso = (R)A.kn / (R)A.support;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      const R an = v[i][0] * inv[0] * v[j][0] + v[i][1] * inv[1] * v[j][1] + v[i][2] * inv[2] * v[j][2];
+      const R is = (i == j) ? iso : (R)0;
+      G[i][j] = is * ((R)1 - w) + an * w;
+    }
+}
+
+template <bool EXACT>
+struct AnisoOp : OpBase {
+  static constexpr bool BOUNDARY = false;
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pos;
+  float4* __restrict__ smoothPos;
+  float4* __restrict__ g;
+  uint32_t* __restrict__ fallbacks;
+  AnisoArgs a;
+  struct Target {
+    float px, py, pz, rx, ry, rz;
+    float sx, sy, sz, m00, m10, m20, m11, m21, m22, wsum;
+    int n;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pos[j]; }
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
+    const float4 p = pos[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.rx = p.x - U.boxMin[0]; t.ry = p.y - U.boxMin[1]; t.rz = p.z - U.boxMin[2];
+    t.sx = t.sy = t.sz = t.m00 = t.m10 = t.m20 = t.m11 = t.m21 = t.m22 = t.wsum = 0.0f;
+    t.n = 0;
+  }
+  // floor((x - boxMin) * invSupport), clamped (aniso_wgsl.js:72-73)
+  __device__ Cell cell(const Uniform& U, const Target& t) const {
+    float f[3] = {floorf(t.rx * a.invSupport), floorf(t.ry * a.invSupport),
+                  floorf(t.rz * a.invSupport)};
+    int c[3];
+    for (int k = 0; k < 3; k++)
+      c[k] = (int)fminf(fmaxf(f[k], 0.0f), (float)(U.gridDim[k] - 1));
+    return Cell{c[0], c[1], c[2]};
+  }
+  // rows whose slab lies beyond the radius (aniso_wgsl.js:88-97)
+  __device__ bool skip_row(const Uniform&, const Target& t, int z, int y) const {
+    const float dzm = fmaxf(fmaxf((float)z * a.support - t.rz, t.rz - (float)(z + 1) * a.support), 0.0f);
+    const float dym = fmaxf(fmaxf((float)y * a.support - t.ry, t.ry - (float)(y + 1) * a.support), 0.0f);
+    const float r2max = a.radius * a.radius;
+    const float dz2 = dzm * dzm;
+    return dz2 >= r2max || dz2 + dym * dym >= r2max;
+  }
+  __device__ void pair(const Uniform&, Target& t, uint32_t, const float4* v) const {
+    const float dx = v[0].x - t.px, dy = v[0].y - t.py, dz = v[0].z - t.pz;
+    const float r2 = r2_of(dx, dy, dz);
+    const float r2max = a.radius * a.radius;
+    if (r2 >= r2max) return;
+    const float invR = 1.0f / a.radius;
+    const float tt = (EXACT ? sqrtf(r2) : __builtin_amdgcn_sqrtf(r2)) * invR;
+    const float w = 1.0f - tt * tt * tt;
+    t.sx = t.sx + dx * w; t.sy = t.sy + dy * w; t.sz = t.sz + dz * w;
+    const float wx = dx * w, wy = dy * w, wz = dz * w;
+    t.m00 = t.m00 + dx * wx; t.m10 = t.m10 + dy * wx; t.m20 = t.m20 + dz * wx;
+    t.m11 = t.m11 + dy * wy; t.m21 = t.m21 + dz * wy;
+    t.m22 = t.m22 + dz * wz;
+    t.wsum = t.wsum + w;
+    t.n += 1;
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform&, uint32_t i, Target& t) const {
+    if (t.wsum <= 0.0f) {
+      smoothPos[i] = make_float4(t.px, t.py, t.pz, 0.0f);
+      const float iso = a.kn / a.support;
+      g[i * 2] = make_float4(iso, 0.0f, 0.0f, iso);
+      g[i * 2 + 1] = make_float4(0.0f, iso, iso * iso * iso, 0.0f);
+      return;
+    }
+    const float bx = t.sx / t.wsum, by = t.sy / t.wsum, bz = t.sz / t.wsum;
+    smoothPos[i] = make_float4(t.px + bx * a.lambda, t.py + by * a.lambda, t.pz + bz * a.lambda, 0.0f);
+    float C[3][3];
+    C[0][0] = t.m00 / t.wsum - bx * bx;
+    C[1][0] = t.m10 / t.wsum - by * bx;
+    C[2][0] = t.m20 / t.wsum - bz * bx;
+    C[1][1] = t.m11 / t.wsum - by * by;
+    C[2][1] = t.m21 / t.wsum - bz * by;
+    C[2][2] = t.m22 / t.wsum - bz * bz;
+    C[0][1] = C[1][0]; C[0][2] = C[2][0]; C[1][2] = C[2][1];
+    float af[3][3], vf[3][3];
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) af[r][c] = C[r][c];
+    jacobi3<float>(af, vf, 6, 1.0e-12f);
+    const float sf[3] = {af[0][0], af[1][1], af[2][2]};
+    const float scale = fmaxf(fabsf(sf[0]), fmaxf(fabsf(sf[1]), fabsf(sf[2])));
+    const float gap = fminf(fabsf(sf[0] - sf[1]), fminf(fabsf(sf[0] - sf[2]), fabsf(sf[1] - sf[2])));
+    float G[3][3];
+    if (a.fp64Fallback && scale > 0.0f && gap < a.gapTol * scale) {
+      // near-degenerate spectrum: solve again in FP64 to convergence
+      double ad[3][3], vd[3][3], Gd[3][3];
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) ad[r][c] = (double)C[r][c];
+      jacobi3<double>(ad, vd, 12, 1.0e-300);
+      const double sd[3] = {ad[0][0], ad[1][1], ad[2][2]};
+      aniso_g<double>(a, vd, sd, t.n, Gd);
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) G[r][c] = (float)Gd[r][c];
+      atomicAdd(fallbacks, 1u);
+    } else {
+      aniso_g<float>(a, vf, sf, t.n, G);
+    }
+    g[i * 2] = make_float4(G[0][0], G[0][1], G[0][2], G[1][1]);
+    g[i * 2 + 1] = make_float4(G[1][2], G[2][2], det3<float>(G), (float)t.n);
   }
 };
 
@@ -1023,13 +1210,13 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
 
-template <class Op, int TX, int NT, int MAXREC>
+template <class Op, int TX, int NT, int MAXREC, int CR = 1>
 hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
   const int gx = U.gridDim[0];
   const int tpr = (gx + TX - 1) / TX;
   const long nt = (long)tpr * U.gridDim[1] * U.gridDim[2];
   if (nt <= 0 || U.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_tile<Op, TX, NT, MAXREC>), dim3((unsigned)nt), dim3(NT), 0, s, U, op,
+  hipLaunchKernelGGL((k_tile<Op, TX, NT, MAXREC, CR>), dim3((unsigned)nt), dim3(NT), 0, s, U, op,
                      A.cellStart, A.bcellStart, A.bpos, tpr);
   return hipGetLastError();
 }
@@ -1087,20 +1274,20 @@ hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveV
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, bool exact,
                          const TileCfg& t, hipStream_t s) {
   if (exact) {
-    LambdaOp<true> op{pred, A.density, A.lambda};
+    LambdaOp<true> op{{}, pred, A.density, A.lambda};
     return run_tile1(U, A, op, t, s);
   }
-  LambdaOp<false> op{pred, A.density, A.lambda};
+  LambdaOp<false> op{{}, pred, A.density, A.lambda};
   return run_tile1(U, A, op, t, s);
 }
 
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
                         bool exact, const TileCfg& t, hipStream_t s) {
   if (exact) {
-    DeltaOp<true> op{pred, A.lambda, out};
+    DeltaOp<true> op{{}, pred, A.lambda, out};
     return run_tile1(U, A, op, t, s);
   }
-  DeltaOp<false> op{pred, A.lambda, out};
+  DeltaOp<false> op{{}, pred, A.lambda, out};
   return run_tile1(U, A, op, t, s);
 }
 
@@ -1124,17 +1311,17 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
                        bool exact, bool normals, const TileCfg& t, hipStream_t s) {
   float4* p = const_cast<float4*>(pos);
   if (normals) {
-    if (exact) return run_tile2(U, A, XsphOp<true, true>{pred, p, A.density, A.corr, A.normal}, t, s);
-    return run_tile2(U, A, XsphOp<false, true>{pred, p, A.density, A.corr, A.normal}, t, s);
+    if (exact) return run_tile2(U, A, XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+    return run_tile2(U, A, XsphOp<false, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
   }
-  if (exact) return run_tile2(U, A, XsphOp<true, false>{pred, p, A.density, A.corr, A.normal}, t, s);
-  return run_tile2(U, A, XsphOp<false, false>{pred, p, A.density, A.corr, A.normal}, t, s);
+  if (exact) return run_tile2(U, A, XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+  return run_tile2(U, A, XsphOp<false, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, hipStream_t s) {
-  if (exact) return run_tile2(U, A, TensionOp<true>{pred, pos, A.density, A.normal, A.corr, vel}, t, s);
-  return run_tile2(U, A, TensionOp<false>{pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+  if (exact) return run_tile2(U, A, TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+  return run_tile2(U, A, TensionOp<false>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
@@ -1176,8 +1363,20 @@ hipError_t launch_resize(const ResizeArgs& r, float4* pos, hipStream_t s) {
 
 hipError_t launch_census(const Uniform& U, const Arrays& A, const float4* pred,
                          const TileCfg& t, hipStream_t s) {
-  CensusOp op{pred, A.census[0], A.census[1], A.census[2]};
+  CensusOp op{{}, pred, A.census[0], A.census[1], A.census[2]};
   return run_tile1(U, A, op, t, s);
+}
+
+hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
+                        const AnisoArgs& a, float4* smoothPos, float4* g, uint32_t* fallbacks,
+                        bool exact, hipStream_t s) {
+  // (2*2+1)^2 rows of 16+4 cells at ~8 particles per cell: 3584 records
+  if (exact) {
+    AnisoOp<true> op{{}, pos, smoothPos, g, fallbacks, a};
+    return run_tile<AnisoOp<true>, 16, 128, 3584, 2>(U, A, op, s);
+  }
+  AnisoOp<false> op{{}, pos, smoothPos, g, fallbacks, a};
+  return run_tile<AnisoOp<false>, 16, 128, 3584, 2>(U, A, op, s);
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {

@@ -69,6 +69,16 @@ struct Arrays {
   uint32_t* census[3];
 };
 
+// AnisoParams uniform (aniso_wgsl.js:4-21) as mesh.js:221-242 fills it,
+// plus the FP64 fallback switch.
+struct AnisoArgs {
+  float lambda, invSupport, support, radius;
+  float kr, kn, ks, stretch, minExtent;
+  int neps;
+  int fp64Fallback;
+  float gapTol;  // relative eigenvalue gap below which the FP64 solve runs
+};
+
 // Launch configuration of the tiled neighbour kernels.
 struct TileCfg {
   int tx;       // cells per tile along x
@@ -127,6 +137,9 @@ hipError_t launch_impulse(const float4* pos, float4* vel, const RayArgs& r, hipS
 hipError_t launch_resize(const ResizeArgs& r, float4* pos, hipStream_t s);
 hipError_t launch_census(const Uniform& U, const Arrays& A, const float4* pred,
                          const TileCfg& t, hipStream_t s);
+hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
+                        const AnisoArgs& a, float4* smoothPos, float4* g, uint32_t* fallbacks,
+                        bool exact, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 int stats_blocks(uint32_t n);

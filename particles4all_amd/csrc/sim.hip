@@ -77,6 +77,12 @@ struct p4a_sim {
   size_t eventNext = 0;
   // tile shapes
   TileCfg tile1{32, 256}, tile2{16, 128};
+  // anisotropy outputs (allocated on first use)
+  float4* smoothPos = nullptr;
+  float4* anisoG = nullptr;
+  uint32_t* anisoFallbacks = nullptr;
+  int anisoCap = 0;
+  std::vector<void*> anisoAllocs;
 };
 
 namespace {
@@ -429,6 +435,8 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_KEY: return {s->A.key, n * 4};
     case P4A_BUF_ORDER: return {s->A.order, n * 4};
     case P4A_BUF_BPSI: return {s->A.bpsi, s->bposCount * 4};
+    case P4A_BUF_SMOOTH_POS: return {s->smoothPos, s->smoothPos ? n * 16 : 0};
+    case P4A_BUF_ANISO_G: return {s->anisoG, s->anisoG ? n * 32 : 0};
     default: return {nullptr, 0};
   }
 }
@@ -481,6 +489,7 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->devAllocs);
   free_list(s->cellAllocs);
   free_list(s->bndAllocs);
+  free_list(s->anisoAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
   hipStreamDestroy(s->stream);
@@ -507,6 +516,11 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   free_list(s->devAllocs);
   free_list(s->cellAllocs);
   free_list(s->bndAllocs);
+  free_list(s->anisoAllocs);
+  s->smoothPos = nullptr;
+  s->anisoG = nullptr;
+  s->anisoFallbacks = nullptr;
+  s->anisoCap = 0;
   s->ready = false;
   s->params = *p;
   s->h = sc->h;
@@ -992,6 +1006,63 @@ P4A_API int p4a_body_pose(p4a_sim* s, float* centre, float* rot) {
     if (rot)
       for (int row = 0; row < 3; row++)
         for (int col = 0; col < 3; col++) rot[b * 9 + col * 3 + row] = r[(b * 3 + row) * 4 + col];
+  }
+  return P4A_OK;
+}
+
+P4A_API int p4a_aniso_params_default(p4a_aniso_params* a) {
+  if (!a) return P4A_ERR_ARG;
+  std::memset(a, 0, sizeof(*a));
+  a->radius_scale = 2.0;
+  a->lambda = 0.9;
+  a->kr = 4.0;
+  a->kn = 1.0;
+  a->ks = 1.0;
+  a->stretch = 2.0;
+  a->min_extent = 0.0;
+  a->neps = 25;
+  a->fp64_fallback = 1;
+  a->gap_tol = 1e-4;
+  return P4A_OK;
+}
+
+P4A_API int p4a_aniso(p4a_sim* s, const p4a_aniso_params* a, int32_t* fp64_solves) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!a) return P4A_ERR_ARG;
+  if (!(a->radius_scale > 0) || a->radius_scale > 2.0)
+    return fail(s, P4A_ERR_ARG, "radius_scale must be in (0, 2]: the pass walks +-2 cells");
+  HIP_TRY(s, hipSetDevice(s->device));
+  if (s->anisoCap < s->cap) {
+    free_list(s->anisoAllocs);
+    if ((rc = dalloc(s, s->anisoAllocs, &s->smoothPos, (size_t)s->cap))) return rc;
+    if ((rc = dalloc(s, s->anisoAllocs, &s->anisoG, (size_t)s->cap * 2))) return rc;
+    if ((rc = dalloc(s, s->anisoAllocs, &s->anisoFallbacks, 1))) return rc;
+    s->anisoCap = s->cap;
+  }
+  const Uniform U = make_uniform(s, 0.0);
+  AnisoArgs A;
+  const double support = s->h, radius = a->radius_scale * support;
+  A.lambda = (float)a->lambda;
+  A.invSupport = (float)(1.0 / support);
+  A.support = (float)support;
+  A.radius = (float)radius;
+  A.kr = (float)a->kr;
+  A.kn = (float)a->kn;
+  A.ks = (float)(a->ks / (0.15 * radius * radius));
+  A.stretch = (float)a->stretch;
+  A.minExtent = (float)a->min_extent;
+  A.neps = a->neps;
+  A.fp64Fallback = a->fp64_fallback;
+  A.gapTol = (float)a->gap_tol;
+  LAUNCH(s, "aniso_clear", launch_fill_u32(s->anisoFallbacks, 0u, 1, s->stream));
+  LAUNCH(s, "aniso", launch_aniso(U, s->A, s->A.pos[s->parity], A, s->smoothPos, s->anisoG,
+                                  s->anisoFallbacks, exact_math(s), s->stream));
+  if (fp64_solves) {
+    uint32_t c = 0;
+    HIP_TRY(s, hipMemcpyAsync(&c, s->anisoFallbacks, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    *fp64_solves = (int32_t)c;
   }
   return P4A_OK;
 }

@@ -1,0 +1,310 @@
+"""GPU parity: every kernel group of the substep against the oracle on
+identical inputs (the GPU's own inputs are read back and handed to the
+oracle), through the C ABI.
+
+Bars:
+  * integer outputs (cell ids, cellStart, stable permutation, neighbour
+    census) are bit-exact;
+  * in exact-math mode (IEEE sqrt/division, no contraction) every float
+    output equals the oracle's float mode bit for bit;
+  * in fast mode (hardware rsq/rcp) float outputs agree with the FP64
+    oracle within 1e-5*h for positions (or one float ulp of the coordinate,
+    whichever is larger), 1e-5*h/dt for velocities and 1e-5*max|lambda| for
+    lambda; max and RMS errors are printed;
+  * rigid-body stages (FP64 on both sides) agree within one float ulp of the
+    coordinate plus 1e-5*h.
+"""
+import numpy as np
+import pytest
+
+from particles4all_amd import Sim, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F = np.float64
+
+
+def rd(sim, name):
+    return sim.read(name).astype(F)
+
+
+def scene_sim(maker=workloads.dam_break_32, math=1, **kw):
+    p, sc, frame = maker(math=math, **kw)
+    sim = Sim(0)
+    sim.reset(p, sc)
+    return sim, p, sc
+
+
+def oracle_boundary(sim):
+    bpos = rd(sim, "BPOS")
+    return bpos, bpos[:, 3].copy(), sim.read("BCELL_START")
+
+
+def test_grid_build_bit_exact():
+    sim, p, sc = scene_sim(scale=0.5)
+    dt = 1 / 120
+    P = O.constants_from(p, sc, dt)
+    for rep in range(3):
+        pos0, vel0, pid0 = rd(sim, "POS"), rd(sim, "VEL"), sim.read("PID")
+        sim.run_stage("PREDICT", dt)
+        pred, _ = O.predict(O.F32, P, pos0, vel0)
+        sim.run_stage("GRID", dt)
+        key = O.cell_ids(O.F32, P, pred)
+        cs, slot, order = O.grid(P, key)
+        assert np.array_equal(sim.read("KEY"), key)
+        gcs = sim.read("CELL_START")
+        assert np.array_equal(gcs[: len(cs)], cs)
+        assert np.all(np.diff(gcs[: len(cs)].astype(np.int64)) >= 0) and gcs[len(cs) - 1] == sc.n
+        assert np.array_equal(sim.read("SLOT"), slot)
+        assert np.array_equal(sim.read("ORDER"), order)
+        assert np.array_equal(rd(sim, "PRED"), pred[order])
+        assert np.array_equal(rd(sim, "POS"), pos0[order])
+        assert np.array_equal(sim.read("PID"), pid0[order])
+        # run the rest of the substep so the next grid build sees motion
+        for _ in range(p.iterations):
+            sim.run_stage("LAMBDA", dt)
+            sim.run_stage("DELTA", dt)
+        sim.run_stage("FINISH", dt)
+    # walls: sortBoundary restated by the oracle
+    rec, start = O.sort_boundary(P, sc.boundary_pts, sc.boundary_psi)
+    assert np.array_equal(rd(sim, "BPOS"), rec)
+    assert np.array_equal(sim.read("BCELL_START"), start)
+    sim.close()
+
+
+def _stage_inputs(sim, p, sc, dt):
+    sim.run_stage("PREDICT", dt)
+    sim.run_stage("GRID", dt)
+    return O.constants_from(p, sc, dt)
+
+
+@pytest.mark.parametrize("scale", [0.5, 1.0])
+def test_solver_stages_bit_exact_in_exact_math(scale):
+    sim, p, sc = scene_sim(scale=scale, bodies=False)
+    dt = 1 / 120
+    P = _stage_inputs(sim, p, sc, dt)
+    bpos, bpsi, bcs = oracle_boundary(sim)
+    cs = sim.read("CELL_START")
+    for it in range(p.iterations):
+        pred = rd(sim, "PRED")
+        sim.run_stage("LAMBDA", dt)
+        den, lam = O.lambda_(O.F32, P, pred, cs, bpos, bpsi, bcs)
+        assert np.array_equal(rd(sim, "DENSITY"), den), f"density, iteration {it}"
+        assert np.array_equal(rd(sim, "LAMBDA"), lam), f"lambda, iteration {it}"
+        sim.run_stage("DELTA", dt)
+        out = O.delta(O.F32, P, pred, rd(sim, "LAMBDA"), cs, bpos, bpsi, bcs)
+        assert np.array_equal(rd(sim, "PRED"), out), f"delta, iteration {it}"
+    # finish: velFromPos, xsph, normals, tension, finalize
+    pred, pos, den = rd(sim, "PRED"), rd(sim, "POS"), rd(sim, "DENSITY")
+    tension = p.surface_tension_k > 0
+    vel, corr, normal = O.xsph(O.F32, P, pred, pos, den, cs, tension)
+    if tension:
+        corr = O.tension(O.F32, P, pred, den, normal, cs, corr)
+    vel, pos2 = O.finalize(O.F32, pred, corr, vel)
+    sim.run_stage("FINISH", dt)
+    assert np.array_equal(rd(sim, "CORR"), corr)
+    if tension:
+        assert np.array_equal(rd(sim, "NORMAL"), normal)
+    assert np.array_equal(rd(sim, "VEL"), vel)
+    assert np.array_equal(rd(sim, "POS"), pos2)
+    sim.close()
+
+
+def test_finish_with_tension_bit_exact():
+    sim, p, sc = scene_sim(workloads.block_1m, scale=0.125)
+    assert p.surface_tension_k > 0
+    dt = 1 / 120
+    P = _stage_inputs(sim, p, sc, dt)
+    bpos, bpsi, bcs = oracle_boundary(sim)
+    cs = sim.read("CELL_START")
+    for it in range(p.iterations):
+        sim.run_stage("LAMBDA", dt)
+        sim.run_stage("DELTA", dt)
+    pred, pos, den = rd(sim, "PRED"), rd(sim, "POS"), rd(sim, "DENSITY")
+    vel, corr, normal = O.xsph(O.F32, P, pred, pos, den, cs, True)
+    corr = O.tension(O.F32, P, pred, den, normal, cs, corr)
+    vel, pos2 = O.finalize(O.F32, pred, corr, vel)
+    sim.run_stage("FINISH", dt)
+    assert np.array_equal(rd(sim, "NORMAL"), normal)
+    assert np.array_equal(rd(sim, "CORR"), corr)
+    assert np.array_equal(rd(sim, "VEL"), vel)
+    assert np.array_equal(rd(sim, "POS"), pos2)
+    sim.close()
+
+
+def test_neighbour_census_bit_exact():
+    sim, p, sc = scene_sim(scale=1.0)
+    dt = 1 / 120
+    P = _stage_inputs(sim, p, sc, dt)
+    for it in range(3):
+        pred = rd(sim, "PRED")
+        cnt, s1, s2 = sim.census()
+        a, b, c = O.census(P, pred, sim.read("CELL_START"))
+        assert np.array_equal(cnt, a) and np.array_equal(s1, b) and np.array_equal(s2, c)
+        assert cnt.mean() > 20
+        sim.run_stage("LAMBDA", dt)
+        sim.run_stage("DELTA", dt)  # stale grid, moved particles
+    sim.close()
+
+
+def _err(a, b):
+    d = np.abs(a - b)
+    return float(d.max()), float(np.sqrt((d ** 2).mean()))
+
+
+def test_fast_math_stages_within_tolerance():
+    sim, p, sc = scene_sim(math=0, bodies=False)
+    dt = 1 / 120
+    h = sc.h
+    P = _stage_inputs(sim, p, sc, dt)
+    bpos, bpsi, bcs = oracle_boundary(sim)
+    cs = sim.read("CELL_START")
+    pred = rd(sim, "PRED")
+    sim.run_stage("LAMBDA", dt)
+    den, lam = O.lambda_(O.F64, P, pred, cs, bpos, bpsi, bcs)
+    glam = rd(sim, "LAMBDA")
+    e_l = _err(glam, lam)
+    print(f"lambda vs FP64: max {e_l[0]:.3e} rms {e_l[1]:.3e} (max|lambda| {np.abs(lam).max():.3e})")
+    assert e_l[0] <= 1e-5 * np.abs(lam).max()
+    np.testing.assert_allclose(rd(sim, "DENSITY"), den, rtol=2e-6)
+    sim.run_stage("DELTA", dt)
+    out = O.delta(O.F64, P, pred, glam, cs, bpos, bpsi, bcs)
+    g = rd(sim, "PRED")
+    e_p = _err(g[:, :3], out[:, :3])
+    tol = np.maximum(1e-5 * h, np.spacing(np.abs(out[:, :3]).astype(np.float32)).astype(F))
+    print(f"delta vs FP64: max {e_p[0]:.3e} rms {e_p[1]:.3e} (1e-5 h = {1e-5 * h:.1e})")
+    assert np.all(np.abs(g[:, :3] - out[:, :3]) <= tol)
+    sim.close()
+
+
+def test_body_stages_against_fp64_oracle():
+    sim, p, sc = scene_sim(scale=1.0, math=1)
+    dt = 1 / 120
+    h = sc.h
+    P = _stage_inputs(sim, p, sc, dt)
+    body_of = sim.read("BODY")[:, 0]
+    rest = rd(sim, "REST")
+    inv = 1.0 / np.maximum(1e-6, sc.body_density)
+    inv = inv.astype(np.float32).astype(F)
+    centre = np.array(sc.body_centre, F)
+    pred = rd(sim, "PRED")
+    centre = O.body_pre(pred, body_of, centre)
+    sim.run_stage("BODY_PRE", dt)
+    gc = rd(sim, "BODY_CENTRE")[:, :3]
+    np.testing.assert_allclose(gc, centre, atol=2 * np.spacing(np.float32(64.0)))
+    rot = np.tile(np.eye(3).reshape(1, 9), (len(centre), 1))
+    for it in range(p.iterations):
+        sim.run_stage("LAMBDA", dt)
+        sim.run_stage("DELTA", dt)
+        pred = rd(sim, "PRED")
+        sim.run_stage("BODY_SOLVE", dt)
+        out, centre, rot = O.body_solve(pred, body_of, rest, inv, centre, rot)
+        g = rd(sim, "PRED")
+        sel = body_of > 0
+        tol = 1e-5 * h + np.spacing(np.abs(out[sel, :3]).astype(np.float32)).astype(F)
+        err = np.abs(g[sel, :3] - out[sel, :3])
+        print(f"body iteration {it}: max {err.max():.3e}")
+        assert np.all(err <= tol)
+        assert np.array_equal(g[~sel], pred[~sel])  # fluid untouched
+        gr = rd(sim, "BODY_ROT")[:, :3].reshape(-1, 9)
+        np.testing.assert_allclose(gr, rot, atol=1e-6)
+    sim.close()
+
+
+def test_graph_replay_equals_stage_by_stage():
+    simA, p, sc = scene_sim(scale=0.5, math=0)
+    simB, _, _ = scene_sim(scale=0.5, math=0)
+    dt = 1 / 120
+    simA.substeps(dt, 6)   # captured graphs, both parities, replayed
+    for _ in range(6):
+        simB.run_stage("PREDICT", dt)
+        simB.run_stage("GRID", dt)
+        if sc.n_bodies:
+            simB.run_stage("BODY_PRE", dt)
+        for _ in range(p.iterations):
+            simB.run_stage("LAMBDA", dt)
+            simB.run_stage("DELTA", dt)
+            if sc.n_bodies:
+                simB.run_stage("BODY_SOLVE", dt)
+        simB.run_stage("FINISH", dt)
+    for name in ("POS", "VEL", "PID", "DENSITY", "LAMBDA"):
+        assert np.array_equal(simA.read(name), simB.read(name)), name
+    simA.close()
+    simB.close()
+
+
+def _substep_pair(math):
+    sim, p, sc = scene_sim(math=math)
+    dt = 1 / 120
+    P = O.constants_from(p, sc, dt)
+    st = O.State(rd(sim, "POS"), rd(sim, "VEL"), sim.read("BODY")[:, 0], rd(sim, "REST"))
+    pid0 = sim.read("PID")
+    inv = (1.0 / np.maximum(1e-6, sc.body_density)).astype(np.float32).astype(F)
+    bnd = oracle_boundary(sim)
+    rot = np.tile(np.eye(3).reshape(1, 9), (sc.n_bodies, 1))
+    return sim, p, sc, dt, P, st, pid0, inv, bnd, rot
+
+
+def test_one_substep_bit_exact_against_float_oracle():
+    """A whole substep of the 33,792-particle dam break with its two
+    bodies, exact math, against the oracle's float chain: positions,
+    velocities, lambdas and densities agree bit for bit."""
+    sim, p, sc, dt, P, st, pid0, inv, bnd, rot = _substep_pair(1)
+    info = O.substep(O.F32, P, st, bnd, np.array(sc.body_centre, F), rot, inv, p.iterations,
+                     p.surface_tension_k > 0)
+    sim.substeps(dt, 1)
+    assert np.array_equal(sim.read("PID"), pid0[info["order"]])
+    assert np.array_equal(rd(sim, "POS"), st.pos)
+    assert np.array_equal(rd(sim, "VEL")[:, :3], st.vel[:, :3])
+    assert np.array_equal(rd(sim, "LAMBDA"), info["lam"])
+    assert np.array_equal(rd(sim, "DENSITY"), info["density"])
+    sim.close()
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_one_substep_against_fp64_oracle(math):
+    """The same substep against the FP64 oracle: max and RMS errors are
+    reported; the bound is the float conditioning of four Jacobi-style
+    solver iterations (1e-4 h), with the bulk within 1e-5 h."""
+    sim, p, sc, dt, P, st, pid0, inv, bnd, rot = _substep_pair(math)
+    h = sc.h
+    info = O.substep(O.F64, P, st, bnd, np.array(sc.body_centre, F), rot, inv, p.iterations,
+                     p.surface_tension_k > 0)
+    sim.substeps(dt, 1)
+    assert np.array_equal(sim.read("PID"), pid0[info["order"]])
+    gp, gv, gl = rd(sim, "POS")[:, :3], rd(sim, "VEL")[:, :3], rd(sim, "LAMBDA")
+    ep, ev, el = _err(gp, st.pos[:, :3]), _err(gv, st.vel[:, :3]), _err(gl, info["lam"])
+    frac = float(np.mean(np.abs(gp - st.pos[:, :3]) <= 1e-5 * h))
+    print(f"math={math}: pos max {ep[0]:.3e} rms {ep[1]:.3e} (within 1e-5 h: {frac:.6f}); "
+          f"vel max {ev[0]:.3e} rms {ev[1]:.3e}; lambda max {el[0]:.3e} rms {el[1]:.3e}")
+    assert ep[0] <= 1e-4 * h and ep[1] <= 1e-6 * h and frac > 0.999
+    assert ev[0] <= 1e-4 * h / dt
+    assert el[0] <= 1e-4 * np.abs(info["lam"]).max()
+    sim.close()
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_aniso_against_oracle(math):
+    """Anisotropy pass on a moved dam break (stale grid, live positions):
+    neighbour counts exact, smoothed centres and G against the FP64 oracle."""
+    sim, p, sc = scene_sim(math=math, bodies=False)
+    dt = 1 / 120
+    sim.substeps(dt, 4)
+    P = O.constants_from(p, sc, dt)
+    pos, cs = rd(sim, "POS"), sim.read("CELL_START")
+    fb = sim.aniso()
+    gs, gg = rd(sim, "SMOOTH_POS"), rd(sim, "ANISO_G").reshape(-1, 8)
+    s64, g64 = O.aniso(O.F64, P, pos, cs)
+    s32, g32 = O.aniso(O.F32, P, pos, cs)
+    g64, g32 = g64.reshape(-1, 8), g32.reshape(-1, 8)
+    assert np.array_equal(gg[:, 7], g32[:, 7])  # counts (integers)
+    es = _err(gs[:, :3], s64[:, :3])
+    scale = np.abs(g64[:, [0, 3, 5]]).max()
+    eg = _err(gg[:, :6], g64[:, :6])
+    print(f"aniso math={math}: smoothPos max {es[0]:.3e}; G max {eg[0]:.3e} rms {eg[1]:.3e} "
+          f"(|G| {scale:.3e}); FP64 re-solves {fb}")
+    assert es[0] <= 1e-5 * sc.h
+    assert eg[0] <= 2e-4 * scale
+    np.testing.assert_allclose(gg[:, 6], g64[:, 6], rtol=1e-3, atol=1e-6 * scale ** 3)
+    sim.close()

@@ -333,3 +333,39 @@ def test_boundary_sort_is_stable_counting_sort():
         seg = rec[start[c]:start[c + 1], :3]
         cc = np.clip(np.floor(seg / P.h), 0, np.array(g) - 1).astype(int)
         assert np.all((cc[:, 2] * g[1] + cc[:, 1]) * g[0] + cc[:, 0] == c)
+
+
+def test_jacobi_matches_numpy_eigh():
+    rng = np.random.default_rng(6)
+    for mode, tol in ((O.F64, 1e-10), (O.F32, 2e-5)):
+        for _ in range(200):
+            m = rng.normal(size=(3, 3))
+            a = m @ m.T
+            w, v = O.jacobi(mode, a)
+            ref = np.linalg.eigh(a)[0]
+            np.testing.assert_allclose(np.sort(w), ref, atol=tol * np.abs(ref).max())
+            # the columns of v are orthonormal eigenvectors: a v = v diag(w)
+            np.testing.assert_allclose(a @ v, v * w, atol=10 * tol * np.abs(ref).max())
+
+
+def test_aniso_on_a_lattice_is_isotropic_with_closed_form_count():
+    P = make_P(box=(16.0, 16.0, 16.0))
+    g = np.arange(16) + 0.5
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    key, cs, slot, order = fresh_grid(P, float4(pts), O.F64)
+    ps = float4(pts[order])
+    smooth, G = O.aniso(O.F64, P, ps, cs)
+    G = G.reshape(-1, 8)
+    R = 2 * P.h  # radius_scale 2
+    want = sum(1 for a in range(-5, 6) for b in range(-5, 6) for c in range(-5, 6)
+               if a * a + b * b + c * c < R * R)
+    inner = np.all((ps[:, :3] > R + 0.5) & (ps[:, :3] < 16 - R - 0.5), axis=1)
+    assert inner.sum() > 50
+    assert np.all(G[inner, 7] == want)             # neighbour count incl. self
+    np.testing.assert_allclose(smooth[inner, :3], ps[inner, :3], atol=1e-12)  # no drift
+    off = np.abs(G[inner][:, [1, 2, 4]]).max()
+    diag = G[inner][:, [0, 3, 5]]
+    assert off < 1e-12 * diag.max()                # G is a multiple of I
+    np.testing.assert_allclose(diag, diag[:, :1] * np.ones((1, 3)), rtol=1e-12)
+    np.testing.assert_allclose(G[inner, 6], diag[:, 0] ** 3, rtol=1e-12)

@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the device copy probe on MIB mebibytes and exit")
     return ap.parse_args()
 
 
@@ -84,6 +86,14 @@ def main():
                                    "(the multi-GPU slab decomposition is design-only)"}))
         sys.exit(2)
     from particles4all_amd import Sim, workloads
+    from particles4all_amd._lib import copy_probe
+
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        ms = copy_probe(nbytes, reps=5)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms,
+                          "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
 
     t0 = time.time()
     p, sc, dt = workloads.WORKLOADS[args.workload](scale=args.scale)
@@ -91,9 +101,17 @@ def main():
     sim = Sim(0)
     sim.reset(p, sc)
     n = sim.info().n
+    copy_ms = copy_probe(1 << 30, reps=5)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"device copy probe: {copy_gbs:.0f} GB/s (read+write) on 1 GiB")
     frame = 1.0 / (60.0 * max(1, p.substeps))  # one substep per step()
     for _ in range(args.warmup):
         sim.step(frame)
+    sim.sync()
+    chosen, tune_ms = sim.tune()
+    log("autotuner: " + ", ".join(f"{k} {v:.3f}ms" for k, v in tune_ms.items()) +
+        f" -> {chosen}")
+    sim.step(frame)  # captures the graphs with the chosen shapes
     sim.sync()
     log("warmup done")
     t0 = time.perf_counter()
@@ -140,8 +158,9 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,
                      "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
-                     "mean_launch_ms": dom_ms},
+                     "mean_launch_ms": dom_ms, "copy_probe_GBps": copy_gbs},
         "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()},
+        "tile_shapes": chosen,
         "fp16_offset": None,
     }
     if not args.no_cpu_baseline:

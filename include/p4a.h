@@ -335,6 +335,22 @@ P4A_API int p4a_set_timing(p4a_sim* h, int32_t on);
 P4A_API int p4a_timing(p4a_sim* h, char* names, size_t names_bytes, double* ms,
                        int32_t* launches, int32_t capacity, int32_t* count);
 
+/* Device copy probe: copies `bytes` (a multiple of 16) between two fresh
+ * device buffers `reps` times with the library's float4 copy kernel and
+ * returns the mean ms per copy.  Calibrates hardware byte counters and
+ * gives the attainable streaming bandwidth; no counterpart in the
+ * reference. */
+P4A_API int p4a_copy_probe(int device, size_t bytes, int32_t reps, double* ms);
+
+/* Autotuner: times the neighbour kernels' tile shapes on the live scene
+ * (lambda for the one-record kernels, xsph+normals for the two-record
+ * ones) and keeps the fastest for later steps.  chosen[0..1] receive the
+ * variant ids; ms (capacity entries) the mean ms of every variant tried,
+ * group 1 first.  The particle state is not changed. */
+P4A_API int p4a_tune(p4a_sim* h, int32_t chosen[2], double* ms, int32_t capacity);
+/* Name of tile variant v of group 1 or 2. */
+P4A_API const char* p4a_tile_variant_name(int32_t group, int32_t v);
+
 /* Library version string. */
 P4A_API const char* p4a_version(void);
 

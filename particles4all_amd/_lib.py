@@ -90,7 +90,8 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_apply_ray_impulse", "p4a_hold_body", "p4a_release_body", "p4a_read",
            "p4a_write", "p4a_device_ptr", "p4a_get_info", "p4a_stats", "p4a_body_pose",
            "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
-           "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso"]
+           "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso",
+           "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name"]
 
 
 def lib():
@@ -137,6 +138,10 @@ def lib():
     L.p4a_timing.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
                              C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
     L.p4a_version.restype = C.c_char_p
+    L.p4a_tune.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int32]
+    L.p4a_tile_variant_name.argtypes = [C.c_int32, C.c_int32]
+    L.p4a_tile_variant_name.restype = C.c_char_p
+    L.p4a_copy_probe.argtypes = [C.c_int, C.c_size_t, C.c_int32, C.POINTER(C.c_double)]
     L.p4a_aniso_params_default.argtypes = [C.POINTER(AnisoParams)]
     L.p4a_aniso.argtypes = [vp, C.POINTER(AnisoParams), C.POINTER(C.c_int32)]
     _lib = L
@@ -149,6 +154,15 @@ def _ptr(a):
 
 def _dvec(v):
     return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def copy_probe(nbytes, reps=5, device=0):
+    """Mean ms of one device-to-device copy of `nbytes` (float4 kernel)."""
+    ms = C.c_double()
+    rc = lib().p4a_copy_probe(device, nbytes, reps, C.byref(ms))
+    if rc:
+        raise RuntimeError(f"p4a_copy_probe failed: {ERRORS.get(rc, rc)}")
+    return ms.value
 
 
 def default_params(**kw):
@@ -379,6 +393,20 @@ class Sim:
         cnt = C.c_int32()
         self._check(lib().p4a_aniso(self.h, C.byref(a), C.byref(cnt)), "aniso")
         return cnt.value
+
+    def tune(self):
+        """Autotune the tile shapes; returns (chosen names, {name: ms})."""
+        ch = (C.c_int32 * 2)()
+        ms = (C.c_double * 16)()
+        self._check(lib().p4a_tune(self.h, ch, ms, 16), "tune")
+        L = lib()
+        times, k = {}, 0
+        for g in (1, 2):
+            for v in range(5):
+                times[f"g{g}:" + L.p4a_tile_variant_name(g, v).decode()] = ms[k]
+                k += 1
+        return ([L.p4a_tile_variant_name(1, ch[0]).decode(),
+                 L.p4a_tile_variant_name(2, ch[1]).decode()], times)
 
     def device_ptr(self, name):
         p = C.c_void_p()

@@ -263,7 +263,7 @@ template <int NREC, int TX, int MAXREC, int CR>
 struct TileLds {
   static constexpr int ROWS = (2 * CR + 1) * (2 * CR + 1);
   static constexpr int W = TX + 2 * CR + 1;
-  float4 rec[NREC][MAXREC];
+  float4 rec[NREC][MAXREC + 4];
   uint32_t cs[ROWS][W];     // cellStart of cells xa..xb of each row
   int off[ROWS + 1];        // LDS offset of each row's first record
   uint32_t base[ROWS];      // global index of each row's first record
@@ -373,18 +373,44 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
           const int rel = S.off[r] - (int)S.base[r];
           const int b = (int)S.cs[r][xlo - xa] + rel;
           const int e = (int)S.cs[r][xhi + 1 - xa] + rel;
-          for (int q = b; q < e; q++) {
-            float4 v[Op::NREC];
+          if constexpr (Op::FAST) {
+            // branchless, four records in flight; the tail pads with
+            // records that lie beyond the support and weigh nothing
+            int q = b;
+            for (; q + 3 < e; q += 4) {
+              float4 v[4][Op::NREC];
 #pragma unroll
-            for (int k = 0; k < Op::NREC; k++) v[k] = S.rec[k][q];
-            op.pair(U, t, (uint32_t)(q - rel), v);
+              for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int k = 0; k < Op::NREC; k++) v[u][k] = S.rec[k][q + u];
+#pragma unroll
+              for (int u = 0; u < 4; u++) op.fpair(U, t, v[u]);
+            }
+            if (q < e) {
+              float4 v[3][Op::NREC];
+#pragma unroll
+              for (int u = 0; u < 3; u++)
+#pragma unroll
+                for (int k = 0; k < Op::NREC; k++)
+                  v[u][k] = (q + u < e) ? S.rec[k][q + u] : Op::sentinel(k);
+#pragma unroll
+              for (int u = 0; u < 3; u++) op.fpair(U, t, v[u]);
+            }
+          } else {
+            for (int q = b; q < e; q++) {
+              float4 v[Op::NREC];
+#pragma unroll
+              for (int k = 0; k < Op::NREC; k++) v[k] = S.rec[k][q];
+              op.pair(U, t, (uint32_t)(q - rel), v);
+            }
           }
         } else {
           const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
           for (uint32_t j = b; j < e; j++) {
             float4 v[Op::NREC];
             op.load(U, j, v);
-            op.pair(U, t, j, v);
+            if constexpr (Op::FAST) op.fpair(U, t, v);
+            else op.pair(U, t, j, v);
           }
         }
         if (doB) {
@@ -400,6 +426,7 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
 template <bool EXACT>
 struct LambdaOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -482,6 +509,7 @@ struct LambdaOp : OpBase {
 // --- delta (wgsl.js:545-612) -----------------------------------------------
 template <bool EXACT>
 struct DeltaOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -567,6 +595,7 @@ struct DeltaOp : OpBase {
 // vel_j = (pred_j - pos_j) * invDt, the value velFromPos writes.
 template <bool EXACT, bool NORMALS>
 struct XsphOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -633,6 +662,7 @@ struct XsphOp : OpBase {
 // Records: (pred.xyz, rho_j), (normal_j.xyz, 0).
 template <bool EXACT>
 struct TensionOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -703,8 +733,264 @@ struct TensionOp : OpBase {
   }
 };
 
+// --- fast-math forms ---------------------------------------------------------
+// The same sums as the exact kernels, reorganised for throughput: every
+// candidate is evaluated without branches (weights vanish outside the
+// support through clamps and selects), constant factors (poly6, spiky,
+// volume, mass) are applied once per particle, sqrt/division pairs use the
+// hardware reciprocal square root, and the self term of the density is
+// taken inside the sum (r = 0 contributes exactly W(0)).
+constexpr float kFar = 1.0e18f;  // tail padding: r^2 ~ 1e36 stays finite
+
+struct FastBase : OpBase {
+  static constexpr bool FAST = true;
+  __device__ static float4 sentinel(int k) {
+    return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+};
+
+struct LambdaFast : FastBase {
+  static constexpr bool BOUNDARY = true;
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pred;
+  float* __restrict__ density;
+  float* __restrict__ lambda;
+  struct Target {
+    float px, py, pz;
+    float st3, gx, gy, gz, sg2;   // fluid: sum t^3, sum s r, sum s^2 r^2
+    float rb, bx, by, bz;         // walls: sum psi t^3, sum psi s r
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    const float4 p = pred[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.st3 = t.gx = t.gy = t.gz = t.sg2 = t.rb = t.bx = t.by = t.bz = 0.0f;
+  }
+  __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    const float tt = fmaxf(U.h2 - r2, 0.0f);
+    t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
+    float s = hr * hr * ir;
+    s = (r2 > 1.0e-12f) ? s : 0.0f;
+    t.gx = __fmaf_rn(s, rx, t.gx);
+    t.gy = __fmaf_rn(s, ry, t.gy);
+    t.gz = __fmaf_rn(s, rz, t.gz);
+    t.sg2 = __fmaf_rn(s * s, r2, t.sg2);
+  }
+  __device__ void bpair(const Uniform& U, Target& t, float4 b) const {
+    const float rx = t.px - b.x, ry = t.py - b.y, rz = t.pz - b.z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    const float tt = U.h2 - r2;
+    t.rb = __fmaf_rn(b.w, tt * tt * tt, t.rb);
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = __fmaf_rn(-r2, ir, U.h);
+    const float s = b.w * hr * hr * ir;
+    t.bx = __fmaf_rn(s, rx, t.bx);
+    t.by = __fmaf_rn(s, ry, t.by);
+    t.bz = __fmaf_rn(s, rz, t.bz);
+  }
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float rho = U.poly6 * __fmaf_rn(U.mass, t.st3, t.rb);
+    density[i] = rho;
+    const float C = rho * U.invRho0 - 1.0f;
+    float lam = 0.0f;
+    if (C > 0.0f) {
+      const float cf = U.volume * U.spikyGrad, cb = U.invRho0 * U.spikyGrad;
+      const float gx = cf * t.gx + cb * t.bx, gy = cf * t.gy + cb * t.by, gz = cf * t.gz + cb * t.bz;
+      const float gg = gx * gx + gy * gy + gz * gz;
+      lam = -C / (gg + cf * cf * t.sg2 + U.cfmEps + 1.0e-12f);
+    }
+    lambda[i] = lam;
+  }
+};
+
+struct DeltaFast : FastBase {
+  static constexpr bool BOUNDARY = true;
+  static constexpr int NREC = 1;
+  const float4* __restrict__ pred;
+  const float* __restrict__ lambda;
+  float4* __restrict__ out;
+  struct Target {
+    float px, py, pz, li;
+    float dx, dy, dz, bx, by, bz, nc;
+  };
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+    const float4 p = pred[j];
+    v[0] = make_float4(p.x, p.y, p.z, lambda[j]);
+  }
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    const float4 p = pred[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z; t.li = lambda[i];
+    t.dx = t.dy = t.dz = t.bx = t.by = t.bz = 0.0f;
+    t.nc = 1.0f;
+  }
+  __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = __fmaf_rn(-r2, ir, U.h);
+    const float tt = U.h2 - r2;
+    const float ratio = (U.poly6 * U.sCorrWq) * (tt * tt * tt);
+    const float r2r = ratio * ratio;
+    const float sc = -U.sCorrK * r2r * r2r;
+    float f = (t.li + v[0].w + sc) * (hr * hr * ir);
+    f = ok ? f : 0.0f;
+    t.dx = __fmaf_rn(f, rx, t.dx);
+    t.dy = __fmaf_rn(f, ry, t.dy);
+    t.dz = __fmaf_rn(f, rz, t.dz);
+    t.nc += ok ? 1.0f : 0.0f;
+  }
+  __device__ void bpair(const Uniform& U, Target& t, float4 b) const {
+    const float rx = t.px - b.x, ry = t.py - b.y, rz = t.pz - b.z;
+    const float r2 = r2_of(rx, ry, rz);
+    if (r2 >= U.h2 || r2 < 1.0e-12f) return;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = __fmaf_rn(-r2, ir, U.h);
+    const float f = (t.li * b.w) * (hr * hr * ir);
+    t.bx = __fmaf_rn(f, rx, t.bx);
+    t.by = __fmaf_rn(f, ry, t.by);
+    t.bz = __fmaf_rn(f, rz, t.bz);
+  }
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float cf = U.spikyGrad * U.volume, cb = U.spikyGrad * U.invRho0;
+    const float dx = cf * t.dx + cb * t.bx, dy = cf * t.dy + cb * t.by, dz = cf * t.dz + cb * t.bz;
+    float scale = U.omega;
+    if (U.sorAverage == 1u) scale = U.omega / fmaxf(t.nc, 1.0f);
+    float ox = t.px + dx * scale, oy = t.py + dy * scale, oz = t.pz + dz * scale;
+    ox = fminf(fmaxf(ox, U.clampMin[0]), U.clampMax[0]);
+    oy = fminf(fmaxf(oy, U.clampMin[1]), U.clampMax[1]);
+    oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
+    out[i] = make_float4(ox, oy, oz, 1.0f);
+  }
+};
+
+// Records: (pred.xyz, q = m / max(rho_j, rho0/2)), (vel_j.xyz, 0).
+template <bool NORMALS>
+struct XsphFast : FastBase {
+  static constexpr bool BOUNDARY = false;
+  static constexpr int NREC = 2;
+  const float4* __restrict__ pred;
+  float4* __restrict__ pos;
+  const float* __restrict__ density;
+  float4* __restrict__ corr;
+  float4* __restrict__ normal;
+  struct Target {
+    float px, py, pz, vx, vy, vz;
+    float ax, ay, az, nx, ny, nz;
+  };
+  __device__ void load(const Uniform& U, uint32_t j, float4* v) const {
+    const float4 p = pred[j], q = pos[j];
+    v[0] = make_float4(p.x, p.y, p.z, U.mass / fmaxf(density[j], 0.5f * U.rho0));
+    v[1] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt, 0.0f);
+  }
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
+    const float4 p = pred[i], q = pos[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.vx = (p.x - q.x) * U.invDt; t.vy = (p.y - q.y) * U.invDt; t.vz = (p.z - q.z) * U.invDt;
+    t.ax = t.ay = t.az = t.nx = t.ny = t.nz = 0.0f;
+  }
+  __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    const float tt = fmaxf(U.h2 - r2, 0.0f);
+    const float f = tt * tt * tt * v[0].w;
+    t.ax = __fmaf_rn(f, v[1].x - t.vx, t.ax);
+    t.ay = __fmaf_rn(f, v[1].y - t.vy, t.ay);
+    t.az = __fmaf_rn(f, v[1].z - t.vz, t.az);
+    if (NORMALS) {
+      const float ir = __builtin_amdgcn_rsqf(r2);
+      const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
+      float g = hr * hr * ir * v[0].w;
+      g = (r2 > 1.0e-12f) ? g : 0.0f;
+      t.nx = __fmaf_rn(g, rx, t.nx);
+      t.ny = __fmaf_rn(g, ry, t.ny);
+      t.nz = __fmaf_rn(g, rz, t.nz);
+    }
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float cx = U.xsphC * U.poly6;
+    corr[i] = make_float4(cx * t.ax, cx * t.ay, cx * t.az, 0.0f);
+    if (NORMALS) {
+      const float cn = U.h * U.spikyGrad;
+      normal[i] = make_float4(cn * t.nx, cn * t.ny, cn * t.nz, 0.0f);
+    }
+  }
+};
+
+// Records: (pred.xyz, rho_j), (normal_j.xyz, 0).
+struct TensionFast : FastBase {
+  static constexpr bool BOUNDARY = false;
+  static constexpr int NREC = 2;
+  const float4* __restrict__ pred;
+  float4* __restrict__ pos;
+  const float* __restrict__ density;
+  const float4* __restrict__ normal;
+  float4* __restrict__ corr;
+  float4* __restrict__ vel;
+  struct Target {
+    float px, py, pz, nx, ny, nz, rho;
+    float ax, ay, az;
+  };
+  __device__ static float4 sentinel(int k) {
+    return k == 0 ? make_float4(kFar, kFar, kFar, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+    const float4 p = pred[j];
+    v[0] = make_float4(p.x, p.y, p.z, density[j]);
+    v[1] = normal[j];
+  }
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    const float4 p = pred[i], nn = normal[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z;
+    t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
+    t.rho = density[i];
+    t.ax = t.ay = t.az = 0.0f;
+  }
+  __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
+    const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
+    const float r2 = r2_of(rx, ry, rz);
+    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    // outside the support ir = 0 keeps every term below finite
+    const float ir = ok ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float r = r2 * ir;
+    const float u = (U.h - r) * r;
+    const float k = u * u * u;
+    const float cw = (2.0f * r > U.h) ? k : __fmaf_rn(2.0f, k, -U.cohesTerm);
+    const float fc = (-U.tension * U.mass * U.cohes) * cw * ir;
+    const float mg = -U.tension;
+    const float fx = __fmaf_rn(fc, rx, mg * (t.nx - v[1].x));
+    const float fy = __fmaf_rn(fc, ry, mg * (t.ny - v[1].y));
+    const float fz = __fmaf_rn(fc, rz, mg * (t.nz - v[1].z));
+    float K = __builtin_amdgcn_rcpf(t.rho + v[0].w);
+    K = ok ? K : 0.0f;
+    t.ax = __fmaf_rn(K, fx, t.ax);
+    t.ay = __fmaf_rn(K, fy, t.ay);
+    t.az = __fmaf_rn(K, fz, t.az);
+  }
+  __device__ void bpair(const Uniform&, Target&, float4) const {}
+  __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
+    const float4 c = corr[i];
+    const float w = U.dt * (2.0f * U.rho0);
+    const float cx = __fmaf_rn(w, t.ax, c.x), cy = __fmaf_rn(w, t.ay, c.y),
+                cz = __fmaf_rn(w, t.az, c.z);
+    const float4 q = pos[i];
+    const float vx = (t.px - q.x) * U.invDt, vy = (t.py - q.y) * U.invDt,
+                vz = (t.pz - q.z) * U.invDt;
+    corr[i] = make_float4(cx, cy, cz, 0.0f);
+    vel[i] = make_float4(vx + cx, vy + cy, vz + cz, 0.0f);
+    pos[i] = pred[i];
+  }
+};
+
 // --- neighbour census: |N(i)| and index checksums -------------------------
 struct CensusOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -803,6 +1089,7 @@ __device__ void aniso_g(const AnisoArgs& A, const R v[3][3], const R sig[3], int
 
 template <bool EXACT>
 struct AnisoOp : OpBase {
+  static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 1;
   const float4* __restrict__ pos;
@@ -1203,6 +1490,12 @@ k_stats_final(const double* __restrict__ part, int nb, uint32_t n, double* __res
   }
 }
 
+__global__ void __launch_bounds__(WG) k_copy4(const float4* __restrict__ a, float4* __restrict__ b,
+                                               size_t n) {
+  const size_t stride = (size_t)gridDim.x * WG;
+  for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1221,25 +1514,47 @@ hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t
   return hipGetLastError();
 }
 
-// Tile shapes: NREC=1 kernels stage up to 3072 records (48 KiB) at TX=32,
-// NREC=2 kernels up to 1536 (48 KiB) at TX=16.  A tile whose halo holds
-// more falls back to global memory for all its particles.
+// Tile shapes (cells per tile along x, threads, LDS record capacity).  A
+// tile whose halo holds more records than the capacity walks global memory
+// for all its particles; capacity 0 makes every tile do so.  p4a_tune times
+// the variants on the live scene and keeps the fastest.
 template <class Op>
 hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s) {
-  if (t.tx == 16) return run_tile<Op, 16, 128, 2048>(U, A, op, s);
-  return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+  switch (t.variant) {
+    case 1: return run_tile<Op, 16, 128, 2048>(U, A, op, s);
+    case 2: return run_tile<Op, 32, 128, 3072>(U, A, op, s);
+    case 3: return run_tile<Op, 64, 256, 5120>(U, A, op, s);
+    case 4: return run_tile<Op, 32, 256, 0>(U, A, op, s);
+    default: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+  }
 }
 template <class Op>
 hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s) {
-  if (t.tx == 32) return run_tile<Op, 32, 256, 1920>(U, A, op, s);
-  return run_tile<Op, 16, 128, 1536>(U, A, op, s);
+  switch (t.variant) {
+    case 1: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+    case 2: return run_tile<Op, 8, 64, 1024>(U, A, op, s);
+    case 3: return run_tile<Op, 16, 256, 1536>(U, A, op, s);
+    case 4: return run_tile<Op, 16, 128, 0>(U, A, op, s);
+    default: return run_tile<Op, 16, 128, 1536>(U, A, op, s);
+  }
 }
 
 }  // namespace
 
 int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
+
+int tile_variants(int group) { return group == 1 ? 5 : 5; }
+
+const char* tile_variant_name(int group, int v) {
+  static const char* g1[] = {"tx32/t256/lds3072", "tx16/t128/lds2048", "tx32/t128/lds3072",
+                             "tx64/t256/lds5120", "tx32/t256/global"};
+  static const char* g2[] = {"tx16/t128/lds1536", "tx32/t256/lds3072", "tx8/t64/lds1024",
+                             "tx16/t256/lds1536", "tx16/t128/global"};
+  if (v < 0 || v >= 5) return "?";
+  return group == 1 ? g1[v] : g2[v];
+}
 
 hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
@@ -1277,7 +1592,7 @@ hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, 
     LambdaOp<true> op{{}, pred, A.density, A.lambda};
     return run_tile1(U, A, op, t, s);
   }
-  LambdaOp<false> op{{}, pred, A.density, A.lambda};
+  LambdaFast op{{}, pred, A.density, A.lambda};
   return run_tile1(U, A, op, t, s);
 }
 
@@ -1287,7 +1602,7 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
     DeltaOp<true> op{{}, pred, A.lambda, out};
     return run_tile1(U, A, op, t, s);
   }
-  DeltaOp<false> op{{}, pred, A.lambda, out};
+  DeltaFast op{{}, pred, A.lambda, out};
   return run_tile1(U, A, op, t, s);
 }
 
@@ -1312,16 +1627,16 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
   float4* p = const_cast<float4*>(pos);
   if (normals) {
     if (exact) return run_tile2(U, A, XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-    return run_tile2(U, A, XsphOp<false, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+    return run_tile2(U, A, XsphFast<true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
   }
   if (exact) return run_tile2(U, A, XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-  return run_tile2(U, A, XsphOp<false, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+  return run_tile2(U, A, XsphFast<false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, hipStream_t s) {
   if (exact) return run_tile2(U, A, TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
-  return run_tile2(U, A, TensionOp<false>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+  return run_tile2(U, A, TensionFast{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
@@ -1377,6 +1692,12 @@ hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
   }
   AnisoOp<false> op{{}, pos, smoothPos, g, fallbacks, a};
   return run_tile<AnisoOp<false>, 16, 128, 3584, 2>(U, A, op, s);
+}
+
+hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy4, dim3(256 * 32), dim3(WG), 0, s, a, b, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {

@@ -79,11 +79,14 @@ struct AnisoArgs {
   float gapTol;  // relative eigenvalue gap below which the FP64 solve runs
 };
 
-// Launch configuration of the tiled neighbour kernels.
+// Launch configuration of the tiled neighbour kernels: an index into the
+// variant tables of kernels.hip (group 1: one-record kernels lambda, delta,
+// census; group 2: two-record kernels xsph, tension).
 struct TileCfg {
-  int tx;       // cells per tile along x
-  int threads;  // threads per workgroup
+  int variant;
 };
+int tile_variants(int group);
+const char* tile_variant_name(int group, int v);
 
 // Drag of a held body (wgsl.js:187-214).
 struct DragArgs {
@@ -141,6 +144,7 @@ hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
                         const AnisoArgs& a, float4* smoothPos, float4* g, uint32_t* fallbacks,
                         bool exact, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s);
 
 int stats_blocks(uint32_t n);
 

@@ -76,7 +76,7 @@ struct p4a_sim {
   std::vector<hipEvent_t> eventPool;
   size_t eventNext = 0;
   // tile shapes
-  TileCfg tile1{32, 256}, tile2{16, 128};
+  TileCfg tile1{0}, tile2{0};
   // anisotropy outputs (allocated on first use)
   float4* smoothPos = nullptr;
   float4* anisoG = nullptr;
@@ -270,8 +270,8 @@ int run_substep_body(p4a_sim* s, const Uniform& U) {
 std::string graph_key(const p4a_sim* s, const Uniform& U) {
   std::string k((const char*)&U, sizeof(U));
   k += std::to_string(s->params.iterations) + "/" + std::to_string(s->params.math) + "/" +
-       std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.tx) +
-       "/" + std::to_string(s->tile2.tx);
+       std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.variant) +
+       "/" + std::to_string(s->tile2.variant);
   return k;
 }
 
@@ -453,6 +453,40 @@ extern "C" {
 
 P4A_API const char* p4a_version(void) { return "p4a 0.1 (gfx950)"; }
 
+P4A_API int p4a_copy_probe(int device, size_t bytes, int32_t reps, double* ms) {
+  if (!ms || reps < 1 || bytes == 0 || bytes % 16) return P4A_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return P4A_ERR_NODEV;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return P4A_ERR_NOMEM;
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return P4A_ERR_NOMEM;
+  }
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  int rc = P4A_OK;
+  if (hipStreamCreate(&st) != hipSuccess) rc = P4A_ERR_HIP;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = P4A_ERR_HIP;
+  if (!rc) {
+    const size_t n = bytes / 16;
+    hipMemsetAsync(a, 0, bytes, st);
+    launch_copy4((const float4*)a, (float4*)b, n, st);  // warm
+    hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) launch_copy4((const float4*)a, (float4*)b, n, st);
+    hipEventRecord(e1, st);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = P4A_ERR_HIP;
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+  }
+  hipFree(a);
+  hipFree(b);
+  return rc;
+}
+
 P4A_API int p4a_create(int device, p4a_sim** out) {
   if (!out) return P4A_ERR_ARG;
   *out = nullptr;
@@ -475,8 +509,8 @@ P4A_API int p4a_create(int device, p4a_sim** out) {
   s->statsHost[0] = 1;
   s->statsHost[1] = 1;
   p4a_params_default(&s->params);
-  if (const char* e = std::getenv("P4A_TILE1")) s->tile1.tx = std::atoi(e) == 16 ? 16 : 32;
-  if (const char* e = std::getenv("P4A_TILE2")) s->tile2.tx = std::atoi(e) == 32 ? 32 : 16;
+  if (const char* e = std::getenv("P4A_TILE1")) s->tile1.variant = std::atoi(e) % tile_variants(1);
+  if (const char* e = std::getenv("P4A_TILE2")) s->tile2.variant = std::atoi(e) % tile_variants(2);
   *out = s;
   return P4A_OK;
 }
@@ -1085,6 +1119,63 @@ P4A_API int p4a_neighbour_census(p4a_sim* s, uint32_t* count, uint32_t* sum_j, u
   if (sum_j) HIP_TRY(s, hipMemcpy(sum_j, s->A.census[1], b, hipMemcpyDeviceToHost));
   if (sum_j2) HIP_TRY(s, hipMemcpy(sum_j2, s->A.census[2], b, hipMemcpyDeviceToHost));
   return P4A_OK;
+}
+
+// Autotuner: time each tile variant of the one-record kernels (on lambda)
+// and of the two-record kernels (on xsph + normals) on the live scene and
+// keep the fastest.  Both timed kernels only rewrite per-step scratch
+// (density, lambda, corr, normal), so the state is unchanged.
+P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity) {
+  int rc = check(s);
+  if (rc) return rc;
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  const double dt = (1.0 / 60.0) / std::max(1, s->params.substeps);
+  const Uniform U = make_uniform(s, dt);
+  const bool ex = exact_math(s);
+  hipEvent_t e0, e1;
+  HIP_TRY(s, hipEventCreate(&e0));
+  HIP_TRY(s, hipEventCreate(&e1));
+  int k = 0;
+  for (int group = 1; group <= 2; group++) {
+    float best = 1e30f;
+    int bestV = 0;
+    for (int v = 0; v < tile_variants(group); v++) {
+      TileCfg c{v};
+      auto run = [&]() -> hipError_t {
+        if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, s->stream);
+        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, s->stream);
+      };
+      HIP_TRY(s, run());
+      HIP_TRY(s, hipEventRecord(e0, s->stream));
+      for (int r = 0; r < 3; r++) HIP_TRY(s, run());
+      HIP_TRY(s, hipEventRecord(e1, s->stream));
+      HIP_TRY(s, hipEventSynchronize(e1));
+      float t = 0;
+      HIP_TRY(s, hipEventElapsedTime(&t, e0, e1));
+      t /= 3;
+      if (ms && k < capacity) ms[k] = t;
+      k++;
+      if (t < best) {
+        best = t;
+        bestV = v;
+      }
+    }
+    if (group == 1) s->tile1.variant = bestV;
+    else s->tile2.variant = bestV;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  drop_graphs(s);
+  if (chosen) {
+    chosen[0] = s->tile1.variant;
+    chosen[1] = s->tile2.variant;
+  }
+  return P4A_OK;
+}
+
+P4A_API const char* p4a_tile_variant_name(int32_t group, int32_t v) {
+  return tile_variant_name(group, v);
 }
 
 P4A_API int p4a_set_timing(p4a_sim* s, int32_t on) {

@@ -423,6 +423,65 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   }
 }
 
+// Direct traversal: one thread per particle in sorted order, candidates
+// read from global memory (the cell-sorted arrays give each workgroup a
+// compact neighbourhood that stays in L1/L2).  Workgroups are remapped so
+// each XCD walks a contiguous range of the sorted particles.  Wall samples
+// exist only in cells with some index 0 or gridDim-1, so a particle whose
+// stencil cannot reach such a cell skips the wall loops.
+template <class Op, int NT, int CR>
+__global__ void __launch_bounds__(NT)
+k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+         const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  typename Op::Target t;
+  op.target(U, i, t);
+  const Cell c = op.cell(U, t);
+  const int xlo = max(c.x - CR, 0), xhi = min(c.x + CR, gx - 1);
+  const bool nearWall = c.x <= CR || c.x >= gx - 1 - CR || c.y <= CR || c.y >= gy - 1 - CR ||
+                        c.z <= CR || c.z >= gz - 1 - CR;
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
+  for (int dz = -CR; dz <= CR; dz++) {
+    const int z = c.z + dz;
+    if (z < 0 || z >= gz) continue;
+    for (int dy = -CR; dy <= CR; dy++) {
+      const int y = c.y + dy;
+      if (y < 0 || y >= gy) continue;
+      if (op.skip_row(U, t, z, y)) continue;
+      const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      if constexpr (Op::FAST) {
+        uint32_t j = b;
+        for (; j + 3 < e; j += 4) {
+          float4 v[4][Op::NREC];
+#pragma unroll
+          for (int u = 0; u < 4; u++) op.load(U, j + u, v[u]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) op.fpair(U, t, v[u]);
+        }
+        for (; j < e; j++) {
+          float4 v[Op::NREC];
+          op.load(U, j, v);
+          op.fpair(U, t, v);
+        }
+      } else {
+        for (uint32_t j = b; j < e; j++) {
+          float4 v[Op::NREC];
+          op.load(U, j, v);
+          op.pair(U, t, j, v);
+        }
+      }
+      if (doB) {
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+      }
+    }
+  }
+  op.finish(U, i, t);
+}
+
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
 template <bool EXACT>
 struct LambdaOp : OpBase {
@@ -1503,6 +1562,14 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
 
+template <class Op, int NT, int CR = 1>
+hipError_t run_direct(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_direct<Op, NT, CR>), dim3((U.n + NT - 1) / NT), dim3(NT), 0, s, U, op,
+                     A.cellStart, A.bcellStart, A.bpos);
+  return hipGetLastError();
+}
+
 template <class Op, int TX, int NT, int MAXREC, int CR = 1>
 hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
   const int gx = U.gridDim[0];
@@ -1522,22 +1589,22 @@ template <class Op>
 hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s) {
   switch (t.variant) {
-    case 1: return run_tile<Op, 16, 128, 2048>(U, A, op, s);
-    case 2: return run_tile<Op, 32, 128, 3072>(U, A, op, s);
-    case 3: return run_tile<Op, 64, 256, 5120>(U, A, op, s);
-    case 4: return run_tile<Op, 32, 256, 0>(U, A, op, s);
-    default: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+    case 1: return run_direct<Op, 128>(U, A, op, s);
+    case 2: return run_direct<Op, 64>(U, A, op, s);
+    case 3: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
+    case 4: return run_tile<Op, 16, 128, 2048>(U, A, op, s);
+    default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
 template <class Op>
 hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s) {
   switch (t.variant) {
-    case 1: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
-    case 2: return run_tile<Op, 8, 64, 1024>(U, A, op, s);
-    case 3: return run_tile<Op, 16, 256, 1536>(U, A, op, s);
-    case 4: return run_tile<Op, 16, 128, 0>(U, A, op, s);
-    default: return run_tile<Op, 16, 128, 1536>(U, A, op, s);
+    case 1: return run_direct<Op, 128>(U, A, op, s);
+    case 2: return run_direct<Op, 64>(U, A, op, s);
+    case 3: return run_tile<Op, 16, 128, 1536>(U, A, op, s);
+    case 4: return run_tile<Op, 16, 256, 1536>(U, A, op, s);
+    default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
 
@@ -1548,10 +1615,10 @@ int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
 int tile_variants(int group) { return group == 1 ? 5 : 5; }
 
 const char* tile_variant_name(int group, int v) {
-  static const char* g1[] = {"tx32/t256/lds3072", "tx16/t128/lds2048", "tx32/t128/lds3072",
-                             "tx64/t256/lds5120", "tx32/t256/global"};
-  static const char* g2[] = {"tx16/t128/lds1536", "tx32/t256/lds3072", "tx8/t64/lds1024",
-                             "tx16/t256/lds1536", "tx16/t128/global"};
+  static const char* g1[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx32/t256/lds3072",
+                             "tile/tx16/t128/lds2048"};
+  static const char* g2[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx16/t128/lds1536",
+                             "tile/tx16/t256/lds1536"};
   if (v < 0 || v >= 5) return "?";
   return group == 1 ? g1[v] : g2[v];
 }
@@ -1688,10 +1755,10 @@ hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
   // (2*2+1)^2 rows of 16+4 cells at ~8 particles per cell: 3584 records
   if (exact) {
     AnisoOp<true> op{{}, pos, smoothPos, g, fallbacks, a};
-    return run_tile<AnisoOp<true>, 16, 128, 3584, 2>(U, A, op, s);
+    return run_direct<AnisoOp<true>, 256, 2>(U, A, op, s);
   }
   AnisoOp<false> op{{}, pos, smoothPos, g, fallbacks, a};
-  return run_tile<AnisoOp<false>, 16, 128, 3584, 2>(U, A, op, s);
+  return run_direct<AnisoOp<false>, 256, 2>(U, A, op, s);
 }
 
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s) {

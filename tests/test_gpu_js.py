@@ -1,0 +1,29 @@
+"""The JavaScript Sim surface on the GPU (Node.js addon over the C ABI):
+the reference's verify harness on the small preset, then pour, box-wall
+move, ray impulse and body drag through the JS calls."""
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_js_sim_verify_harness_and_interactions():
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "verify_sim.js")],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print(out)
+    assert out["nBodies"] == 3 and out["bodyShapes"] == ["sphere", "torus", "box"]
+    assert out["cap"] > out["n0"] and out["sub"] == 2
+    assert out["scanMonotonic"] and out["scanTotal"] == out["n0"]
+    assert out["worstDensityRelError"] < 1e-4
+    assert out["nonFinite"] == 0 and out["lambdaMax"] <= 0
+    assert out["outsideBox"] == 0
+    assert out["poured"] > 0 and out["nAfterPour"] == out["n0"] + out["poured"]
+    assert abs(out["boxX"] - 1.5) > 0.05 and out["gen"] >= 2
+    assert out["impulsed"] > 0
+    assert 0.9 < out["stats"]["avgRho"] < 1.1

@@ -808,12 +808,15 @@ struct FastBase : OpBase {
   }
 };
 
+// lambda is also stored in pred.w (unused by every reader of pred), so the
+// following delta reads position and lambda of a neighbour in one load.
 struct LambdaFast : FastBase {
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
-  const float4* __restrict__ pred;
+  const float4* pred;  // not restrict: .w is written through predW
   float* __restrict__ density;
   float* __restrict__ lambda;
+  float* predW;
   struct Target {
     float px, py, pz;
     float st3, gx, gy, gz, sg2;   // fluid: sum t^3, sum s r, sum s^2 r^2
@@ -864,6 +867,7 @@ struct LambdaFast : FastBase {
       lam = -C / (gg + cf * cf * t.sg2 + U.cfmEps + 1.0e-12f);
     }
     lambda[i] = lam;
+    predW[i * 4 + 3] = lam;
   }
 };
 
@@ -877,13 +881,10 @@ struct DeltaFast : FastBase {
     float px, py, pz, li;
     float dx, dy, dz, bx, by, bz, nc;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
-    const float4 p = pred[j];
-    v[0] = make_float4(p.x, p.y, p.z, lambda[j]);
-  }
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
   __device__ void target(const Uniform&, uint32_t i, Target& t) const {
     const float4 p = pred[i];
-    t.px = p.x; t.py = p.y; t.pz = p.z; t.li = lambda[i];
+    t.px = p.x; t.py = p.y; t.pz = p.z; t.li = p.w;
     t.dx = t.dy = t.dz = t.bx = t.by = t.bz = 0.0f;
     t.nc = 1.0f;
   }
@@ -928,13 +929,15 @@ struct DeltaFast : FastBase {
   }
 };
 
-// Records: (pred.xyz, q = m / max(rho_j, rho0/2)), (vel_j.xyz, 0).
+// Records: (pred.xyz, -), (vel_j.xyz, q_j) with vel = (pred - pos)/dt and
+// q = m / max(rho_j, rho0/2) precomputed once per particle (k_prep_finish).
+// normal.w carries rho_i to the tension kernel.
 template <bool NORMALS>
 struct XsphFast : FastBase {
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
-  float4* __restrict__ pos;
+  const float4* __restrict__ velq;
   const float* __restrict__ density;
   float4* __restrict__ corr;
   float4* __restrict__ normal;
@@ -942,29 +945,28 @@ struct XsphFast : FastBase {
     float px, py, pz, vx, vy, vz;
     float ax, ay, az, nx, ny, nz;
   };
-  __device__ void load(const Uniform& U, uint32_t j, float4* v) const {
-    const float4 p = pred[j], q = pos[j];
-    v[0] = make_float4(p.x, p.y, p.z, U.mass / fmaxf(density[j], 0.5f * U.rho0));
-    v[1] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt, 0.0f);
+  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+    v[0] = pred[j];
+    v[1] = velq[j];
   }
-  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
-    const float4 p = pred[i], q = pos[i];
+  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+    const float4 p = pred[i], w = velq[i];
     t.px = p.x; t.py = p.y; t.pz = p.z;
-    t.vx = (p.x - q.x) * U.invDt; t.vy = (p.y - q.y) * U.invDt; t.vz = (p.z - q.z) * U.invDt;
+    t.vx = w.x; t.vy = w.y; t.vz = w.z;
     t.ax = t.ay = t.az = t.nx = t.ny = t.nz = 0.0f;
   }
   __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
     const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
     const float r2 = r2_of(rx, ry, rz);
     const float tt = fmaxf(U.h2 - r2, 0.0f);
-    const float f = tt * tt * tt * v[0].w;
+    const float f = tt * tt * tt * v[1].w;
     t.ax = __fmaf_rn(f, v[1].x - t.vx, t.ax);
     t.ay = __fmaf_rn(f, v[1].y - t.vy, t.ay);
     t.az = __fmaf_rn(f, v[1].z - t.vz, t.az);
     if (NORMALS) {
       const float ir = __builtin_amdgcn_rsqf(r2);
       const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-      float g = hr * hr * ir * v[0].w;
+      float g = hr * hr * ir * v[1].w;
       g = (r2 > 1.0e-12f) ? g : 0.0f;
       t.nx = __fmaf_rn(g, rx, t.nx);
       t.ny = __fmaf_rn(g, ry, t.ny);
@@ -977,18 +979,18 @@ struct XsphFast : FastBase {
     corr[i] = make_float4(cx * t.ax, cx * t.ay, cx * t.az, 0.0f);
     if (NORMALS) {
       const float cn = U.h * U.spikyGrad;
-      normal[i] = make_float4(cn * t.nx, cn * t.ny, cn * t.nz, 0.0f);
+      normal[i] = make_float4(cn * t.nx, cn * t.ny, cn * t.nz, density[i]);
     }
   }
 };
 
-// Records: (pred.xyz, rho_j), (normal_j.xyz, 0).
+// Records: (pred.xyz, -), (normal_j.xyz, rho_j).
 struct TensionFast : FastBase {
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
   float4* __restrict__ pos;
-  const float* __restrict__ density;
+  const float4* __restrict__ velq;
   const float4* __restrict__ normal;
   float4* __restrict__ corr;
   float4* __restrict__ vel;
@@ -997,18 +999,17 @@ struct TensionFast : FastBase {
     float ax, ay, az;
   };
   __device__ static float4 sentinel(int k) {
-    return k == 0 ? make_float4(kFar, kFar, kFar, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
   __device__ void load(const Uniform&, uint32_t j, float4* v) const {
-    const float4 p = pred[j];
-    v[0] = make_float4(p.x, p.y, p.z, density[j]);
+    v[0] = pred[j];
     v[1] = normal[j];
   }
   __device__ void target(const Uniform&, uint32_t i, Target& t) const {
     const float4 p = pred[i], nn = normal[i];
     t.px = p.x; t.py = p.y; t.pz = p.z;
     t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
-    t.rho = density[i];
+    t.rho = nn.w;
     t.ax = t.ay = t.az = 0.0f;
   }
   __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
@@ -1026,7 +1027,7 @@ struct TensionFast : FastBase {
     const float fx = __fmaf_rn(fc, rx, mg * (t.nx - v[1].x));
     const float fy = __fmaf_rn(fc, ry, mg * (t.ny - v[1].y));
     const float fz = __fmaf_rn(fc, rz, mg * (t.nz - v[1].z));
-    float K = __builtin_amdgcn_rcpf(t.rho + v[0].w);
+    float K = __builtin_amdgcn_rcpf(t.rho + v[1].w);
     K = ok ? K : 0.0f;
     t.ax = __fmaf_rn(K, fx, t.ax);
     t.ay = __fmaf_rn(K, fy, t.ay);
@@ -1038,14 +1039,24 @@ struct TensionFast : FastBase {
     const float w = U.dt * (2.0f * U.rho0);
     const float cx = __fmaf_rn(w, t.ax, c.x), cy = __fmaf_rn(w, t.ay, c.y),
                 cz = __fmaf_rn(w, t.az, c.z);
-    const float4 q = pos[i];
-    const float vx = (t.px - q.x) * U.invDt, vy = (t.py - q.y) * U.invDt,
-                vz = (t.pz - q.z) * U.invDt;
+    const float4 vq = velq[i];
     corr[i] = make_float4(cx, cy, cz, 0.0f);
-    vel[i] = make_float4(vx + cx, vy + cy, vz + cz, 0.0f);
-    pos[i] = pred[i];
+    vel[i] = make_float4(vq.x + cx, vq.y + cy, vq.z + cz, 0.0f);
+    pos[i] = make_float4(t.px, t.py, t.pz, 1.0f);
   }
 };
+
+// velFromPos (wgsl.js:72-83) and the xsph weight factor, once per
+// particle: velq = ((pred - pos) / dt, m / max(rho, rho0/2)).
+__global__ void __launch_bounds__(WG)
+k_prep_finish(Uniform U, const float4* __restrict__ pred, const float4* __restrict__ pos,
+              const float* __restrict__ density, float4* __restrict__ velq) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= U.n) return;
+  const float4 p = pred[i], q = pos[i];
+  velq[i] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt,
+                        U.mass / fmaxf(density[i], 0.5f * U.rho0));
+}
 
 // --- neighbour census: |N(i)| and index checksums -------------------------
 struct CensusOp : OpBase {
@@ -1659,7 +1670,7 @@ hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, 
     LambdaOp<true> op{{}, pred, A.density, A.lambda};
     return run_tile1(U, A, op, t, s);
   }
-  LambdaFast op{{}, pred, A.density, A.lambda};
+  LambdaFast op{{}, pred, A.density, A.lambda, (float*)pred};
   return run_tile1(U, A, op, t, s);
 }
 
@@ -1694,22 +1705,30 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
   float4* p = const_cast<float4*>(pos);
   if (normals) {
     if (exact) return run_tile2(U, A, XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-    return run_tile2(U, A, XsphFast<true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+    return run_tile2(U, A, XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, t, s);
   }
   if (exact) return run_tile2(U, A, XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-  return run_tile2(U, A, XsphFast<false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
+  return run_tile2(U, A, XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, t, s);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, hipStream_t s) {
   if (exact) return run_tile2(U, A, TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
-  return run_tile2(U, A, TensionFast{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
+  return run_tile2(U, A, TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, t, s);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                            float4* vel, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_finalize, dim3(blocks(U.n)), dim3(WG), 0, s, U, pred, pos, A.corr, vel);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
+                              const float4* pos, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prep_finish, dim3(blocks(U.n)), dim3(WG), 0, s, U, pred, pos, A.density,
+                     A.velq);
   return hipGetLastError();
 }
 

@@ -46,6 +46,7 @@ struct Arrays {
   float* lambda;
   float4* corr;
   float4* normal;
+  float4* velq;         // fast path: (pred - pos)/dt and m/max(rho, rho0/2)
   uint32_t* cellCount;  // nCells+1, zero between grid builds
   uint32_t* cellStart;  // nCells+2
   uint32_t* blockSum;
@@ -133,6 +134,8 @@ hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred,
                           float4* vel, bool exact, const TileCfg& t, hipStream_t s);
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                            float4* vel, hipStream_t s);
+hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
+                              const float4* pos, hipStream_t s);
 hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s);
 hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,
                             hipStream_t s);

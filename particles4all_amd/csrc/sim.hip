@@ -238,6 +238,8 @@ int run_finish(p4a_sim* s, const Uniform& U) {
   const int par = s->parity;
   const bool ex = exact_math(s);
   const bool tension = s->params.surface_tension_k > 0;
+  if (!ex)
+    LAUNCH(s, "prep_finish", launch_prep_finish(U, s->A, s->predCur, s->A.pos[par], s->stream));
   LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, s->stream));
   if (tension)
     LAUNCH(s, "tension", launch_tension(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par], ex,
@@ -594,6 +596,7 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   ALLOC(A.lambda, cap);
   ALLOC(A.corr, cap);
   ALLOC(A.normal, cap);
+  ALLOC(A.velq, cap);
   ALLOC(A.bodyOfPid, cap);
   ALLOC(A.restOfPid, cap);
   for (int k = 0; k < 3; k++) ALLOC(A.census[k], cap);

@@ -12,8 +12,12 @@ normals, tension + finalize) plus the stats reduction, all resident in HBM.
 Prints one JSON line; `value` is particle-substeps/s.  After the timed
 region a second pass of the same steps records per-kernel HIP events on
 the simulation stream (eager launches) for the roofline of the dominant
-kernel.  The CPU baseline times the oracle's float restatement of the same
-substep (OpenMP) on a bounded sample of the same workload.
+kernel.  The FP16-offset form (solver iterations read 8-byte half-offset
+records) is then timed the same way on the same simulation, and its
+accuracy is measured against the FP64 oracle: densities of both forms on
+one shared evolved state, and positions/densities after one substep from
+the initial scene.  The CPU baseline times the oracle's float restatement
+of the same substep (OpenMP) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -45,6 +49,8 @@ def parse():
     ap.add_argument("--workload", default="block_4m_bodies")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64-oracle accuracy figures of the FP16-offset form")
     ap.add_argument("--cpu-substeps", type=int, default=2)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the device copy probe on MIB mebibytes and exit")
@@ -77,6 +83,105 @@ def cpu_baseline(p, sc, dt, substeps):
             "kind": "port",
             "sample": f"{substeps} substep(s) of the full {sc.n}-particle workload, oracle float "
                       f"restatement, {threads} OpenMP threads, {el:.1f} s"}
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE passes, copy-probe calibrated),
+    or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_fp32.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC "):
+            vals = dict(kv.split("=") for kv in line.split()[1:])
+            return int(vals[kernel]) if kernel in vals else None
+    return None
+
+
+def timed_steps(sim, frame, steps):
+    sim.sync()
+    t0 = time.perf_counter()
+    sub = 0
+    for _ in range(steps):
+        sub += sim.step(frame)
+    sim.sync()
+    return sub, time.perf_counter() - t0
+
+
+def kernel_timing(sim, frame, steps):
+    sim.set_timing(True)
+    sim.substeps(frame, steps)
+    sim.sync()
+    kt = sim.timing()
+    sim.set_timing(False)
+    return kt
+
+
+def density_errors(sim, p, sc, frame):
+    """Densities of the FP16 and the FP32 form on the same predicted state
+    (the simulation's current one) against the FP64 oracle; |error|/rho0."""
+    import numpy as np
+    from oracle import oracle as O
+    P = O.constants_from(p, sc, frame)
+    p.precision = 1
+    sim.set_params(p)
+    sim.run_stage("PREDICT", frame)
+    sim.run_stage("GRID", frame)
+    pred = sim.read("PRED").astype(np.float64)
+    cs = sim.read("CELL_START")
+    bpos = sim.read("BPOS").astype(np.float64)
+    bcs = sim.read("BCELL_START")
+    sim.run_stage("LAMBDA", frame)
+    d16 = sim.read("DENSITY").astype(np.float64)
+    p.precision = 0
+    sim.set_params(p)
+    sim.run_stage("LAMBDA", frame)
+    d32 = sim.read("DENSITY").astype(np.float64)
+    den, _ = O.lambda_(O.F64, P, pred, cs, bpos, bpos[:, 3].copy(), bcs)
+    out = {}
+    for name, d in (("fp16_offset", d16), ("fp32", d32)):
+        e = np.abs(d - den) / P.rho0
+        out[name] = {"mean": float(e.mean()), "max": float(e.max())}
+    return out
+
+
+def substep_errors(p, sc, dt):
+    """One substep from the initial scene in both forms against the FP64
+    oracle, which starts from the state read back after the reset (the live
+    cell-sorted order): position error (units of h) and density error
+    (units of rho0)."""
+    import numpy as np
+    from particles4all_amd import Sim
+    from oracle import oracle as O
+    P = O.constants_from(p, sc, dt)
+    rec, start = O.sort_boundary(P, sc.boundary_pts, sc.boundary_psi)
+    inv = (1.0 / np.maximum(1e-6, sc.body_density)).astype(np.float32).astype(np.float64)
+    rot = np.tile(np.eye(3).reshape(1, 9), (max(sc.n_bodies, 0), 1))
+    out, st, info, pid0 = {}, None, None, None
+    for name, prec in (("fp16_offset", 1), ("fp32", 0)):
+        p.precision = prec
+        sim = Sim(0)
+        sim.reset(p, sc)
+        if st is None:
+            f64 = lambda k: sim.read(k).astype(np.float64)
+            st = O.State(f64("POS"), f64("VEL"), sim.read("BODY")[:, 0], f64("REST"))
+            pid0 = sim.read("PID")
+            info = O.substep(O.F64, P, st, (rec, rec[:, 3].copy(), start),
+                             np.array(sc.body_centre, np.float64), rot, inv, p.iterations,
+                             p.surface_tension_k > 0)
+        sim.substeps(dt, 1)
+        same = bool(np.array_equal(sim.read("PID"), pid0[info["order"]]))
+        pos = sim.read("POS").astype(np.float64)[:, :3]
+        den = sim.read("DENSITY").astype(np.float64)
+        sim.close()
+        ep = np.abs(pos - st.pos[:, :3]) / sc.h
+        ed = np.abs(den - info["density"]) / P.rho0
+        out[name] = {"same_order": same, "pos_max_h": float(ep.max()),
+                     "pos_rms_h": float(np.sqrt((ep ** 2).mean())),
+                     "density_mean": float(ed.mean()), "density_max": float(ed.max())}
+    p.precision = 0
+    return out
 
 
 def main():
@@ -114,21 +219,12 @@ def main():
     sim.step(frame)  # captures the graphs with the chosen shapes
     sim.sync()
     log("warmup done")
-    t0 = time.perf_counter()
-    sub = 0
-    for _ in range(args.steps):
-        sub += sim.step(frame)
-    sim.sync()
-    el = time.perf_counter() - t0
+    sub, el = timed_steps(sim, frame, args.steps)
     value = n * sub / el
     log(f"timed: {args.steps} steps ({sub} substeps) in {el:.3f}s")
 
     # per-kernel timing pass (same work, eager launches with events)
-    sim.set_timing(True)
-    sim.substeps(frame, args.steps)
-    sim.sync()
-    kt = sim.timing()
-    sim.set_timing(False)
+    kt = kernel_timing(sim, frame, args.steps)
     dom = max((k for k in kt if k in ALGO_BYTES), key=lambda k: kt[k][0] * kt[k][1])
     dom_ms = kt[dom][0]
     achieved = ALGO_BYTES[dom] * n / (dom_ms * 1e-3) / 1e9
@@ -136,6 +232,25 @@ def main():
     log("kernels (mean ms/launch, launches): " +
         ", ".join(f"{k} {v[0]:.3f}x{v[1]}" for k, v in kt.items()))
     log(f"stats avgRho {st[0]:.4f} maxRho {st[1]:.4f} maxSpeed {st[2]:.3f}")
+
+    # FP16-offset form on the same simulation
+    p.precision = 1
+    sim.set_params(p)
+    for _ in range(2):
+        sim.step(frame)
+    sub16, el16 = timed_steps(sim, frame, args.steps)
+    value16 = n * sub16 / el16
+    kt16 = kernel_timing(sim, frame, args.steps)
+    solver = lambda k: sum(k[s][0] * k[s][1] for s in ("lambda", "delta") if s in k)
+    fp16 = {"value": value16, "speedup_substep": value16 / value,
+            "speedup_solver_kernels": solver(kt) / max(solver(kt16), 1e-9),
+            "kernels_ms": {k: round(v[0], 4) for k, v in kt16.items() if k in ("lambda", "delta")}}
+    log(f"fp16-offset: {value16:.4g} particle-substeps/s ({fp16['speedup_substep']:.3f}x); "
+        f"lambda {kt16['lambda'][0]:.3f} ms, delta {kt16['delta'][0]:.3f} ms")
+    if not args.no_accuracy:
+        fp16["density_error_evolved"] = density_errors(sim, p, sc, frame)
+        log(f"density error on the evolved state: {fp16['density_error_evolved']}")
+    p.precision = 0
 
     out = {
         "metric": "particle-substeps/s at 4M particles, 4 iters; HBM GB/s vs peak; FP16 vs FP32",
@@ -156,15 +271,18 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": profiled_traffic(args.workload, dom),
                      "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
                      "mean_launch_ms": dom_ms, "copy_probe_GBps": copy_gbs},
         "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()},
         "tile_shapes": chosen,
-        "fp16_offset": None,
+        "fp16_offset": fp16,
     }
+    sim.close()
+    if not args.no_accuracy:
+        fp16["first_substep_vs_fp64"] = substep_errors(p, sc, frame)
+        log(f"first substep vs FP64 oracle: {fp16['first_substep_vs_fp64']}")
     if not args.no_cpu_baseline:
-        sim.close()
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, sc, dt, args.cpu_substeps)
     print(json.dumps(out), flush=True)

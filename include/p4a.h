@@ -45,8 +45,13 @@ enum p4a_status {
 /* Arithmetic form of the neighbour kernels. */
 enum p4a_precision {
   P4A_PREC_FP32 = 0,         /* float positions, float sums (reference form) */
-  P4A_PREC_FP16_OFFSET = 1   /* neighbour positions as fp16 offsets from the
-                                cell origin, fp32 accumulation */
+  P4A_PREC_FP16_OFFSET = 1   /* solver iterations (lambda, delta) read
+                                neighbours as 8-byte records: x as fp32,
+                                y and z as fp16 offsets from the centre of
+                                the cell each particle was binned into; fp32
+                                lambda, sums and master positions.  Fast
+                                math only: with P4A_MATH_EXACT, reset and
+                                set_params fail with P4A_ERR_ARG */
 };
 
 /* Rounding of sqrt and division inside the neighbour kernels. */
@@ -244,6 +249,10 @@ enum p4a_buf {
   P4A_BUF_BPSI = 18,       /* float nb */
   P4A_BUF_SMOOTH_POS = 19, /* float4 n: smoothed centres of the last p4a_aniso */
   P4A_BUF_ANISO_G = 20,    /* float4 2n: packed G of the last p4a_aniso */
+  P4A_BUF_OFS = 21,        /* 8 bytes n: FP16-offset records of the current
+                              pred (float x; half y, z offsets from the
+                              binned cell's centre) */
+  P4A_BUF_CELL_OF_SLOT = 22, /* uint n: cell each slot was binned into */
   P4A_BUF_COUNT_
 };
 

@@ -186,7 +186,7 @@ void lambda_k(const OrcParams& P, int n, const double* pred, const uint32_t* cs,
 template <class R>
 void delta_k(const OrcParams& P, int n, const double* pred, const double* lambda,
              const uint32_t* cs, const double* bpos, const double* bpsi, const uint32_t* bcs,
-             double* out) {
+             double* out, const double* base) {
   K<R> k(P);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int i = 0; i < n; i++) {
@@ -224,7 +224,9 @@ void delta_k(const OrcParams& P, int n, const double* pred, const double* lambda
     const R ez = dz * k.volume + bz * k.invRho0;
     R scale = k.omega;
     if (k.sor == 1) scale = k.omega / std::max(nc, (R)1);
-    R o[3] = {px + ex * scale, py + ey * scale, pz + ez * scale};
+    R bxp = px, byp = py, bzp = pz;
+    if (base) { bxp = (R)base[i * 4]; byp = (R)base[i * 4 + 1]; bzp = (R)base[i * 4 + 2]; }
+    R o[3] = {bxp + ex * scale, byp + ey * scale, bzp + ez * scale};
     for (int a = 0; a < 3; a++) o[a] = std::min(std::max(o[a], k.clampMin[a]), k.clampMax[a]);
     out[i * 4] = o[0]; out[i * 4 + 1] = o[1]; out[i * 4 + 2] = o[2]; out[i * 4 + 3] = 1.0;
   }
@@ -577,8 +579,19 @@ void orc_lambda(int mode, const OrcParams* P, int n, const double* pred, const u
 void orc_delta(int mode, const OrcParams* P, int n, const double* pred, const double* lambda,
                const uint32_t* cs, const double* bpos, const double* bpsi, const uint32_t* bcs,
                double* out) {
-  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out);
-  else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out);
+  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, nullptr);
+  else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, nullptr);
+}
+
+// delta evaluated at positions `pred` and applied to positions `base`:
+// out = clamp(base + correction(pred)).  Restates the FP16-offset form,
+// which evaluates the correction at quantized positions and moves the FP32
+// master positions.
+void orc_delta_at(int mode, const OrcParams* P, int n, const double* pred, const double* base,
+                  const double* lambda, const uint32_t* cs, const double* bpos,
+                  const double* bpsi, const uint32_t* bcs, double* out) {
+  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, base);
+  else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, base);
 }
 
 void orc_xsph(int mode, const OrcParams* P, int n, const double* pred, const double* pos,

@@ -182,6 +182,40 @@ def delta(mode, P, pred, lam, cs, bpos=None, bpsi=None, bcs=None):
     return out
 
 
+def delta_at(mode, P, pred, base, lam, cs, bpos=None, bpsi=None, bcs=None):
+    """delta evaluated at `pred`, applied to `base`: clamp(base + correction)."""
+    pred = np.ascontiguousarray(pred, np.float64)
+    base = np.ascontiguousarray(base, np.float64)
+    lam = np.ascontiguousarray(lam, np.float64)
+    cs = np.ascontiguousarray(cs, np.uint32)
+    bp, bs, bc = _bnd(bpos, bpsi, bcs)
+    out = np.zeros_like(pred)
+    lib().orc_delta_at(mode, C.byref(P), len(pred), _p(pred), _p(base), _p(lam), _p(cs), _p(bp),
+                       _p(bs), _p(bc) if bc is not None else None, _p(out))
+    return out
+
+
+def cell_centres(P, cells):
+    """Float centre boxMin + (x + 0.5) * h of flat cell indices."""
+    gx, gy = P.gridDim[0], P.gridDim[1]
+    c = np.asarray(cells, np.int64)
+    xyz = np.stack([c % gx, (c // gx) % gy, c // (gx * gy)], axis=1).astype(np.float32)
+    return np.array(P.boxMin[:], np.float32) + (xyz + np.float32(0.5)) * np.float32(P.h)
+
+
+def fp16_quantize(P, pred, cells):
+    """Positions of the FP16-offset form: x as float; y and z as
+    centre(binned cell) + half(pred - centre), the offset formed in float
+    and rounded to nearest even."""
+    c32 = cell_centres(P, cells)[:, 1:]
+    p32 = np.asarray(pred)[:, :3].astype(np.float32)
+    o = (p32[:, 1:] - c32).astype(np.float16)
+    q = np.array(pred, np.float64)
+    q[:, 0] = p32[:, 0]
+    q[:, 1:3] = c32.astype(np.float64) + o.astype(np.float64)
+    return q
+
+
 def xsph(mode, P, pred, pos, density, cs, normals):
     pred = np.ascontiguousarray(pred, np.float64)
     pos = np.ascontiguousarray(pos, np.float64)
@@ -289,6 +323,7 @@ def _setup(L):
     L.orc_predict.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
     L.orc_lambda.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.orc_delta.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_delta_at.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
     L.orc_xsph.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp]
     L.orc_tension.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
     L.orc_finalize.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
@@ -308,10 +343,15 @@ class State:
         self.rest = np.array(rest, np.float64)
 
 
-def substep(mode, P, st, bnd, centre, rot, inv_ratio, iterations, tension_on):
+def substep(mode, P, st, bnd, centre, rot, inv_ratio, iterations, tension_on, fp16=False):
     """One substep of Sim.step (sim.js:458-596) on State `st` (modified in
     place).  bnd = (bpos, bpsi, bcs) from sort_boundary.  Returns a dict of
-    intermediate arrays (post-sort order) for comparisons."""
+    intermediate arrays (post-sort order) for comparisons.
+
+    fp16=True restates the FP16-offset form of the solver iterations (not a
+    reference feature): lambda and delta are evaluated at the quantized
+    positions of fp16_quantize (binned cell = the grid build's cell id), and
+    delta moves the unquantized positions."""
     pred, _ = predict(mode, P, st.pos, st.vel)
     key = cell_ids(mode, P, pred)
     cs, slot, order = grid(P, key)
@@ -323,9 +363,15 @@ def substep(mode, P, st, bnd, centre, rot, inv_ratio, iterations, tension_on):
     if len(centre):
         centre = body_pre(pred, body_of, centre)
     lam_last = den_last = None
+    cells = key[order]
     for _ in range(max(1, iterations)):
-        den_last, lam_last = lambda_(mode, P, pred, cs, bpos, bpsi, bcs)
-        pred = delta(mode, P, pred, lam_last, cs, bpos, bpsi, bcs)
+        if fp16:
+            q = fp16_quantize(P, pred, cells)
+            den_last, lam_last = lambda_(mode, P, q, cs, bpos, bpsi, bcs)
+            pred = delta_at(mode, P, q, pred, lam_last, cs, bpos, bpsi, bcs)
+        else:
+            den_last, lam_last = lambda_(mode, P, pred, cs, bpos, bpsi, bcs)
+            pred = delta(mode, P, pred, lam_last, cs, bpos, bpsi, bcs)
         if len(centre):
             pred, centre, rot = body_solve(pred, body_of, rest, inv_ratio, centre, rot)
     vel, corr, normal = xsph(mode, P, pred, pos, den_last, cs, tension_on)

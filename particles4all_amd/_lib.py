@@ -24,7 +24,8 @@ SCENE_PRESET, SCENE_LATTICE = 0, 1
 
 BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CORR=8,
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
-           PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20)
+           PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20,
+           OFS=21, CELL_OF_SLOT=22)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -319,7 +320,8 @@ class Sim:
                 "BODY_ROT": (np.float32, i.n_bodies * 3, 4), "PID": (np.uint32, n, 1),
                 "KEY": (np.uint32, n, 1), "ORDER": (np.uint32, n, 1),
                 "BPSI": (np.float32, i.n_boundary, 1), "SMOOTH_POS": (np.float32, n, 4),
-                "ANISO_G": (np.float32, 2 * n, 4)}[name]
+                "ANISO_G": (np.float32, 2 * n, 4), "OFS": (np.uint32, n, 2),
+                "CELL_OF_SLOT": (np.uint32, n, 1)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")

@@ -60,6 +60,30 @@ __device__ __forceinline__ uint32_t cell_index(const Uniform& U, Cell c) {
   return (uint32_t)((c.z * U.gridDim[1] + c.y) * U.gridDim[0] + c.x);
 }
 
+// Cell coordinates of a flat cell index and the FP16-offset record of a
+// position binned into that cell: x kept as float, y and z as half offsets
+// from the cell's centre boxMin + (index + 0.5) h, rounded to nearest even.
+__device__ __forceinline__ void cell_xyz(const Uniform& U, uint32_t c, int& x, int& y, int& z) {
+  const uint32_t gx = (uint32_t)U.gridDim[0], gy = (uint32_t)U.gridDim[1];
+  x = (int)(c % gx);
+  const uint32_t yz = c / gx;
+  y = (int)(yz % gy);
+  z = (int)(yz / gy);
+}
+__device__ __forceinline__ float cell_centre(const Uniform& U, int a, int k) {
+  return U.boxMin[a] + ((float)k + 0.5f) * U.h;
+}
+__device__ __forceinline__ HRec encode_ofs(const Uniform& U, uint32_t c, float px, float py,
+                                           float pz) {
+  int x, y, z;
+  cell_xyz(U, c, x, y, z);
+  HRec o;
+  o.x = px;
+  o.y = __float2half_rn(py - cell_centre(U, 1, y));
+  o.z = __float2half_rn(pz - cell_centre(U, 2, z));
+  return o;
+}
+
 // poly6(r2) of wgsl.js:50-53 for r2 < h2: ((poly6 * t) * t) * t.
 __device__ __forceinline__ float poly6_w(const Uniform& U, float r2) {
   float t = U.h2 - r2;
@@ -235,7 +259,8 @@ k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __rest
             float4* __restrict__ predOut, uint32_t* __restrict__ pidOut,
             uint32_t* __restrict__ order, uint32_t* __restrict__ slot,
             uint32_t* __restrict__ slotOf, const float4* __restrict__ vel,
-            float4* __restrict__ velOut) {
+            float4* __restrict__ velOut, HRec* __restrict__ ofsOut,
+            uint32_t* __restrict__ cellOfSlot, Uniform U) {
   const uint32_t s = blockIdx.x * WG + threadIdx.x;
   if (s >= n) return;
   const uint32_t i = tmp[s];
@@ -253,6 +278,11 @@ k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __rest
   pidOut[d] = q;
   slotOf[q] = d;
   if (vel) velOut[d] = vel[i];
+  if (ofsOut) {
+    const float4 p = pred[i];
+    ofsOut[d] = encode_ofs(U, c, p.x, p.y, p.z);
+    cellOfSlot[d] = c;
+  }
   if (s == b) count[c] = 0u;
 }
 
@@ -1058,6 +1088,146 @@ k_prep_finish(Uniform U, const float4* __restrict__ pred, const float4* __restri
                         U.mass / fmaxf(density[i], 0.5f * U.rho0));
 }
 
+// --- FP16-offset form (solver iterations) -----------------------------------
+// Neighbours are read as 8-byte records (x as float, y and z as half
+// offsets from the centre of the cell the particle was binned into at the
+// last grid build) instead of 16-byte float4 positions.  The walk goes row
+// by row as the FP32 form does: within one (y, z) row of cells every
+// candidate shares the row's y and z centres, so the target's quantized y
+// and z are taken relative to that row once and each candidate costs one
+// fused half-to-float FMA per axis.  The target uses its own quantized
+// position (self pair exactly zero, pair distances symmetric) for fluid
+// pairs and, in absolute float form, for wall samples: the form is PBF
+// evaluated at the quantized positions.  The master positions stay FP32 in
+// pred (delta moves them by the correction) and the records are re-encoded
+// after every position update (grid build, delta, shape matching).  Lambda
+// stays FP32 (delta reads it from the lambda array); sums accumulate in
+// float.
+
+// a - o as one fused op on the half operand (gfx950 v_fma_mix_f32); the
+// product o * -1 is exact, so this equals the rounded difference.  The -1
+// comes from the uniform block so the FMA is not folded into a subtraction.
+__device__ __forceinline__ float dh(float a, __half o, float minusOne) {
+  return __fmaf_rn(__half2float(o), minusOne, a);
+}
+
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+           const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot) {
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  typename Op::Target t;
+  op.target(U, i, t);
+  int bx, by, bz;
+  cell_xyz(U, cellOfSlot[i], bx, by, bz);
+  const HRec ri = ofs[i];
+  const float oy = __half2float(ri.y), oz = __half2float(ri.z);
+  const Cell c = cell_of(U, t.px, t.py, t.pz);
+  const int xlo = max(c.x - 1, 0), xhi = min(c.x + 1, gx - 1);
+  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
+                        c.z >= gz - 2;
+  const bool doB = U.nBoundary > 0 && nearWall;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int z = c.z + dz;
+    if (z < 0 || z >= gz) continue;
+    const float az = __fmaf_rn((float)(bz - z), U.h, oz);
+    for (int dy = -1; dy <= 1; dy++) {
+      const int y = c.y + dy;
+      if (y < 0 || y >= gy) continue;
+      const float ay = __fmaf_rn((float)(by - y), U.h, oy);
+      const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      uint32_t j = b;
+      for (; j + 3 < e; j += 4) {
+        HRec r[4];
+        float l[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) op.load16(ofs, j + u, r[u], l[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) op.fpair16(U, t, ay, az, r[u], l[u]);
+      }
+      for (; j < e; j++) {
+        HRec r;
+        float l;
+        op.load16(ofs, j, r, l);
+        op.fpair16(U, t, ay, az, r, l);
+      }
+      if (doB) {
+        const float sy = t.py, sz = t.pz;
+        t.py = cell_centre(U, 1, by) + oy;
+        t.pz = cell_centre(U, 2, bz) + oz;
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+        t.py = sy; t.pz = sz;
+      }
+    }
+  }
+  op.finish16(U, i, t);
+}
+
+// lambda over FP16-offset records
+struct Lambda16 : LambdaFast {
+  __device__ void load16(const HRec* ofs, uint32_t j, HRec& r, float& l) const {
+    r = ofs[j];
+    l = 0.0f;
+  }
+  __device__ void fpair16(const Uniform& U, Target& t, float ay, float az, HRec o,
+                          float) const {
+    const float rx = t.px - o.x, ry = dh(ay, o.y, U.minusOne), rz = dh(az, o.z, U.minusOne);
+    const float r2 = r2_of(rx, ry, rz);
+    const float tt = fmaxf(U.h2 - r2, 0.0f);
+    t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
+    float s = hr * hr * ir;
+    s = (r2 > 1.0e-12f) ? s : 0.0f;
+    t.gx = __fmaf_rn(s, rx, t.gx);
+    t.gy = __fmaf_rn(s, ry, t.gy);
+    t.gz = __fmaf_rn(s, rz, t.gz);
+    t.sg2 = __fmaf_rn(s * s, r2, t.sg2);
+  }
+  __device__ void finish16(const Uniform& U, uint32_t i, Target& t) const {
+    LambdaFast::finish(U, i, t);
+  }
+};
+
+// delta over FP16-offset records (lambda_j from the lambda array); writes
+// the moved FP32 position and its re-encoded record
+struct Delta16 : DeltaFast {
+  HRec* ofsOut;
+  const uint32_t* cellOfSlot;
+  __device__ void load16(const HRec* ofs, uint32_t j, HRec& r, float& l) const {
+    r = ofs[j];
+    l = lambda[j];
+  }
+  __device__ void fpair16(const Uniform& U, Target& t, float ay, float az, HRec o,
+                          float lj) const {
+    const float rx = t.px - o.x, ry = dh(ay, o.y, U.minusOne), rz = dh(az, o.z, U.minusOne);
+    const float r2 = r2_of(rx, ry, rz);
+    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float hr = __fmaf_rn(-r2, ir, U.h);
+    const float tt = U.h2 - r2;
+    const float ratio = (U.poly6 * U.sCorrWq) * (tt * tt * tt);
+    const float r2r = ratio * ratio;
+    const float sc = -U.sCorrK * r2r * r2r;
+    float f = (t.li + lj + sc) * (hr * hr * ir);
+    f = ok ? f : 0.0f;
+    t.dx = __fmaf_rn(f, rx, t.dx);
+    t.dy = __fmaf_rn(f, ry, t.dy);
+    t.dz = __fmaf_rn(f, rz, t.dz);
+    t.nc += ok ? 1.0f : 0.0f;
+  }
+  __device__ void finish16(const Uniform& U, uint32_t i, Target& t) const {
+    DeltaFast::finish(U, i, t);
+    const float4 np = out[i];
+    ofsOut[i] = encode_ofs(U, cellOfSlot[i], np.x, np.y, np.z);
+  }
+};
+
 // --- neighbour census: |N(i)| and index checksums -------------------------
 struct CensusOp : OpBase {
   static constexpr bool FAST = false;
@@ -1336,7 +1506,8 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
              const uint32_t* __restrict__ bodyPids, const uint32_t* __restrict__ bodyFirst,
              const float4* __restrict__ restOfPid, const float* __restrict__ invRatio,
              double* __restrict__ centreD, double* __restrict__ rotD,
-             float4* __restrict__ centreF, float4* __restrict__ rotF) {
+             float4* __restrict__ centreF, float4* __restrict__ rotF, HRec* __restrict__ ofs,
+             const uint32_t* __restrict__ cellOfSlot, Uniform U) {
   __shared__ double sh[WG / 64][15];
   __shared__ float sc[12];
   const int b = blockIdx.x;
@@ -1423,7 +1594,10 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
     const float px = sc[3] * r.x + sc[4] * r.y + sc[5] * r.z;
     const float py = sc[6] * r.x + sc[7] * r.y + sc[8] * r.z;
     const float pz = sc[9] * r.x + sc[10] * r.y + sc[11] * r.z;
-    pred[slotOf[q]] = make_float4(ox + px, oy + py, oz + pz, 1.0f);
+    const uint32_t sl = slotOf[q];
+    const float4 np = make_float4(ox + px, oy + py, oz + pz, 1.0f);
+    pred[sl] = np;
+    if (ofs) ofs[sl] = encode_ofs(U, cellOfSlot[sl], np.x, np.y, np.z);
   }
 }
 
@@ -1652,7 +1826,7 @@ hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s) {
 }
 
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
-                          hipStream_t s) {
+                          bool fp16, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scatter_tmp, dim3(blocks(U.n)), dim3(WG), 0, s, U.n, A.key, A.cellStart,
                      A.cellCount, A.tmp);
@@ -1660,7 +1834,8 @@ hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveV
                      A.cellStart, A.cellCount, A.pos[par], A.pred[par], A.pid[par],
                      A.pos[par ^ 1], A.pred[par ^ 1], A.pid[par ^ 1], A.order, A.slot,
                      A.slotOf, moveVel ? A.vel[par] : nullptr,
-                     moveVel ? A.vel[par ^ 1] : nullptr);
+                     moveVel ? A.vel[par ^ 1] : nullptr, fp16 ? A.ofs[0] : nullptr,
+                     A.cellOfSlot, U);
   return hipGetLastError();
 }
 
@@ -1684,6 +1859,44 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
   return run_tile1(U, A, op, t, s);
 }
 
+template <class Op>
+hipError_t run_direct16(const Uniform& U, const Arrays& A, const Op& op, const HRec* ofs,
+                        int threads, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  if (threads == 256)
+    hipLaunchKernelGGL((k_direct16<Op, 256>), dim3((U.n + 255) / 256), dim3(256), 0, s, U, op,
+                       A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
+  else if (threads == 128)
+    hipLaunchKernelGGL((k_direct16<Op, 128>), dim3((U.n + 127) / 128), dim3(128), 0, s, U, op,
+                       A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
+  else
+    hipLaunchKernelGGL((k_direct16<Op, 64>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op,
+                       A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
+  return hipGetLastError();
+}
+
+hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
+                           const HRec* ofs, int threads, hipStream_t s) {
+  Lambda16 op;
+  op.pred = pred;
+  op.density = A.density;
+  op.lambda = A.lambda;
+  op.predW = (float*)pred;
+  return run_direct16(U, A, op, ofs, threads, s);
+}
+
+hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
+                          const HRec* ofs, float4* out, HRec* ofsOut, int threads,
+                          hipStream_t s) {
+  Delta16 op;
+  op.pred = pred;
+  op.lambda = A.lambda;
+  op.out = out;
+  op.ofsOut = ofsOut;
+  op.cellOfSlot = A.cellOfSlot;
+  return run_direct16(U, A, op, ofs, threads, s);
+}
+
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s) {
   if (U.nBodies == 0) return hipSuccess;
@@ -1692,11 +1905,12 @@ hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred
   return hipGetLastError();
 }
 
-hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, hipStream_t s) {
+hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
+                             hipStream_t s) {
   if (U.nBodies == 0) return hipSuccess;
   hipLaunchKernelGGL(k_body_solve, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
                      A.bodyFirst, A.restOfPid, A.bodyInvRatio, A.bodyCentreD, A.bodyRotD,
-                     A.bodyCentre, A.bodyRot);
+                     A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U);
   return hipGetLastError();
 }
 

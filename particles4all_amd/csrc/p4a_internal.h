@@ -3,9 +3,17 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <cstdint>
 
 namespace p4a {
+
+// FP16-offset neighbour record (8 bytes): x as float, y and z as half
+// offsets from the centre of the cell the particle was binned into.
+struct __align__(8) HRec {
+  float x;
+  __half y, z;
+};
 
 // The reference's `Params` uniform (wgsl.js:4-40) with the values
 // Sim.uploadParams derives (sim.js:371-413).  Passed to every kernel by
@@ -27,6 +35,7 @@ struct Uniform {
   float bodyScale, covScale, omega;
   uint32_t sorAverage;
   float invDt, volume, tension, cohes, cohesTerm;
+  float minusOne;  // -1 as a run-time operand (keeps fused half-to-float FMAs)
 };
 
 // Device arrays of one scene.  Particle arrays are in the live
@@ -47,6 +56,8 @@ struct Arrays {
   float4* corr;
   float4* normal;
   float4* velq;         // fast path: (pred - pos)/dt and m/max(rho, rho0/2)
+  HRec* ofs[2];           // FP16 form: offsets of pred[parity] / predX
+  uint32_t* cellOfSlot; // cell each sorted slot was binned into
   uint32_t* cellCount;  // nCells+1, zero between grid builds
   uint32_t* cellStart;  // nCells+2
   uint32_t* blockSum;
@@ -119,6 +130,11 @@ struct ResizeArgs {
 hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s);
 hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s);
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
+                          bool fp16, hipStream_t s);
+hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
+                           const HRec* ofs, int threads, hipStream_t s);
+hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
+                          const HRec* ofs, float4* out, HRec* ofsOut, int threads,
                           hipStream_t s);
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
                          bool exact, const TileCfg& t, hipStream_t s);
@@ -126,7 +142,7 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
                         bool exact, const TileCfg& t, hipStream_t s);
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s);
-hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred,
+hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
                              hipStream_t s);
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, hipStream_t s);

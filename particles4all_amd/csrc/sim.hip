@@ -40,6 +40,7 @@ struct p4a_sim {
   hipStream_t stream = nullptr;
   std::string err;
   bool ready = false;
+  bool ofsValid = false;  // FP16 records match the current pred buffers
   p4a_params params{};
   // scene
   double h = 0, d = 0, mass = 0, denomRest = 0;
@@ -140,6 +141,7 @@ Uniform make_uniform(const p4a_sim* s, double dt) {
   const double h = s->h, d = p.spacing;
   Uniform U;
   std::memset(&U, 0, sizeof(U));
+  U.minusOne = -1.0f;
   for (int k = 0; k < 3; k++) {
     U.boxMin[k] = 0.0f;
     U.boxMax[k] = (float)p.box[k];
@@ -219,6 +221,64 @@ int launch(p4a_sim* s, const char* name, F&& f) {
 
 bool exact_math(const p4a_sim* s) { return s->params.math == P4A_MATH_EXACT; }
 
+// The FP16-offset form exists in the fast arithmetic only.
+const char* form_error(const p4a_params* p) {
+  if (p->math != P4A_MATH_FAST && p->math != P4A_MATH_EXACT) return "unknown math";
+  if (p->precision != P4A_PREC_FP32 && p->precision != P4A_PREC_FP16_OFFSET)
+    return "unknown precision";
+  if (p->precision == P4A_PREC_FP16_OFFSET && p->math == P4A_MATH_EXACT)
+    return "the FP16-offset form runs with P4A_MATH_FAST only";
+  return nullptr;
+}
+bool fp16_form(const p4a_sim* s) { return s->params.precision == P4A_PREC_FP16_OFFSET; }
+
+// FP16 records of a pred buffer: ofs[1] belongs to predX, ofs[0] to the
+// pred of the live parity.
+HRec* ofs_of(const p4a_sim* s, const float4* pred) {
+  return pred == s->A.predX ? s->A.ofs[1] : s->A.ofs[0];
+}
+
+// block size of the direct kernel for a tile variant (LDS variants map to 64)
+int direct_threads(const TileCfg& c) { return c.variant == 0 ? 256 : c.variant == 1 ? 128 : 64; }
+
+int stale_records(p4a_sim* s) {
+  return fail(s, P4A_ERR_ARG,
+              "FP16 records are stale: the FP16-offset form starts at a grid build "
+              "(p4a_step or P4A_STAGE_GRID)");
+}
+
+int launch_lambda_any(p4a_sim* s, const Uniform& U) {
+  if (fp16_form(s)) {
+    if (!s->ofsValid) return stale_records(s);
+    LAUNCH(s, "lambda", launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur),
+                                        direct_threads(s->tile1), s->stream));
+  } else {
+    LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, s->stream));
+  }
+  return P4A_OK;
+}
+
+int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
+  if (fp16_form(s)) {
+    if (!s->ofsValid) return stale_records(s);
+    LAUNCH(s, "delta", launch_delta16(U, s->A, s->predCur, ofs_of(s, s->predCur), out,
+                                      ofs_of(s, out), direct_threads(s->tile1), s->stream));
+  } else {
+    LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1,
+                                    s->stream));
+    s->ofsValid = false;
+  }
+  return P4A_OK;
+}
+
+int launch_body_solve_any(p4a_sim* s, const Uniform& U) {
+  if (!fp16_form(s)) s->ofsValid = false;
+  LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur,
+                                            fp16_form(s) ? ofs_of(s, s->predCur) : nullptr,
+                                            s->stream));
+  return P4A_OK;
+}
+
 float4* other_pred(p4a_sim* s, int par) {
   return s->predCur == s->A.pred[par] ? s->A.predX : s->A.pred[par];
 }
@@ -228,7 +288,8 @@ float4* other_pred(p4a_sim* s, int par) {
 int run_grid(p4a_sim* s, const Uniform& U, bool moveVel) {
   const int par = s->parity;
   LAUNCH(s, "scan", launch_scan(U, s->A, s->stream));
-  LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, s->stream));
+  LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, fp16_form(s), s->stream));
+  s->ofsValid = fp16_form(s);
   s->parity ^= 1;
   s->predCur = s->A.pred[s->parity];
   return P4A_OK;
@@ -256,15 +317,14 @@ int run_substep_body(p4a_sim* s, const Uniform& U) {
   int rc = run_grid(s, U, false);
   if (rc) return rc;
   const int par = s->parity;
-  const bool ex = exact_math(s);
   if (s->nBodies > 0) LAUNCH(s, "body_pre", launch_body_pre(U, s->A, s->predCur, s->stream));
   const int iters = std::max(1, s->params.iterations);
   for (int it = 0; it < iters; it++) {
-    LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, ex, s->tile1, s->stream));
+    if ((rc = launch_lambda_any(s, U))) return rc;
     float4* out = other_pred(s, par);
-    LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, ex, s->tile1, s->stream));
+    if ((rc = launch_delta_any(s, U, out))) return rc;
     s->predCur = out;
-    if (s->nBodies > 0) LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur, s->stream));
+    if (s->nBodies > 0 && (rc = launch_body_solve_any(s, U))) return rc;
   }
   return run_finish(s, U);
 }
@@ -272,6 +332,7 @@ int run_substep_body(p4a_sim* s, const Uniform& U) {
 std::string graph_key(const p4a_sim* s, const Uniform& U) {
   std::string k((const char*)&U, sizeof(U));
   k += std::to_string(s->params.iterations) + "/" + std::to_string(s->params.math) + "/" +
+       std::to_string(s->params.precision) + "/" +
        std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.variant) +
        "/" + std::to_string(s->tile2.variant);
   return k;
@@ -439,6 +500,8 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_BPSI: return {s->A.bpsi, s->bposCount * 4};
     case P4A_BUF_SMOOTH_POS: return {s->smoothPos, s->smoothPos ? n * 16 : 0};
     case P4A_BUF_ANISO_G: return {s->anisoG, s->anisoG ? n * 32 : 0};
+    case P4A_BUF_OFS: return {ofs_of(s, s->predCur), n * 8};
+    case P4A_BUF_CELL_OF_SLOT: return {s->A.cellOfSlot, n * 4};
     default: return {nullptr, 0};
   }
 }
@@ -546,6 +609,7 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   if (sc->n_bodies < 0 || (sc->n_bodies > 0 && (!sc->body_centre || !sc->body_density)))
     return fail(s, P4A_ERR_ARG, "body arrays missing");
   if (!(sc->h > 0) || !(p->spacing > 0)) return fail(s, P4A_ERR_ARG, "h and spacing must be > 0");
+  if (const char* fe = form_error(p)) return fail(s, P4A_ERR_ARG, fe);
   HIP_TRY(s, hipSetDevice(s->device));
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   drop_graphs(s);
@@ -587,6 +651,9 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
     ALLOC(A.pid[k], cap);
   }
   ALLOC(A.predX, cap);
+  ALLOC(A.ofs[0], cap);
+  ALLOC(A.ofs[1], cap);
+  ALLOC(A.cellOfSlot, cap);
   ALLOC(A.key, cap);
   ALLOC(A.tmp, cap);
   ALLOC(A.order, cap);
@@ -710,6 +777,7 @@ P4A_API int p4a_set_params(p4a_sim* s, const p4a_params* p) {
   int rc = check(s);
   if (rc) return rc;
   if (!p) return P4A_ERR_ARG;
+  if (const char* fe = form_error(p)) return fail(s, P4A_ERR_ARG, fe);
   p4a_params& q = s->params;
   q.substeps = p->substeps;
   q.iterations = p->iterations;
@@ -794,16 +862,16 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
       LAUNCH(s, "body_pre", launch_body_pre(U, s->A, s->predCur, s->stream));
       break;
     case P4A_STAGE_LAMBDA:
-      LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, ex, s->tile1, s->stream));
+      if ((rc = launch_lambda_any(s, U))) return rc;
       break;
     case P4A_STAGE_DELTA: {
       float4* out = other_pred(s, s->parity);
-      LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, ex, s->tile1, s->stream));
+      if ((rc = launch_delta_any(s, U, out))) return rc;
       s->predCur = out;
       break;
     }
     case P4A_STAGE_BODY_SOLVE:
-      LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur, s->stream));
+      if ((rc = launch_body_solve_any(s, U))) return rc;
       break;
     case P4A_STAGE_FINISH:
       if ((rc = run_finish(s, U))) return rc;
@@ -823,6 +891,7 @@ P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
   int rc = check(s);
   if (rc) return rc;
   if (!box) return P4A_ERR_ARG;
+  s->ofsValid = false;
   const double oldBox[3] = {s->params.box[0], s->params.box[1], s->params.box[2]};
   bool moved = false;
   for (int a = 0; a < 3; a++)
@@ -868,6 +937,7 @@ P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, i
   int rc = check(s);
   if (rc) return rc;
   if (appended) *appended = 0;
+  s->ofsValid = false;
   if (count <= 0) return P4A_OK;
   if (!pos3 || !vel3) return P4A_ERR_ARG;
   const int room = std::max(0, s->cap - s->n);
@@ -981,6 +1051,7 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   int rc = check(s);
   if (rc) return rc;
   if (!src && bytes) return P4A_ERR_ARG;
+  s->ofsValid = false;
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
       id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");

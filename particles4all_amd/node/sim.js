@@ -11,7 +11,7 @@ const { buildScene } = require('./scene');
 const BUF = {
   pos: 0, vel: 1, pred: 2, body: 3, rest: 4, lambda: 5, density: 6, slot: 7, corr: 8,
   normal: 9, cellStart: 10, bpos: 11, bcellStart: 12, bodyCentre: 13, bodyRot: 14,
-  pid: 15, key: 16, order: 17, bpsi: 18,
+  pid: 15, key: 16, order: 17, bpsi: 18, ofs: 21, cellOfSlot: 22,
 };
 const STAGE = { predict: 0, grid: 1, bodyPre: 2, lambda: 3, delta: 4, bodySolve: 5,
                 finish: 6, stats: 7 };
@@ -158,6 +158,8 @@ class Sim {
       bpos: ['f32', this.nBoundary * 4], bcellStart: ['u32', nc + 2],
       bodyCentre: ['f32', this.nBodies * 4], bodyRot: ['f32', this.nBodies * 12],
       pid: ['u32', n], key: ['u32', n], order: ['u32', n], bpsi: ['f32', this.nBoundary],
+      // FP16-offset records as raw 32-bit words: float x, then half y | half z
+      ofs: ['u32', n * 2], cellOfSlot: ['u32', n],
     }[name];
     if (!shape) throw new Error(`unknown buffer ${name}`);
     return native.read(this.handle, BUF[name], shape[0], shape[1]);

@@ -369,3 +369,53 @@ def test_aniso_on_a_lattice_is_isotropic_with_closed_form_count():
     assert off < 1e-12 * diag.max()                # G is a multiple of I
     np.testing.assert_allclose(diag, diag[:, :1] * np.ones((1, 3)), rtol=1e-12)
     np.testing.assert_allclose(G[inner, 6], diag[:, 0] ** 3, rtol=1e-12)
+
+
+def test_delta_at_moves_the_base_by_the_correction():
+    """delta_at(pred, base=pred) is delta; with another base the same
+    correction is added to it (away from the clamp)."""
+    m = lattice_mass(1.0, 2.0, 1000.0) * 1.3
+    P = make_P(box=(24.0, 20.0, 20.0), mass=m)
+    p = random_cloud()
+    key, cs, slot, order = fresh_grid(P, float4(p), O.F64)
+    ps = float4(p[order])
+    den, lam = O.lambda_(O.F64, P, ps, cs)
+    out = O.delta(O.F64, P, ps, lam, cs)
+    assert np.array_equal(O.delta_at(O.F64, P, ps, ps, lam, cs), out)
+    base = ps.copy()
+    base[:, :3] += 0.125
+    out2 = O.delta_at(O.F64, P, ps, base, lam, cs)
+    np.testing.assert_allclose(out2[:, :3] - base[:, :3], out[:, :3] - ps[:, :3], atol=1e-12)
+
+
+def test_fp16_quantize_rounds_offsets_from_cell_centres():
+    """x is kept as float; y and z offsets are taken from the cell centre
+    (boxMin + (index + 0.5) h) and rounded to nearest-even half: a position
+    at centre + k/1024 is kept exactly; centre + 0.3 moves by half(0.3) -
+    0.3 (the corner-origin offset 1.3 would round on a twice coarser grid);
+    every move is within half a half-ulp of the offset."""
+    P = make_P()  # h = 2, boxMin = 0
+    cells = np.array([0, 1, 8 + 3, 8 * 6 * 2 + 8 * 5 + 7], np.int64)
+    centres = O.cell_centres(P, cells).astype(np.float64)
+    assert np.array_equal(centres[0], [1.0, 1.0, 1.0])
+    assert np.array_equal(centres[3], [15.0, 11.0, 5.0])
+    exact = float4(centres + np.array([17, -300, 1023]) / 1024.0)
+    q = O.fp16_quantize(P, exact, cells)
+    assert np.array_equal(q[:, :3], exact[:, :3])
+    off = float4(centres + 0.3)
+    q = O.fp16_quantize(P, off, cells)
+    assert np.array_equal(q[:, 0], off[:, 0])
+    o = (off[:, 1:3] - centres[:, 1:]).astype(np.float32)  # exact: |o| < 1
+    step = o.astype(np.float16).astype(np.float64) - o
+    np.testing.assert_array_equal(q[:, 1:3] - off[:, 1:3], step)
+    o_corner = o + np.float32(1.0)
+    step_corner = o_corner.astype(np.float16).astype(np.float64) - o_corner
+    assert np.any(step_corner != step)
+    rng = np.random.default_rng(5)
+    pts = float4(centres[rng.integers(0, 4, 2000)] + rng.uniform(-1.5, 1.5, (2000, 3)))
+    cid = cells[np.argmin(np.abs(pts[:, None, :3] - centres[None]).sum(-1), axis=1)]
+    q = O.fp16_quantize(P, pts, cid)
+    assert np.array_equal(q[:, 0], pts[:, 0])
+    o = pts[:, 1:3] - O.cell_centres(P, cid)[:, 1:]
+    bound = 0.5 * np.spacing(np.abs(o).astype(np.float16)).astype(np.float64)
+    assert np.all(np.abs(q[:, 1:3] - pts[:, 1:3]) <= bound + 1e-9)

@@ -253,6 +253,10 @@ enum p4a_buf {
                               pred (float x; half y, z offsets from the
                               binned cell's centre) */
   P4A_BUF_CELL_OF_SLOT = 22, /* uint n: cell each slot was binned into */
+  P4A_BUF_DRIFT = 23,      /* float 256: per position update since the last grid
+                              build (delta, shape matching, in launch order),
+                              the largest per-axis displacement of a particle */
+  P4A_BUF_FIELD = 24,      /* float n_verts: the last p4a_field_generate */
   P4A_BUF_COUNT_
 };
 
@@ -310,6 +314,44 @@ P4A_API int p4a_aniso_params_default(p4a_aniso_params* a);
 /* Run the pass on the live positions; *fp64_solves (may be null) receives
  * the number of particles whose eigensolve took the FP64 path. */
 P4A_API int p4a_aniso(p4a_sim* h, const p4a_aniso_params* a, int32_t* fp64_solves);
+
+/* ---------------------------------------------------------------------
+ * Density field on a voxel-vertex lattice.
+ *
+ * p4a_field_configure = SurfaceMesh.configure (src/mesh.js:96-128):
+ * voxel = longest(box_max - box_min) / resolution, vert_dim = ceil(size /
+ * voxel) + 3 per axis, origin = box_min - voxel; vertex v sits at origin +
+ * voxel * v; flat index (v.z * vert_dim.y + v.y) * vert_dim.x + v.x.
+ *
+ * p4a_field_generate = SurfaceMesh.generateField (src/mesh.js:263-271):
+ *  P4A_FIELD_ISOTROPIC: meshDensityWGSL (src/mesh_wgsl.js:23-69), per
+ *    vertex p the sum over the 27 cells of the (stale) simulation grid
+ *    around p of m / max(rho_j, rho0/2) * poly6 * (h^2 - r^2)^3, r^2 < h^2,
+ *    over the live positions;
+ *  P4A_FIELD_ANISOTROPIC: the Yu-Turk anisotropic field of the last
+ *    p4a_aniso pass (no reference counterpart): sum of m / max(rho_j,
+ *    rho0/2) * 315/(64 pi) * det(G_j) * (1 - |G_j (p - smoothPos_j)|^2)^3
+ *    inside the ellipsoid |G_j (p - smoothPos_j)| < 1.  Terms are rounded to
+ *    multiples of 2^-40 and summed in 64-bit integers, so the result does
+ *    not depend on the order of the GPU's atomics.
+ * skip_bodies: 1 skips body particles, 0 keeps them, -1 skips them when
+ * the scene has bodies (mesh.js:145).  Read the result with P4A_BUF_FIELD.
+ * ------------------------------------------------------------------- */
+enum p4a_field_kind {
+  P4A_FIELD_ISOTROPIC = 0,
+  P4A_FIELD_ANISOTROPIC = 1
+};
+typedef struct p4a_field_info {
+  int32_t vert_dim[3];
+  int32_t pad_;
+  double voxel;
+  double origin[3];
+  int64_t n_verts;
+} p4a_field_info;
+P4A_API int p4a_field_configure(p4a_sim* h, const double box_min[3], const double box_max[3],
+                                int32_t resolution);
+P4A_API int p4a_field_generate(p4a_sim* h, int32_t kind, int32_t skip_bodies);
+P4A_API int p4a_field_get_info(p4a_sim* h, p4a_field_info* out);
 
 /* Wait for all queued GPU work of the handle. */
 P4A_API int p4a_sync(p4a_sim* h);

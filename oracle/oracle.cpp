@@ -725,4 +725,132 @@ void orc_stats(int n, const double* density, const double* vel, double invRho0, 
   out[1] = mr; out[2] = mv; out[3] = ke;
 }
 
+// Density field of SurfaceMesh.generateField: meshDensityWGSL
+// (mesh_wgsl.js:23-69) in float with its operation order.  Vertex v sits at
+// origin + voxel * v; the stencil is the 27 cells of the simulation grid
+// around floor(p * invSupport) (boxMin = 0 as setParams writes it,
+// mesh.js:137); per candidate j: skipped if a body particle and skipBodies;
+// vol = mass / max(rho_j, 0.5 rho0); r2 = |p - pos_j|^2; if r2 < h^2:
+// acc += vol * poly6 * t * t * t with t = h^2 - r2.
+void orc_field_iso(const OrcParams* P, const int32_t* vertDim, double voxel,
+                   const double* origin, double mass, double rho0, int n, const double* pos,
+                   const double* density, const uint32_t* body, int skipBodies,
+                   const uint32_t* cs, float* field) {
+  const int gx = P->gridDim[0], gy = P->gridDim[1], gz = P->gridDim[2];
+  const float vox = (float)voxel, ox = (float)origin[0], oy = (float)origin[1],
+              oz = (float)origin[2];
+  const float invS = (float)(1.0 / P->h), s2 = (float)(P->h * P->h), poly6 = (float)P->poly6;
+  const float m = (float)mass, r0 = (float)rho0;
+  const long nv = (long)vertDim[0] * vertDim[1] * vertDim[2];
+  (void)n;
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (long idx = 0; idx < nv; idx++) {
+    const int vx = (int)(idx % vertDim[0]);
+    const int vy = (int)((idx / vertDim[0]) % vertDim[1]);
+    const int vz = (int)(idx / ((long)vertDim[0] * vertDim[1]));
+    const float px = ox + vox * (float)vx, py = oy + vox * (float)vy, pz = oz + vox * (float)vz;
+    const int cx = std::min(std::max((int)std::floor((px - (float)P->boxMin[0]) * invS), 0), gx - 1);
+    const int cy = std::min(std::max((int)std::floor((py - (float)P->boxMin[1]) * invS), 0), gy - 1);
+    const int cz = std::min(std::max((int)std::floor((pz - (float)P->boxMin[2]) * invS), 0), gz - 1);
+    const int x0 = std::max(cx - 1, 0), x1 = std::min(cx + 1, gx - 1);
+    float acc = 0.0f;
+    for (int dz = -1; dz <= 1; dz++) {
+      const int z = cz + dz;
+      if (z < 0 || z >= gz) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int y = cy + dy;
+        if (y < 0 || y >= gy) continue;
+        const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+        for (uint32_t j = cs[rb + x0]; j < cs[rb + x1 + 1]; j++) {
+          if (skipBodies && body[j] != 0u) continue;
+          const float vol = m / std::max((float)density[j], 0.5f * r0);
+          const float rx = px - (float)pos[j * 4], ry = py - (float)pos[j * 4 + 1],
+                      rz = pz - (float)pos[j * 4 + 2];
+          const float r2 = std::fmaf(rz, rz, std::fmaf(ry, ry, rx * rx));
+          if (r2 >= s2) continue;
+          const float t = s2 - r2;
+          acc += vol * poly6 * t * t * t;
+        }
+      }
+    }
+    field[idx] = acc;
+  }
+}
+
+// Anisotropic (Yu-Turk) field of this build, restated from its definition
+// with the kernel's operation order: per particle M = G G, w = vol *
+// (315/(64 pi)) * det G (0 for a skipped body particle), bounding box
+// half-extents sqrt((M^-1)_kk) * 1.001 + 1e-6 (FP64); per vertex in the box
+// with s2 = r^T M r < 1 the term w (1 - s2)^3 rounded to a multiple of
+// 2^-40, summed exactly in 64-bit integers.
+void orc_field_aniso(const int32_t* vertDim, double voxel, const double* origin, double mass,
+                     double rho0, int n, const double* smoothPos, const double* g,
+                     const double* density, const uint32_t* body, int skipBodies,
+                     float* field) {
+  const long nv = (long)vertDim[0] * vertDim[1] * vertDim[2];
+  std::vector<long long> acc(nv, 0);
+  const float vox = (float)voxel, m = (float)mass, r0 = (float)rho0;
+  const float org[3] = {(float)origin[0], (float)origin[1], (float)origin[2]};
+  const double scale = 1099511627776.0;  // 2^40
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int j = 0; j < n; j++) {
+    const float a = (float)g[8 * j], b = (float)g[8 * j + 1], c = (float)g[8 * j + 2],
+                d = (float)g[8 * j + 3], e = (float)g[8 * j + 4], f = (float)g[8 * j + 5],
+                detG = (float)g[8 * j + 6];
+    const float m00 = a * a + b * b + c * c;
+    const float m11 = b * b + d * d + e * e;
+    const float m22 = c * c + e * e + f * f;
+    const float m01 = a * b + b * d + c * e;
+    const float m02 = a * c + b * e + c * f;
+    const float m12 = b * c + d * e + e * f;
+    const float vol = m / std::max((float)density[j], 0.5f * r0);
+    float w = vol * (1.5666814710608448f * detG);
+    if (skipBodies && body[j] != 0u) w = 0.0f;
+    const double A00 = m00, A11 = m11, A22 = m22, A01 = m01, A02 = m02, A12 = m12;
+    const double c00 = A11 * A22 - A12 * A12, c11 = A00 * A22 - A02 * A02,
+                 c22 = A00 * A11 - A01 * A01;
+    const double det = A00 * c00 - A01 * (A01 * A22 - A12 * A02) + A02 * (A01 * A12 - A11 * A02);
+    float ext[3] = {0.0f, 0.0f, 0.0f};
+    if (det > 0.0 && w > 0.0f) {
+      ext[0] = (float)(std::sqrt(std::max(c00 / det, 0.0)) * 1.001 + 1e-6);
+      ext[1] = (float)(std::sqrt(std::max(c11 / det, 0.0)) * 1.001 + 1e-6);
+      ext[2] = (float)(std::sqrt(std::max(c22 / det, 0.0)) * 1.001 + 1e-6);
+    } else {
+      w = 0.0f;
+    }
+    if (!(w > 0.0f)) continue;
+    const float ctr[3] = {(float)smoothPos[4 * j], (float)smoothPos[4 * j + 1],
+                          (float)smoothPos[4 * j + 2]};
+    bool finite = true;
+    for (int k = 0; k < 3; k++) finite = finite && std::isfinite(ctr[k]) && std::isfinite(ext[k]);
+    if (!finite) continue;
+    const float inv = 1.0f / vox;
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+      const float aa = (ctr[k] - ext[k] - org[k]) * inv, bb = (ctr[k] + ext[k] - org[k]) * inv;
+      lo[k] = std::max((int)std::ceil(std::max(aa, -1.0f)), 0);
+      hi[k] = std::min((int)std::floor(std::min(bb, (float)vertDim[k])), vertDim[k] - 1);
+    }
+    for (int z = lo[2]; z <= hi[2]; z++) {
+      const float rz = (org[2] + vox * (float)z) - ctr[2];
+      for (int y = lo[1]; y <= hi[1]; y++) {
+        const float ry = (org[1] + vox * (float)y) - ctr[1];
+        const long rowBase = ((long)z * vertDim[1] + y) * vertDim[0];
+        for (int x = lo[0]; x <= hi[0]; x++) {
+          const float rx = (org[0] + vox * (float)x) - ctr[0];
+          const float s2 = m00 * (rx * rx) + m11 * (ry * ry) + m22 * (rz * rz) +
+                           2.0f * (m01 * (rx * ry) + m02 * (rx * rz) + m12 * (ry * rz));
+          if (!(s2 < 1.0f)) continue;
+          const float u = 1.0f - s2;
+          const float term = ((w * u) * u) * u;
+          const long long q = std::llrint((double)term * scale);
+#pragma omp atomic
+          acc[rowBase + x] += q;
+        }
+      }
+    }
+  }
+  for (long i = 0; i < nv; i++) field[i] = (float)((double)acc[i] * (1.0 / scale));
+}
+
 }  // extern "C"

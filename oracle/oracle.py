@@ -216,6 +216,44 @@ def fp16_quantize(P, pred, cells):
     return q
 
 
+def field_lattice(box_min, box_max, resolution):
+    """SurfaceMesh.configure (mesh.js:96-128): (vert_dim, voxel, origin)."""
+    size = [b - a for a, b in zip(box_min, box_max)]
+    voxel = max(size) / resolution
+    vd = [int(math.ceil(s / voxel)) + 3 for s in size]
+    return vd, voxel, [a - voxel for a in box_min]
+
+
+def field_iso(P, lattice, mass, rho0, pos, density, body, skip_bodies, cs):
+    """meshDensityWGSL (mesh_wgsl.js:23-69) in float; returns the flat field."""
+    vd, voxel, origin = lattice
+    pos = np.ascontiguousarray(pos, np.float64)
+    density = np.ascontiguousarray(density, np.float64)
+    body = np.ascontiguousarray(body, np.uint32)
+    cs = np.ascontiguousarray(cs, np.uint32)
+    out = np.zeros(vd[0] * vd[1] * vd[2], np.float32)
+    vdc = np.array(vd, np.int32)
+    org = np.array(origin, np.float64)
+    lib().orc_field_iso(C.byref(P), _p(vdc), voxel, _p(org), mass, rho0, len(pos), _p(pos),
+                        _p(density), _p(body), int(skip_bodies), _p(cs), _p(out))
+    return out
+
+
+def field_aniso(lattice, mass, rho0, smooth_pos, g, density, body, skip_bodies):
+    """The anisotropic splat of smoothed centres and G (2 float4 per particle)."""
+    vd, voxel, origin = lattice
+    sp = np.ascontiguousarray(smooth_pos, np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    density = np.ascontiguousarray(density, np.float64)
+    body = np.ascontiguousarray(body, np.uint32)
+    out = np.zeros(vd[0] * vd[1] * vd[2], np.float32)
+    vdc = np.array(vd, np.int32)
+    org = np.array(origin, np.float64)
+    lib().orc_field_aniso(_p(vdc), voxel, _p(org), mass, rho0, len(sp), _p(sp), _p(g),
+                          _p(density), _p(body), int(skip_bodies), _p(out))
+    return out
+
+
 def xsph(mode, P, pred, pos, density, cs, normals):
     pred = np.ascontiguousarray(pred, np.float64)
     pos = np.ascontiguousarray(pos, np.float64)
@@ -324,6 +362,10 @@ def _setup(L):
     L.orc_lambda.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.orc_delta.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp]
     L.orc_delta_at.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_field_iso.argtypes = [vp, vp, C.c_double, vp, C.c_double, C.c_double, C.c_int, vp,
+                                vp, vp, C.c_int, vp, vp]
+    L.orc_field_aniso.argtypes = [vp, C.c_double, vp, C.c_double, C.c_double, C.c_int, vp, vp,
+                                  vp, vp, C.c_int, vp]
     L.orc_xsph.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp]
     L.orc_tension.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
     L.orc_finalize.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]

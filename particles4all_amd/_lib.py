@@ -25,7 +25,7 @@ SCENE_PRESET, SCENE_LATTICE = 0, 1
 BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CORR=8,
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
            PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20,
-           OFS=21, CELL_OF_SLOT=22)
+           OFS=21, CELL_OF_SLOT=22, DRIFT=23, FIELD=24)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -72,6 +72,11 @@ class AnisoParams(C.Structure):
                 ("gap_tol", C.c_double)]
 
 
+class FieldInfo(C.Structure):
+    _fields_ = [("vert_dim", C.c_int32 * 3), ("pad_", C.c_int32), ("voxel", C.c_double),
+                ("origin", C.c_double * 3), ("n_verts", C.c_int64)]
+
+
 class Info(C.Structure):
     _fields_ = [("n", C.c_int32), ("cap", C.c_int32), ("n_fluid", C.c_int32),
                 ("n_body_parts", C.c_int32), ("n_bodies", C.c_int32), ("n_boundary", C.c_int32),
@@ -92,7 +97,8 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_write", "p4a_device_ptr", "p4a_get_info", "p4a_stats", "p4a_body_pose",
            "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
            "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso",
-           "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name"]
+           "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name", "p4a_field_configure",
+           "p4a_field_generate", "p4a_field_get_info"]
 
 
 def lib():
@@ -127,6 +133,10 @@ def lib():
     L.p4a_hold_body.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.c_double, C.c_double]
     L.p4a_release_body.argtypes = [vp]
     L.p4a_read.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
+    L.p4a_field_configure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.c_int32]
+    L.p4a_field_generate.argtypes = [vp, C.c_int32, C.c_int32]
+    L.p4a_field_get_info.argtypes = [vp, C.POINTER(FieldInfo)]
     L.p4a_write.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
     L.p4a_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.p4a_get_info.argtypes = [vp, C.POINTER(Info)]
@@ -321,7 +331,8 @@ class Sim:
                 "KEY": (np.uint32, n, 1), "ORDER": (np.uint32, n, 1),
                 "BPSI": (np.float32, i.n_boundary, 1), "SMOOTH_POS": (np.float32, n, 4),
                 "ANISO_G": (np.float32, 2 * n, 4), "OFS": (np.uint32, n, 2),
-                "CELL_OF_SLOT": (np.uint32, n, 1)}[name]
+                "CELL_OF_SLOT": (np.uint32, n, 1), "DRIFT": (np.float32, 256, 1),
+                "FIELD": (np.float32, self.field_info().n_verts, 1)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")
@@ -395,6 +406,29 @@ class Sim:
         cnt = C.c_int32()
         self._check(lib().p4a_aniso(self.h, C.byref(a), C.byref(cnt)), "aniso")
         return cnt.value
+
+    def field_configure(self, box_min, box_max, resolution):
+        """SurfaceMesh.configure (mesh.js:96-128)."""
+        self._check(lib().p4a_field_configure(self.h, (C.c_double * 3)(*box_min),
+                                              (C.c_double * 3)(*box_max), int(resolution)),
+                    "field_configure")
+
+    def field_generate(self, anisotropic=False, skip_bodies=-1):
+        """SurfaceMesh.generateField (mesh.js:263-271); anisotropic uses the
+        last aniso() pass."""
+        self._check(lib().p4a_field_generate(self.h, 1 if anisotropic else 0, int(skip_bodies)),
+                    "field_generate")
+
+    def field_info(self):
+        f = FieldInfo()
+        self._check(lib().p4a_field_get_info(self.h, C.byref(f)), "field_info")
+        return f
+
+    def field(self):
+        """The last generated field as an array of shape (z, y, x)."""
+        f = self.field_info()
+        vd = list(f.vert_dim)
+        return self.read("FIELD").reshape(vd[2], vd[1], vd[0])
 
     def tune(self):
         """Autotune the tile shapes; returns (chosen names, {name: ms})."""

@@ -99,6 +99,51 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   return xcd * per + min(xcd, rem) + k;
 }
 
+// Records the wave's largest per-axis displacement (p -> o) into a drift
+// slot: one wave-wide max over the active lanes, and an atomic only when it
+// raises the stored bound.  Non-negative floats order like their bits; a NaN
+// or infinite displacement stores a bound that switches culling off.
+__device__ __forceinline__ void record_drift(uint32_t* out, float px, float py, float pz,
+                                             float ox, float oy, float oz) {
+  if (!out) return;
+  const float d = fmaxf(fabsf(ox - px), fmaxf(fabsf(oy - py), fabsf(oz - pz)));
+  const uint32_t m = __ockl_wfred_max_u32(__float_as_uint(d));
+  const uint64_t ex = __builtin_amdgcn_read_exec();
+  if ((uint32_t)__lane_id() == (uint32_t)__builtin_ctzll(ex) &&
+      m > *(volatile const uint32_t*)out)
+    atomicMax(out, m);
+}
+
+// Distance bounds from a target to the neighbouring cells of its stencil
+// (one cell per side).  Every particle lies within D of the cell it was
+// binned into (D = the summed drift slots plus a rounding allowance; edge
+// cells extend outward without bound, which never shortens a distance
+// measured from the inside), so a cell whose squared gap is >= h^2 holds no
+// candidate inside h: the kernels' own r^2 < h^2 test would reject every
+// one of them.  Squared gaps: x/y/z below (l) and above (r) the target's cell.
+struct CullGaps {
+  float xl, xr, yl, yr, zl, zr;
+};
+__device__ __forceinline__ CullGaps cull_gaps(const Uniform& U, Cell c, float px, float py,
+                                              float pz, float D) {
+  // float rounding of the coordinates and of the gap arithmetic: a few ulps
+  // of the largest coordinate involved (1e-6 ~ 8 ulps relative)
+  D += 1.0e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + fabsf(U.boxMin[0]) + fabsf(U.boxMin[1]) +
+                  fabsf(U.boxMin[2]) + 3.0f * U.h);
+  const float x0 = U.boxMin[0] + (float)c.x * U.h;
+  const float y0 = U.boxMin[1] + (float)c.y * U.h;
+  const float z0 = U.boxMin[2] + (float)c.z * U.h;
+  const float xl = fmaxf(px - x0 - D, 0.0f), xr = fmaxf(x0 + U.h - px - D, 0.0f);
+  const float yl = fmaxf(py - y0 - D, 0.0f), yr = fmaxf(y0 + U.h - py - D, 0.0f);
+  const float zl = fmaxf(pz - z0 - D, 0.0f), zr = fmaxf(z0 + U.h - pz - D, 0.0f);
+  return CullGaps{xl * xl, xr * xr, yl * yl, yr * yr, zl * zl, zr * zr};
+}
+__device__ __forceinline__ float drift_bound(const Uniform& U, const uint32_t* in, int n) {
+  float D = U.cullSlack;
+  for (int k = 0; k < n; k++) D += __uint_as_float(in[k]);
+  return D;
+}
+
 // ---------------------------------------------------------------------------
 // Wave helpers for runs of equal keys in a 64-lane wavefront.
 // Valid lanes form a prefix of the wave (i < n); each maximal run of equal
@@ -134,8 +179,10 @@ __device__ __forceinline__ Run wave_runs(uint32_t key, bool ok) {
 __global__ void __launch_bounds__(WG)
 k_predict_count(Uniform U, const float4* __restrict__ pos, const float4* __restrict__ vel,
                 float4* __restrict__ pred, uint32_t* __restrict__ key,
-                uint32_t* __restrict__ count) {
+                uint32_t* __restrict__ count, uint32_t* __restrict__ drift) {
   const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  static_assert(DRIFT_SLOTS <= WG, "block 0 clears the drift slots");
+  if (blockIdx.x == 0 && threadIdx.x < DRIFT_SLOTS) drift[threadIdx.x] = 0u;
   const bool ok = i < U.n;
   uint32_t c = 0xffffffffu;
   if (ok) {
@@ -304,6 +351,10 @@ struct TileLds {
 // Default hooks: the cell of a target is cellOf(pred) (wgsl.js:43-46) and
 // no row is skipped.
 struct OpBase {
+  static constexpr bool CULL = false;  // skips stencil cells beyond h (see k_direct)
+  const uint32_t* driftIn = nullptr;
+  int nDriftIn = 0;
+  uint32_t* driftOut = nullptr;
   template <class T>
   __device__ Cell cell(const Uniform& U, const T& t) const { return cell_of(U, t.px, t.py, t.pz); }
   template <class T>
@@ -473,15 +524,26 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const bool nearWall = c.x <= CR || c.x >= gx - 1 - CR || c.y <= CR || c.y >= gy - 1 - CR ||
                         c.z <= CR || c.z >= gz - 1 - CR;
   const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
+  CullGaps g{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (Op::CULL && CR == 1) {
+    if (op.driftIn)
+      g = cull_gaps(U, c, t.px, t.py, t.pz, drift_bound(U, op.driftIn, op.nDriftIn));
+  }
   for (int dz = -CR; dz <= CR; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
+    const float gz2 = dz < 0 ? g.zl : (dz > 0 ? g.zr : 0.0f);
+    if (gz2 >= U.h2) continue;
     for (int dy = -CR; dy <= CR; dy++) {
       const int y = c.y + dy;
       if (y < 0 || y >= gy) continue;
       if (op.skip_row(U, t, z, y)) continue;
+      const float g2 = gz2 + (dy < 0 ? g.yl : (dy > 0 ? g.yr : 0.0f));
+      if (g2 >= U.h2) continue;
+      const int xa = (g2 + g.xl >= U.h2) ? c.x : xlo;
+      const int xb = (g2 + g.xr >= U.h2) ? c.x : xhi;
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
-      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      const uint32_t b = cellStart[rb + xa], e = cellStart[rb + xb + 1];
       if constexpr (Op::FAST) {
         uint32_t j = b;
         for (; j + 3 < e; j += 4) {
@@ -504,7 +566,7 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         }
       }
       if (doB) {
-        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        const uint32_t bb = bcellStart[rb + xa], be = bcellStart[rb + xb + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
       }
     }
@@ -516,6 +578,7 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 template <bool EXACT>
 struct LambdaOp : OpBase {
   static constexpr bool FAST = false;
+  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -599,6 +662,7 @@ struct LambdaOp : OpBase {
 template <bool EXACT>
 struct DeltaOp : OpBase {
   static constexpr bool FAST = false;
+  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -676,6 +740,7 @@ struct DeltaOp : OpBase {
     oy = fminf(fmaxf(oy, U.clampMin[1]), U.clampMax[1]);
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
+    record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
   }
 };
 
@@ -685,6 +750,7 @@ struct DeltaOp : OpBase {
 template <bool EXACT, bool NORMALS>
 struct XsphOp : OpBase {
   static constexpr bool FAST = false;
+  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -752,6 +818,7 @@ struct XsphOp : OpBase {
 template <bool EXACT>
 struct TensionOp : OpBase {
   static constexpr bool FAST = false;
+  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -833,6 +900,7 @@ constexpr float kFar = 1.0e18f;  // tail padding: r^2 ~ 1e36 stays finite
 
 struct FastBase : OpBase {
   static constexpr bool FAST = true;
+  static constexpr bool CULL = true;
   __device__ static float4 sentinel(int k) {
     return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
@@ -956,6 +1024,7 @@ struct DeltaFast : FastBase {
     oy = fminf(fmaxf(oy, U.clampMin[1]), U.clampMax[1]);
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
+    record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
   }
 };
 
@@ -1130,16 +1199,29 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
                         c.z >= gz - 2;
   const bool doB = U.nBoundary > 0 && nearWall;
+  CullGaps g{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (op.driftIn) {
+    // distances are taken between quantized positions: each end may be off
+    // by half an fp16 ulp of an offset of magnitude <= h/2 + drift
+    const float D = drift_bound(U, op.driftIn, op.nDriftIn);
+    g = cull_gaps(U, c, t.px, t.py, t.pz, D + (U.h + 2.0f * D) * (1.0f / 1024.0f));
+  }
   for (int dz = -1; dz <= 1; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
+    const float gz2 = dz < 0 ? g.zl : (dz > 0 ? g.zr : 0.0f);
+    if (gz2 >= U.h2) continue;
     const float az = __fmaf_rn((float)(bz - z), U.h, oz);
     for (int dy = -1; dy <= 1; dy++) {
       const int y = c.y + dy;
       if (y < 0 || y >= gy) continue;
+      const float g2 = gz2 + (dy < 0 ? g.yl : (dy > 0 ? g.yr : 0.0f));
+      if (g2 >= U.h2) continue;
+      const int xa = (g2 + g.xl >= U.h2) ? c.x : xlo;
+      const int xb = (g2 + g.xr >= U.h2) ? c.x : xhi;
       const float ay = __fmaf_rn((float)(by - y), U.h, oy);
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
-      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      const uint32_t b = cellStart[rb + xa], e = cellStart[rb + xb + 1];
       uint32_t j = b;
       for (; j + 3 < e; j += 4) {
         HRec r[4];
@@ -1159,7 +1241,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         const float sy = t.py, sz = t.pz;
         t.py = cell_centre(U, 1, by) + oy;
         t.pz = cell_centre(U, 2, bz) + oz;
-        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        const uint32_t bb = bcellStart[rb + xa], be = bcellStart[rb + xb + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
         t.py = sy; t.pz = sz;
       }
@@ -1507,7 +1589,7 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
              const float4* __restrict__ restOfPid, const float* __restrict__ invRatio,
              double* __restrict__ centreD, double* __restrict__ rotD,
              float4* __restrict__ centreF, float4* __restrict__ rotF, HRec* __restrict__ ofs,
-             const uint32_t* __restrict__ cellOfSlot, Uniform U) {
+             const uint32_t* __restrict__ cellOfSlot, Uniform U, uint32_t* __restrict__ driftOut) {
   __shared__ double sh[WG / 64][15];
   __shared__ float sc[12];
   const int b = blockIdx.x;
@@ -1588,6 +1670,7 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
   __syncthreads();
   // bodyProject (wgsl.js:763-783)
   const float ox = sc[0], oy = sc[1], oz = sc[2];
+  float dmax = 0.0f;
   for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
     const uint32_t q = bodyPids[k];
     const float4 r = restOfPid[q];
@@ -1596,8 +1679,22 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
     const float pz = sc[9] * r.x + sc[10] * r.y + sc[11] * r.z;
     const uint32_t sl = slotOf[q];
     const float4 np = make_float4(ox + px, oy + py, oz + pz, 1.0f);
+    const float4 op = pred[sl];
+    dmax = fmaxf(dmax, fmaxf(fabsf(np.x - op.x), fmaxf(fabsf(np.y - op.y), fabsf(np.z - op.z))));
     pred[sl] = np;
     if (ofs) ofs[sl] = encode_ofs(U, cellOfSlot[sl], np.x, np.y, np.z);
+  }
+  if (driftOut) {
+    // largest displacement of the body's particles into the drift slot
+    __shared__ uint32_t wmax[WG / 64];
+    const uint32_t m = __ockl_wfred_max_u32(__float_as_uint(dmax));
+    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t v = wmax[0];
+      for (int k = 1; k < WG / 64; k++) v = max(v, wmax[k]);
+      atomicMax(driftOut, v);
+    }
   }
 }
 
@@ -1793,6 +1890,178 @@ hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const Tile
   }
 }
 
+// --- density field (mesh_wgsl.js:23-69, mesh.js:96-128) ---------------------
+// Per-slot record of the isotropic field: live position and
+// w = (m / max(rho, rho0/2)) * poly6, the first two operations of the
+// reference's term vol * poly6 * t * t * t; w = 0 for a body particle when
+// bodies are skipped (the reference's `continue` adds nothing, so adding
+// +0 to the non-negative sum leaves the same bits).
+__global__ void __launch_bounds__(WG)
+k_field_records(FieldArgs F, uint32_t n, const float4* __restrict__ pos,
+                const float* __restrict__ density, const uint32_t* __restrict__ pid,
+                const uint32_t* __restrict__ bodyOfPid, float4* __restrict__ rec) {
+  const uint32_t j = blockIdx.x * WG + threadIdx.x;
+  if (j >= n) return;
+  const float4 p = pos[j];
+  const float vol = F.mass / fmaxf(density[j], 0.5f * F.rho0);
+  float w = vol * F.poly6;
+  if (F.skipBodies && bodyOfPid[pid[j]] != 0u) w = 0.0f;
+  rec[j] = make_float4(p.x, p.y, p.z, w);
+}
+
+// One thread per voxel vertex p = origin + voxel * v: the 27 cells around
+// p's cell of the simulation grid (stale, as the reference reads it),
+// acc += ((w * t) * t) * t for r^2 < h^2 in the cell-sorted order.
+__global__ void __launch_bounds__(WG)
+k_field_iso(Uniform U, FieldArgs F, const uint32_t* __restrict__ cellStart,
+            const float4* __restrict__ rec, float* __restrict__ field) {
+  const uint32_t idx = blockIdx.x * WG + threadIdx.x;
+  if (idx >= F.nVerts) return;
+  const int vx = (int)(idx % (uint32_t)F.vertDim[0]);
+  const int vy = (int)((idx / (uint32_t)F.vertDim[0]) % (uint32_t)F.vertDim[1]);
+  const int vz = (int)(idx / ((uint32_t)F.vertDim[0] * (uint32_t)F.vertDim[1]));
+  const float px = F.origin[0] + F.voxel * (float)vx;
+  const float py = F.origin[1] + F.voxel * (float)vy;
+  const float pz = F.origin[2] + F.voxel * (float)vz;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  const int cx = min(max((int)floorf((px - U.boxMin[0]) * F.invSupport), 0), gx - 1);
+  const int cy = min(max((int)floorf((py - U.boxMin[1]) * F.invSupport), 0), gy - 1);
+  const int cz = min(max((int)floorf((pz - U.boxMin[2]) * F.invSupport), 0), gz - 1);
+  const int x0 = max(cx - 1, 0), x1 = min(cx + 1, gx - 1);
+  float acc = 0.0f;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int z = cz + dz;
+    if (z < 0 || z >= gz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int y = cy + dy;
+      if (y < 0 || y >= gy) continue;
+      const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+      const uint32_t b = cellStart[rb + x0], e = cellStart[rb + x1 + 1];
+      for (uint32_t j = b; j < e; j++) {
+        const float4 q = rec[j];
+        const float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
+        const float r2 = r2_of(rx, ry, rz);
+        const float t = F.support2 - r2;
+        const float term = ((q.w * t) * t) * t;
+        acc = (r2 < F.support2) ? acc + term : acc;
+      }
+    }
+  }
+  field[idx] = acc;
+}
+
+// Anisotropic field (Yu & Turk): phi(x) = sum_j V_j sigma det(G_j)
+// (1 - |G_j (x - xbar_j)|^2)^3 inside |G_j (x - xbar_j)| < 1, with
+// xbar = smoothPos, G from the anisotropy pass, sigma = 315/(64 pi) (with
+// G = I/h and xbar = x this is the isotropic poly6 field above).  Per
+// particle: M = G G (so |G r|^2 = r^T M r), w = V sigma det G, and the
+// bounding box of the ellipsoid, half-extents sqrt((M^-1)_kk) (FP64).
+// Records: A = (xbar, w), B = (M00, M11, M22, M01), and in A2 (M02, M12,
+// half-extent x, y, z) packed as B2 below.
+__global__ void __launch_bounds__(WG)
+k_field_aniso_prep(FieldArgs F, uint32_t n, const float4* __restrict__ smoothPos,
+                   const float4* __restrict__ g, const float* __restrict__ density,
+                   const uint32_t* __restrict__ pid, const uint32_t* __restrict__ bodyOfPid,
+                   float4* __restrict__ recA, float4* __restrict__ recB) {
+  const uint32_t j = blockIdx.x * WG + threadIdx.x;
+  if (j >= n) return;
+  const float4 g0 = g[2 * j], g1 = g[2 * j + 1];
+  // G symmetric: (G00, G01, G02, G11), (G12, G22, det G, count)
+  const float a = g0.x, b = g0.y, c = g0.z, d = g0.w, e = g1.x, f = g1.y;
+  const float m00 = a * a + b * b + c * c;
+  const float m11 = b * b + d * d + e * e;
+  const float m22 = c * c + e * e + f * f;
+  const float m01 = a * b + b * d + c * e;
+  const float m02 = a * c + b * e + c * f;
+  const float m12 = b * c + d * e + e * f;
+  const float vol = F.mass / fmaxf(density[j], 0.5f * F.rho0);
+  float w = vol * (1.5666814710608448f * g1.z);  // 315/(64 pi) * det G
+  if (F.skipBodies && bodyOfPid[pid[j]] != 0u) w = 0.0f;
+  // inverse of M in FP64 for the bounding box (cofactors / det)
+  const double A00 = m00, A11 = m11, A22 = m22, A01 = m01, A02 = m02, A12 = m12;
+  const double c00 = A11 * A22 - A12 * A12, c11 = A00 * A22 - A02 * A02,
+               c22 = A00 * A11 - A01 * A01;
+  const double det = A00 * c00 - A01 * (A01 * A22 - A12 * A02) + A02 * (A01 * A12 - A11 * A02);
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+  if (det > 0.0 && w > 0.0f) {
+    // 1e-3 relative margin: the voxel test below decides membership
+    ex = (float)(sqrt(fmax(c00 / det, 0.0)) * 1.001 + 1e-6);
+    ey = (float)(sqrt(fmax(c11 / det, 0.0)) * 1.001 + 1e-6);
+    ez = (float)(sqrt(fmax(c22 / det, 0.0)) * 1.001 + 1e-6);
+  } else {
+    w = 0.0f;
+  }
+  const float4 sp = smoothPos[j];
+  recA[j] = make_float4(sp.x, sp.y, sp.z, w);
+  recB[3 * j] = make_float4(m00, m11, m22, m01);
+  recB[3 * j + 1] = make_float4(m02, m12, 0.0f, 0.0f);
+  recB[3 * j + 2] = make_float4(ex, ey, ez, 0.0f);
+}
+
+// One thread per particle (threads of a wave take particles n/64 apart, so
+// their atomics rarely meet): every voxel vertex in the bounding box with
+// s^2 = r^T M r < 1 receives rint(w (1 - s^2)^3 * 2^40) by a 64-bit
+// integer atomic add.
+__global__ void __launch_bounds__(WG)
+k_field_splat(FieldArgs F, uint32_t n, const float4* __restrict__ recA,
+              const float4* __restrict__ recB, unsigned long long* __restrict__ acc) {
+  const uint32_t t = blockIdx.x * WG + threadIdx.x;
+  const uint32_t nth = gridDim.x * WG;
+  if (t >= nth) return;
+  const uint32_t stride = (n + 63u) / 64u;
+  const uint32_t j = (t % 64u) * stride + t / 64u;
+  if (j >= n || t / 64u >= stride) return;
+  const float4 A = recA[j];
+  if (!(A.w > 0.0f)) return;
+  const float4 B0 = recB[3 * j], B1 = recB[3 * j + 1], E = recB[3 * j + 2];
+  // a non-finite centre or extent would make the box span the whole field
+  if (!isfinite(A.x) || !isfinite(A.y) || !isfinite(A.z) || !isfinite(E.x) ||
+      !isfinite(E.y) || !isfinite(E.z))
+    return;
+  const float inv = 1.0f / F.voxel;
+  int lo[3], hi[3];
+  const float c[3] = {A.x, A.y, A.z}, ext[3] = {E.x, E.y, E.z};
+  for (int k = 0; k < 3; k++) {
+    const float a = (c[k] - ext[k] - F.origin[k]) * inv, b = (c[k] + ext[k] - F.origin[k]) * inv;
+    lo[k] = max((int)ceilf(fmaxf(a, -1.0f)), 0);
+    hi[k] = min((int)floorf(fminf(b, (float)F.vertDim[k])), F.vertDim[k] - 1);
+  }
+  for (int z = lo[2]; z <= hi[2]; z++) {
+    const float rz = (F.origin[2] + F.voxel * (float)z) - A.z;
+    for (int y = lo[1]; y <= hi[1]; y++) {
+      const float ry = (F.origin[1] + F.voxel * (float)y) - A.y;
+      const uint32_t rowBase = ((uint32_t)z * (uint32_t)F.vertDim[1] + (uint32_t)y) *
+                               (uint32_t)F.vertDim[0];
+      for (int x = lo[0]; x <= hi[0]; x++) {
+        const float rx = (F.origin[0] + F.voxel * (float)x) - A.x;
+        // s^2 = M00 x^2 + M11 y^2 + M22 z^2 + 2 (M01 xy + M02 xz + M12 yz)
+        const float s2 = B0.x * (rx * rx) + B0.y * (ry * ry) + B0.z * (rz * rz) +
+                         2.0f * (B0.w * (rx * ry) + B1.x * (rx * rz) + B1.y * (ry * rz));
+        if (s2 < 1.0f) {
+          const float u = 1.0f - s2;
+          const float term = ((A.w * u) * u) * u;
+          const long long q = __double2ll_rn((double)term * FIELD_FIXED_SCALE);
+          atomicAdd(&acc[rowBase + (uint32_t)x], (unsigned long long)q);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+k_field_fixed_to_float(uint32_t nv, unsigned long long* __restrict__ acc,
+                       float* __restrict__ field) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= nv) return;
+  field[i] = (float)((double)(long long)acc[i] * (1.0 / FIELD_FIXED_SCALE));
+}
+
+__global__ void __launch_bounds__(WG)
+k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
 
 int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
@@ -1811,7 +2080,7 @@ const char* tile_variant_name(int group, int v) {
 hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_predict_count, dim3(blocks(U.n)), dim3(WG), 0, s, U, A.pos[par],
-                     A.vel[par], A.pred[par], A.key, A.cellCount);
+                     A.vel[par], A.pred[par], A.key, A.cellCount, A.drift);
   return hipGetLastError();
 }
 
@@ -1839,24 +2108,24 @@ hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveV
   return hipGetLastError();
 }
 
+template <class Op>
+Op drifted(Op op, const Drift& d) {
+  op.driftIn = d.in;
+  op.nDriftIn = d.nIn;
+  op.driftOut = d.out;
+  return op;
+}
+
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, bool exact,
-                         const TileCfg& t, hipStream_t s) {
-  if (exact) {
-    LambdaOp<true> op{{}, pred, A.density, A.lambda};
-    return run_tile1(U, A, op, t, s);
-  }
-  LambdaFast op{{}, pred, A.density, A.lambda, (float*)pred};
-  return run_tile1(U, A, op, t, s);
+                         const TileCfg& t, const Drift& d, hipStream_t s) {
+  if (exact) return run_tile1(U, A, drifted(LambdaOp<true>{{}, pred, A.density, A.lambda}, d), t, s);
+  return run_tile1(U, A, drifted(LambdaFast{{}, pred, A.density, A.lambda, (float*)pred}, d), t, s);
 }
 
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
-                        bool exact, const TileCfg& t, hipStream_t s) {
-  if (exact) {
-    DeltaOp<true> op{{}, pred, A.lambda, out};
-    return run_tile1(U, A, op, t, s);
-  }
-  DeltaFast op{{}, pred, A.lambda, out};
-  return run_tile1(U, A, op, t, s);
+                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s) {
+  if (exact) return run_tile1(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, s);
+  return run_tile1(U, A, drifted(DeltaFast{{}, pred, A.lambda, out}, d), t, s);
 }
 
 template <class Op>
@@ -1876,25 +2145,25 @@ hipError_t run_direct16(const Uniform& U, const Arrays& A, const Op& op, const H
 }
 
 hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
-                           const HRec* ofs, int threads, hipStream_t s) {
+                           const HRec* ofs, int threads, const Drift& d, hipStream_t s) {
   Lambda16 op;
   op.pred = pred;
   op.density = A.density;
   op.lambda = A.lambda;
   op.predW = (float*)pred;
-  return run_direct16(U, A, op, ofs, threads, s);
+  return run_direct16(U, A, drifted(op, d), ofs, threads, s);
 }
 
 hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
-                          hipStream_t s) {
+                          const Drift& d, hipStream_t s) {
   Delta16 op;
   op.pred = pred;
   op.lambda = A.lambda;
   op.out = out;
   op.ofsOut = ofsOut;
   op.cellOfSlot = A.cellOfSlot;
-  return run_direct16(U, A, op, ofs, threads, s);
+  return run_direct16(U, A, drifted(op, d), ofs, threads, s);
 }
 
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
@@ -1906,29 +2175,34 @@ hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred
 }
 
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             hipStream_t s) {
+                             uint32_t* driftOut, hipStream_t s) {
   if (U.nBodies == 0) return hipSuccess;
   hipLaunchKernelGGL(k_body_solve, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
                      A.bodyFirst, A.restOfPid, A.bodyInvRatio, A.bodyCentreD, A.bodyRotD,
-                     A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U);
+                     A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U, driftOut);
   return hipGetLastError();
 }
 
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
-                       bool exact, bool normals, const TileCfg& t, hipStream_t s) {
+                       bool exact, bool normals, const TileCfg& t, const Drift& d,
+                       hipStream_t s) {
   float4* p = const_cast<float4*>(pos);
   if (normals) {
-    if (exact) return run_tile2(U, A, XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-    return run_tile2(U, A, XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, t, s);
+    if (exact)
+      return run_tile2(U, A, drifted(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s);
+    return run_tile2(U, A, drifted(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s);
   }
-  if (exact) return run_tile2(U, A, XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, t, s);
-  return run_tile2(U, A, XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, t, s);
+  if (exact)
+    return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s);
+  return run_tile2(U, A, drifted(XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
-                          float4* vel, bool exact, const TileCfg& t, hipStream_t s) {
-  if (exact) return run_tile2(U, A, TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, t, s);
-  return run_tile2(U, A, TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, t, s);
+                          float4* vel, bool exact, const TileCfg& t, const Drift& d,
+                          hipStream_t s) {
+  if (exact)
+    return run_tile2(U, A, drifted(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, d), t, s);
+  return run_tile2(U, A, drifted(TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, d), t, s);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
@@ -1992,6 +2266,37 @@ hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
   }
   AnisoOp<false> op{{}, pos, smoothPos, g, fallbacks, a};
   return run_direct<AnisoOp<false>, 256, 2>(U, A, op, s);
+}
+
+hipError_t launch_field_iso(const Uniform& U, const Arrays& A, const FieldArgs& F,
+                            const float4* pos, const uint32_t* pid, float4* rec, float* field,
+                            hipStream_t s) {
+  if (U.n > 0)
+    hipLaunchKernelGGL(k_field_records, dim3(blocks(U.n)), dim3(WG), 0, s, F, U.n, pos,
+                       A.density, pid, A.bodyOfPid, rec);
+  if (F.nVerts > 0)
+    hipLaunchKernelGGL(k_field_iso, dim3(blocks(F.nVerts)), dim3(WG), 0, s, U, F, A.cellStart,
+                       rec, field);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs& F,
+                              const float4* smoothPos, const float4* g, const uint32_t* pid,
+                              float4* recA, float4* recB, unsigned long long* acc,
+                              float* field, hipStream_t s) {
+  if (F.nVerts == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks(F.nVerts)), dim3(WG), 0, s, acc,
+                     (size_t)F.nVerts);
+  if (U.n > 0) {
+    hipLaunchKernelGGL(k_field_aniso_prep, dim3(blocks(U.n)), dim3(WG), 0, s, F, U.n, smoothPos,
+                       g, A.density, pid, A.bodyOfPid, recA, recB);
+    const uint32_t stride = (U.n + 63u) / 64u;
+    hipLaunchKernelGGL(k_field_splat, dim3(blocks((size_t)stride * 64u)), dim3(WG), 0, s, F,
+                       U.n, recA, recB, acc);
+  }
+  hipLaunchKernelGGL(k_field_fixed_to_float, dim3(blocks(F.nVerts)), dim3(WG), 0, s, F.nVerts,
+                     acc, field);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s) {

@@ -36,6 +36,20 @@ struct Uniform {
   uint32_t sorAverage;
   float invDt, volume, tension, cohes, cohesTerm;
   float minusOne;  // -1 as a run-time operand (keeps fused half-to-float FMAs)
+  float cullSlack; // rounding allowance of the cell-distance bounds (1e-6 h)
+};
+
+// Drift bookkeeping of the stale grid: every position update after a grid
+// build (delta, shape matching) records the largest per-axis displacement
+// of any particle into its own slot (float bits, atomicMax); a neighbour
+// kernel sums the slots written so far to bound how far any particle can be
+// from the cell it was binned into, and skips stencil cells that are then
+// provably farther than h.  Slots are cleared by predict.
+constexpr int DRIFT_SLOTS = 256;
+struct Drift {
+  const uint32_t* in = nullptr;  // slots to sum (null: walk every stencil cell)
+  int nIn = 0;
+  uint32_t* out = nullptr;       // slot the launch records into (null: none)
 };
 
 // Device arrays of one scene.  Particle arrays are in the live
@@ -57,6 +71,7 @@ struct Arrays {
   float4* normal;
   float4* velq;         // fast path: (pred - pos)/dt and m/max(rho, rho0/2)
   HRec* ofs[2];           // FP16 form: offsets of pred[parity] / predX
+  uint32_t* drift;      // DRIFT_SLOTS displacement bounds (float bits)
   uint32_t* cellOfSlot; // cell each sorted slot was binned into
   uint32_t* cellCount;  // nCells+1, zero between grid builds
   uint32_t* cellStart;  // nCells+2
@@ -90,6 +105,24 @@ struct AnisoArgs {
   int fp64Fallback;
   float gapTol;  // relative eigenvalue gap below which the FP64 solve runs
 };
+
+// Density field on a voxel-vertex lattice (SurfaceMesh.configure, mesh.js:
+// 96-128; meshDensityWGSL, mesh_wgsl.js:23-69).
+struct FieldArgs {
+  int vertDim[3];
+  float voxel;
+  float origin[3];
+  float invSupport;  // f32(1/h)
+  float support2;    // f32(h*h)
+  float poly6;       // 315/(64 pi h^9)
+  float mass, rho0;
+  uint32_t skipBodies;
+  uint32_t nVerts;
+};
+// Fixed-point scale of the anisotropic splat: every term is rounded to a
+// multiple of 2^-40 and summed with 64-bit integer atomics, so the field is
+// independent of the order the atomics resolve in.
+constexpr double FIELD_FIXED_SCALE = 1099511627776.0;  // 2^40
 
 // Launch configuration of the tiled neighbour kernels: an index into the
 // variant tables of kernels.hip (group 1: one-record kernels lambda, delta,
@@ -132,22 +165,24 @@ hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s);
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
                           bool fp16, hipStream_t s);
 hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
-                           const HRec* ofs, int threads, hipStream_t s);
+                           const HRec* ofs, int threads, const Drift& d, hipStream_t s);
 hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
-                          hipStream_t s);
+                          const Drift& d, hipStream_t s);
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
-                         bool exact, const TileCfg& t, hipStream_t s);
+                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s);
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
-                        bool exact, const TileCfg& t, hipStream_t s);
+                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s);
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s);
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             hipStream_t s);
+                             uint32_t* driftOut, hipStream_t s);
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
-                       bool exact, bool normals, const TileCfg& t, hipStream_t s);
+                       bool exact, bool normals, const TileCfg& t, const Drift& d,
+                       hipStream_t s);
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
-                          float4* vel, bool exact, const TileCfg& t, hipStream_t s);
+                          float4* vel, bool exact, const TileCfg& t, const Drift& d,
+                          hipStream_t s);
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                            float4* vel, hipStream_t s);
 hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
@@ -163,6 +198,13 @@ hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
                         const AnisoArgs& a, float4* smoothPos, float4* g, uint32_t* fallbacks,
                         bool exact, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_field_iso(const Uniform& U, const Arrays& A, const FieldArgs& F,
+                            const float4* pos, const uint32_t* pid, float4* rec, float* field,
+                            hipStream_t s);
+hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs& F,
+                              const float4* smoothPos, const float4* g, const uint32_t* pid,
+                              float4* recA, float4* recB, unsigned long long* acc,
+                              float* field, hipStream_t s);
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s);
 
 int stats_blocks(uint32_t n);

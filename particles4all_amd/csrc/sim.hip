@@ -41,6 +41,8 @@ struct p4a_sim {
   std::string err;
   bool ready = false;
   bool ofsValid = false;  // FP16 records match the current pred buffers
+  bool cullValid = false; // the grid was built from the positions the drift slots track
+  int driftCount = 0;     // drift slots written since the last grid build
   p4a_params params{};
   // scene
   double h = 0, d = 0, mass = 0, denomRest = 0;
@@ -84,6 +86,19 @@ struct p4a_sim {
   uint32_t* anisoFallbacks = nullptr;
   int anisoCap = 0;
   std::vector<void*> anisoAllocs;
+  // density field (SurfaceMesh.configure / generateField)
+  bool fieldConfigured = false;
+  int fieldVertDim[3] = {0, 0, 0};
+  double fieldVoxel = 0.0, fieldOrigin[3] = {0.0, 0.0, 0.0};
+  size_t fieldVerts = 0, fieldVertCap = 0;
+  float* field = nullptr;
+  unsigned long long* fieldAcc = nullptr;
+  std::vector<void*> fieldAllocs;
+  int fieldRecCap = 0;
+  float4* fieldRec = nullptr;
+  float4* fieldRecA = nullptr;
+  float4* fieldRecB = nullptr;
+  std::vector<void*> fieldRecAllocs;
 };
 
 namespace {
@@ -152,6 +167,7 @@ Uniform make_uniform(const p4a_sim* s, double dt) {
   U.dt = (float)dt;
   U.h = (float)h;
   U.h2 = (float)(h * h);
+  U.cullSlack = (float)(1e-6 * h);
   const double poly6 = 315.0 / (64.0 * kPi * std::pow(h, 9));
   U.poly6 = (float)poly6;
   U.spikyGrad = (float)(-45.0 / (kPi * std::pow(h, 6)));
@@ -247,27 +263,46 @@ int stale_records(p4a_sim* s) {
               "(p4a_step or P4A_STAGE_GRID)");
 }
 
+// Drift slots of the stale grid: a neighbour launch culls with the slots
+// written since the grid build; a position update records into the next.
+Drift drift_read(const p4a_sim* s) {
+  Drift d;
+  if (s->cullValid && s->driftCount <= DRIFT_SLOTS) {
+    d.in = s->A.drift;
+    d.nIn = s->driftCount;
+  }
+  return d;
+}
+uint32_t* drift_slot(const p4a_sim* s) {
+  return s->driftCount < DRIFT_SLOTS ? s->A.drift + s->driftCount : nullptr;
+}
+
 int launch_lambda_any(p4a_sim* s, const Uniform& U) {
+  const Drift d = drift_read(s);
   if (fp16_form(s)) {
     if (!s->ofsValid) return stale_records(s);
     LAUNCH(s, "lambda", launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur),
-                                        direct_threads(s->tile1), s->stream));
+                                        direct_threads(s->tile1), d, s->stream));
   } else {
-    LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, s->stream));
+    LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, d,
+                                      s->stream));
   }
   return P4A_OK;
 }
 
 int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
+  Drift d = drift_read(s);
+  d.out = drift_slot(s);
   if (fp16_form(s)) {
     if (!s->ofsValid) return stale_records(s);
     LAUNCH(s, "delta", launch_delta16(U, s->A, s->predCur, ofs_of(s, s->predCur), out,
-                                      ofs_of(s, out), direct_threads(s->tile1), s->stream));
+                                      ofs_of(s, out), direct_threads(s->tile1), d, s->stream));
   } else {
-    LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1,
+    LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1, d,
                                     s->stream));
     s->ofsValid = false;
   }
+  s->driftCount++;
   return P4A_OK;
 }
 
@@ -275,7 +310,8 @@ int launch_body_solve_any(p4a_sim* s, const Uniform& U) {
   if (!fp16_form(s)) s->ofsValid = false;
   LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur,
                                             fp16_form(s) ? ofs_of(s, s->predCur) : nullptr,
-                                            s->stream));
+                                            drift_slot(s), s->stream));
+  s->driftCount++;
   return P4A_OK;
 }
 
@@ -290,6 +326,8 @@ int run_grid(p4a_sim* s, const Uniform& U, bool moveVel) {
   LAUNCH(s, "scan", launch_scan(U, s->A, s->stream));
   LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, fp16_form(s), s->stream));
   s->ofsValid = fp16_form(s);
+  s->cullValid = true;
+  s->driftCount = 0;
   s->parity ^= 1;
   s->predCur = s->A.pred[s->parity];
   return P4A_OK;
@@ -301,10 +339,12 @@ int run_finish(p4a_sim* s, const Uniform& U) {
   const bool tension = s->params.surface_tension_k > 0;
   if (!ex)
     LAUNCH(s, "prep_finish", launch_prep_finish(U, s->A, s->predCur, s->A.pos[par], s->stream));
-  LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, s->stream));
+  const Drift d = drift_read(s);
+  LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, d,
+                                s->stream));
   if (tension)
     LAUNCH(s, "tension", launch_tension(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par], ex,
-                                        s->tile2, s->stream));
+                                        s->tile2, d, s->stream));
   else
     LAUNCH(s, "finalize", launch_finalize(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par],
                                           s->stream));
@@ -314,6 +354,7 @@ int run_finish(p4a_sim* s, const Uniform& U) {
 // One substep after the optional drag: predict .. finalize.
 int run_substep_body(p4a_sim* s, const Uniform& U) {
   LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
+  s->cullValid = false;
   int rc = run_grid(s, U, false);
   if (rc) return rc;
   const int par = s->parity;
@@ -502,6 +543,8 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_ANISO_G: return {s->anisoG, s->anisoG ? n * 32 : 0};
     case P4A_BUF_OFS: return {ofs_of(s, s->predCur), n * 8};
     case P4A_BUF_CELL_OF_SLOT: return {s->A.cellOfSlot, n * 4};
+    case P4A_BUF_DRIFT: return {s->A.drift, (size_t)DRIFT_SLOTS * 4};
+    case P4A_BUF_FIELD: return {s->field, s->field ? s->fieldVerts * 4 : 0};
     default: return {nullptr, 0};
   }
 }
@@ -589,6 +632,8 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->cellAllocs);
   free_list(s->bndAllocs);
   free_list(s->anisoAllocs);
+  free_list(s->fieldAllocs);
+  free_list(s->fieldRecAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
   hipStreamDestroy(s->stream);
@@ -617,10 +662,13 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   free_list(s->cellAllocs);
   free_list(s->bndAllocs);
   free_list(s->anisoAllocs);
+  free_list(s->fieldRecAllocs);
   s->smoothPos = nullptr;
   s->anisoG = nullptr;
   s->anisoFallbacks = nullptr;
   s->anisoCap = 0;
+  s->fieldRec = s->fieldRecA = s->fieldRecB = nullptr;
+  s->fieldRecCap = 0;
   s->ready = false;
   s->params = *p;
   s->h = sc->h;
@@ -651,6 +699,7 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
     ALLOC(A.pid[k], cap);
   }
   ALLOC(A.predX, cap);
+  ALLOC(A.drift, DRIFT_SLOTS);
   ALLOC(A.ofs[0], cap);
   ALLOC(A.ofs[1], cap);
   ALLOC(A.cellOfSlot, cap);
@@ -854,6 +903,7 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
         LAUNCH(s, "body_drag", launch_body_drag(U, s->A, s->A.vel[s->parity], D, s->stream));
       }
       LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
+      s->cullValid = false;
       break;
     case P4A_STAGE_GRID:
       if ((rc = run_grid(s, U, true))) return rc;
@@ -892,6 +942,7 @@ P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
   if (rc) return rc;
   if (!box) return P4A_ERR_ARG;
   s->ofsValid = false;
+  s->cullValid = false;
   const double oldBox[3] = {s->params.box[0], s->params.box[1], s->params.box[2]};
   bool moved = false;
   for (int a = 0; a < 3; a++)
@@ -938,6 +989,7 @@ P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, i
   if (rc) return rc;
   if (appended) *appended = 0;
   s->ofsValid = false;
+  s->cullValid = false;
   if (count <= 0) return P4A_OK;
   if (!pos3 || !vel3) return P4A_ERR_ARG;
   const int room = std::max(0, s->cap - s->n);
@@ -1052,6 +1104,7 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   if (rc) return rc;
   if (!src && bytes) return P4A_ERR_ARG;
   s->ofsValid = false;
+  s->cullValid = false;
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
       id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");
@@ -1175,6 +1228,105 @@ P4A_API int p4a_aniso(p4a_sim* s, const p4a_aniso_params* a, int32_t* fp64_solve
   return P4A_OK;
 }
 
+P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const double box_max[3],
+                                int32_t resolution) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!box_min || !box_max || resolution < 1) return P4A_ERR_ARG;
+  // SurfaceMesh.configure (mesh.js:96-128), JavaScript double arithmetic
+  double size[3], longest = 0.0;
+  for (int k = 0; k < 3; k++) {
+    size[k] = box_max[k] - box_min[k];
+    if (!(size[k] > 0)) return fail(s, P4A_ERR_ARG, "field box must have positive size");
+    longest = std::max(longest, size[k]);
+  }
+  const double voxel = longest / resolution;
+  size_t nv = 1;
+  int vd[3];
+  for (int k = 0; k < 3; k++) {
+    const double v = std::ceil(size[k] / voxel) + 2.0 + 1.0;
+    if (v > 65536.0) return fail(s, P4A_ERR_RANGE, "field too large");
+    vd[k] = (int)v;
+    nv *= (size_t)vd[k];
+  }
+  if (nv > (size_t)INT32_MAX) return fail(s, P4A_ERR_RANGE, "field too large");
+  HIP_TRY(s, hipSetDevice(s->device));
+  if (nv > s->fieldVertCap) {
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    free_list(s->fieldAllocs);
+    s->field = nullptr;
+    s->fieldAcc = nullptr;
+    s->fieldVertCap = 0;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->field, nv))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->fieldAcc, nv))) return rc;
+    s->fieldVertCap = nv;
+  }
+  for (int k = 0; k < 3; k++) {
+    s->fieldVertDim[k] = vd[k];
+    s->fieldOrigin[k] = box_min[k] - voxel;
+  }
+  s->fieldVoxel = voxel;
+  s->fieldVerts = nv;
+  s->fieldConfigured = true;
+  return P4A_OK;
+}
+
+P4A_API int p4a_field_generate(p4a_sim* s, int32_t kind, int32_t skip_bodies) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!s->fieldConfigured) return fail(s, P4A_ERR_ARG, "p4a_field_configure first");
+  if (kind != P4A_FIELD_ISOTROPIC && kind != P4A_FIELD_ANISOTROPIC) return P4A_ERR_ARG;
+  if (kind == P4A_FIELD_ANISOTROPIC && !s->smoothPos)
+    return fail(s, P4A_ERR_ARG, "the anisotropic field needs a p4a_aniso pass first");
+  HIP_TRY(s, hipSetDevice(s->device));
+  if (s->fieldRecCap < s->cap) {
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    free_list(s->fieldRecAllocs);
+    const size_t cap = (size_t)std::max(1, s->cap);
+    if ((rc = dalloc(s, s->fieldRecAllocs, &s->fieldRec, cap))) return rc;
+    if ((rc = dalloc(s, s->fieldRecAllocs, &s->fieldRecA, cap))) return rc;
+    if ((rc = dalloc(s, s->fieldRecAllocs, &s->fieldRecB, cap * 3))) return rc;
+    s->fieldRecCap = s->cap;
+  }
+  const Uniform U = make_uniform(s, 0.0);
+  FieldArgs F;
+  for (int k = 0; k < 3; k++) {
+    F.vertDim[k] = s->fieldVertDim[k];
+    F.origin[k] = (float)s->fieldOrigin[k];
+  }
+  F.voxel = (float)s->fieldVoxel;
+  F.invSupport = (float)(1.0 / s->h);
+  F.support2 = (float)(s->h * s->h);
+  F.poly6 = (float)(315.0 / (64.0 * kPi * std::pow(s->h, 9)));
+  F.mass = (float)s->mass;
+  F.rho0 = (float)s->params.rest_density;
+  F.skipBodies = skip_bodies < 0 ? (s->nBodyParts > 0 ? 1u : 0u) : (skip_bodies ? 1u : 0u);
+  F.nVerts = (uint32_t)s->fieldVerts;
+  const int par = s->parity;
+  if (kind == P4A_FIELD_ISOTROPIC)
+    LAUNCH(s, "field", launch_field_iso(U, s->A, F, s->A.pos[par], s->A.pid[par], s->fieldRec,
+                                        s->field, s->stream));
+  else
+    LAUNCH(s, "field", launch_field_aniso(U, s->A, F, s->smoothPos, s->anisoG, s->A.pid[par],
+                                          s->fieldRecA, s->fieldRecB, s->fieldAcc, s->field,
+                                          s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_field_get_info(p4a_sim* s, p4a_field_info* out) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!out) return P4A_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  for (int k = 0; k < 3; k++) {
+    out->vert_dim[k] = s->fieldVertDim[k];
+    out->origin[k] = s->fieldOrigin[k];
+  }
+  out->voxel = s->fieldVoxel;
+  out->n_verts = (int64_t)s->fieldVerts;
+  return P4A_OK;
+}
+
 P4A_API int p4a_sync(p4a_sim* s) {
   if (!s) return P4A_ERR_ARG;
   HIP_TRY(s, hipStreamSynchronize(s->stream));
@@ -1217,8 +1369,9 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
     for (int v = 0; v < tile_variants(group); v++) {
       TileCfg c{v};
       auto run = [&]() -> hipError_t {
-        if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, s->stream);
-        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, s->stream);
+        if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, drift_read(s), s->stream);
+        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, drift_read(s),
+                           s->stream);
       };
       HIP_TRY(s, run());
       HIP_TRY(s, hipEventRecord(e0, s->stream));

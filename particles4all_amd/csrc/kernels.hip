@@ -114,36 +114,6 @@ __device__ __forceinline__ void record_drift(uint32_t* out, float px, float py, 
     atomicMax(out, m);
 }
 
-// Distance bounds from a target to the neighbouring cells of its stencil
-// (one cell per side).  Every particle lies within D of the cell it was
-// binned into (D = the summed drift slots plus a rounding allowance; edge
-// cells extend outward without bound, which never shortens a distance
-// measured from the inside), so a cell whose squared gap is >= h^2 holds no
-// candidate inside h: the kernels' own r^2 < h^2 test would reject every
-// one of them.  Squared gaps: x/y/z below (l) and above (r) the target's cell.
-struct CullGaps {
-  float xl, xr, yl, yr, zl, zr;
-};
-__device__ __forceinline__ CullGaps cull_gaps(const Uniform& U, Cell c, float px, float py,
-                                              float pz, float D) {
-  // float rounding of the coordinates and of the gap arithmetic: a few ulps
-  // of the largest coordinate involved (1e-6 ~ 8 ulps relative)
-  D += 1.0e-6f * (fabsf(px) + fabsf(py) + fabsf(pz) + fabsf(U.boxMin[0]) + fabsf(U.boxMin[1]) +
-                  fabsf(U.boxMin[2]) + 3.0f * U.h);
-  const float x0 = U.boxMin[0] + (float)c.x * U.h;
-  const float y0 = U.boxMin[1] + (float)c.y * U.h;
-  const float z0 = U.boxMin[2] + (float)c.z * U.h;
-  const float xl = fmaxf(px - x0 - D, 0.0f), xr = fmaxf(x0 + U.h - px - D, 0.0f);
-  const float yl = fmaxf(py - y0 - D, 0.0f), yr = fmaxf(y0 + U.h - py - D, 0.0f);
-  const float zl = fmaxf(pz - z0 - D, 0.0f), zr = fmaxf(z0 + U.h - pz - D, 0.0f);
-  return CullGaps{xl * xl, xr * xr, yl * yl, yr * yr, zl * zl, zr * zr};
-}
-__device__ __forceinline__ float drift_bound(const Uniform& U, const uint32_t* in, int n) {
-  float D = U.cullSlack;
-  for (int k = 0; k < n; k++) D += __uint_as_float(in[k]);
-  return D;
-}
-
 // ---------------------------------------------------------------------------
 // Wave helpers for runs of equal keys in a 64-lane wavefront.
 // Valid lanes form a prefix of the wave (i < n); each maximal run of equal
@@ -351,10 +321,7 @@ struct TileLds {
 // Default hooks: the cell of a target is cellOf(pred) (wgsl.js:43-46) and
 // no row is skipped.
 struct OpBase {
-  static constexpr bool CULL = false;  // skips stencil cells beyond h (see k_direct)
-  const uint32_t* driftIn = nullptr;
-  int nDriftIn = 0;
-  uint32_t* driftOut = nullptr;
+  uint32_t* driftOut = nullptr;  // drift slot a position update records into
   template <class T>
   __device__ Cell cell(const Uniform& U, const T& t) const { return cell_of(U, t.px, t.py, t.pz); }
   template <class T>
@@ -524,26 +491,15 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const bool nearWall = c.x <= CR || c.x >= gx - 1 - CR || c.y <= CR || c.y >= gy - 1 - CR ||
                         c.z <= CR || c.z >= gz - 1 - CR;
   const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
-  CullGaps g{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if constexpr (Op::CULL && CR == 1) {
-    if (op.driftIn)
-      g = cull_gaps(U, c, t.px, t.py, t.pz, drift_bound(U, op.driftIn, op.nDriftIn));
-  }
   for (int dz = -CR; dz <= CR; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
-    const float gz2 = dz < 0 ? g.zl : (dz > 0 ? g.zr : 0.0f);
-    if (gz2 >= U.h2) continue;
     for (int dy = -CR; dy <= CR; dy++) {
       const int y = c.y + dy;
       if (y < 0 || y >= gy) continue;
       if (op.skip_row(U, t, z, y)) continue;
-      const float g2 = gz2 + (dy < 0 ? g.yl : (dy > 0 ? g.yr : 0.0f));
-      if (g2 >= U.h2) continue;
-      const int xa = (g2 + g.xl >= U.h2) ? c.x : xlo;
-      const int xb = (g2 + g.xr >= U.h2) ? c.x : xhi;
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
-      const uint32_t b = cellStart[rb + xa], e = cellStart[rb + xb + 1];
+      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
       if constexpr (Op::FAST) {
         uint32_t j = b;
         for (; j + 3 < e; j += 4) {
@@ -566,7 +522,7 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         }
       }
       if (doB) {
-        const uint32_t bb = bcellStart[rb + xa], be = bcellStart[rb + xb + 1];
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
       }
     }
@@ -578,7 +534,6 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 template <bool EXACT>
 struct LambdaOp : OpBase {
   static constexpr bool FAST = false;
-  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -662,7 +617,6 @@ struct LambdaOp : OpBase {
 template <bool EXACT>
 struct DeltaOp : OpBase {
   static constexpr bool FAST = false;
-  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -750,7 +704,6 @@ struct DeltaOp : OpBase {
 template <bool EXACT, bool NORMALS>
 struct XsphOp : OpBase {
   static constexpr bool FAST = false;
-  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -818,7 +771,6 @@ struct XsphOp : OpBase {
 template <bool EXACT>
 struct TensionOp : OpBase {
   static constexpr bool FAST = false;
-  static constexpr bool CULL = true;
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
   const float4* __restrict__ pred;
@@ -900,7 +852,6 @@ constexpr float kFar = 1.0e18f;  // tail padding: r^2 ~ 1e36 stays finite
 
 struct FastBase : OpBase {
   static constexpr bool FAST = true;
-  static constexpr bool CULL = true;
   __device__ static float4 sentinel(int k) {
     return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
@@ -1199,29 +1150,16 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
                         c.z >= gz - 2;
   const bool doB = U.nBoundary > 0 && nearWall;
-  CullGaps g{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (op.driftIn) {
-    // distances are taken between quantized positions: each end may be off
-    // by half an fp16 ulp of an offset of magnitude <= h/2 + drift
-    const float D = drift_bound(U, op.driftIn, op.nDriftIn);
-    g = cull_gaps(U, c, t.px, t.py, t.pz, D + (U.h + 2.0f * D) * (1.0f / 1024.0f));
-  }
   for (int dz = -1; dz <= 1; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
-    const float gz2 = dz < 0 ? g.zl : (dz > 0 ? g.zr : 0.0f);
-    if (gz2 >= U.h2) continue;
     const float az = __fmaf_rn((float)(bz - z), U.h, oz);
     for (int dy = -1; dy <= 1; dy++) {
       const int y = c.y + dy;
       if (y < 0 || y >= gy) continue;
-      const float g2 = gz2 + (dy < 0 ? g.yl : (dy > 0 ? g.yr : 0.0f));
-      if (g2 >= U.h2) continue;
-      const int xa = (g2 + g.xl >= U.h2) ? c.x : xlo;
-      const int xb = (g2 + g.xr >= U.h2) ? c.x : xhi;
       const float ay = __fmaf_rn((float)(by - y), U.h, oy);
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
-      const uint32_t b = cellStart[rb + xa], e = cellStart[rb + xb + 1];
+      const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
       uint32_t j = b;
       for (; j + 3 < e; j += 4) {
         HRec r[4];
@@ -1241,7 +1179,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         const float sy = t.py, sz = t.pz;
         t.py = cell_centre(U, 1, by) + oy;
         t.pz = cell_centre(U, 2, bz) + oz;
-        const uint32_t bb = bcellStart[rb + xa], be = bcellStart[rb + xb + 1];
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
         t.py = sy; t.pz = sz;
       }
@@ -2110,8 +2048,6 @@ hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveV
 
 template <class Op>
 Op drifted(Op op, const Drift& d) {
-  op.driftIn = d.in;
-  op.nDriftIn = d.nIn;
   op.driftOut = d.out;
   return op;
 }

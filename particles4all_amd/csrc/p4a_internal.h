@@ -36,20 +36,16 @@ struct Uniform {
   uint32_t sorAverage;
   float invDt, volume, tension, cohes, cohesTerm;
   float minusOne;  // -1 as a run-time operand (keeps fused half-to-float FMAs)
-  float cullSlack; // rounding allowance of the cell-distance bounds (1e-6 h)
 };
 
-// Drift bookkeeping of the stale grid: every position update after a grid
-// build (delta, shape matching) records the largest per-axis displacement
-// of any particle into its own slot (float bits, atomicMax); a neighbour
-// kernel sums the slots written so far to bound how far any particle can be
-// from the cell it was binned into, and skips stencil cells that are then
-// provably farther than h.  Slots are cleared by predict.
+// Drift record of the stale grid: every position update after a grid build
+// (delta, shape matching) records the largest per-axis displacement of any
+// particle into its own slot (float bits, atomicMax); their sum bounds how
+// far a particle can be from the cell it was binned into.  Slots are
+// cleared by predict.
 constexpr int DRIFT_SLOTS = 256;
 struct Drift {
-  const uint32_t* in = nullptr;  // slots to sum (null: walk every stencil cell)
-  int nIn = 0;
-  uint32_t* out = nullptr;       // slot the launch records into (null: none)
+  uint32_t* out = nullptr;  // slot the launch records into (null: none)
 };
 
 // Device arrays of one scene.  Particle arrays are in the live

@@ -41,7 +41,6 @@ struct p4a_sim {
   std::string err;
   bool ready = false;
   bool ofsValid = false;  // FP16 records match the current pred buffers
-  bool cullValid = false; // the grid was built from the positions the drift slots track
   int driftCount = 0;     // drift slots written since the last grid build
   p4a_params params{};
   // scene
@@ -167,7 +166,6 @@ Uniform make_uniform(const p4a_sim* s, double dt) {
   U.dt = (float)dt;
   U.h = (float)h;
   U.h2 = (float)(h * h);
-  U.cullSlack = (float)(1e-6 * h);
   const double poly6 = 315.0 / (64.0 * kPi * std::pow(h, 9));
   U.poly6 = (float)poly6;
   U.spikyGrad = (float)(-45.0 / (kPi * std::pow(h, 6)));
@@ -263,22 +261,14 @@ int stale_records(p4a_sim* s) {
               "(p4a_step or P4A_STAGE_GRID)");
 }
 
-// Drift slots of the stale grid: a neighbour launch culls with the slots
-// written since the grid build; a position update records into the next.
-Drift drift_read(const p4a_sim* s) {
-  Drift d;
-  if (s->cullValid && s->driftCount <= DRIFT_SLOTS) {
-    d.in = s->A.drift;
-    d.nIn = s->driftCount;
-  }
-  return d;
-}
+// Drift slots of the stale grid: each position update records into the
+// next slot (none past DRIFT_SLOTS); neighbour launches record nothing.
 uint32_t* drift_slot(const p4a_sim* s) {
   return s->driftCount < DRIFT_SLOTS ? s->A.drift + s->driftCount : nullptr;
 }
 
 int launch_lambda_any(p4a_sim* s, const Uniform& U) {
-  const Drift d = drift_read(s);
+  const Drift d;
   if (fp16_form(s)) {
     if (!s->ofsValid) return stale_records(s);
     LAUNCH(s, "lambda", launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur),
@@ -291,7 +281,7 @@ int launch_lambda_any(p4a_sim* s, const Uniform& U) {
 }
 
 int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
-  Drift d = drift_read(s);
+  Drift d;
   d.out = drift_slot(s);
   if (fp16_form(s)) {
     if (!s->ofsValid) return stale_records(s);
@@ -326,7 +316,6 @@ int run_grid(p4a_sim* s, const Uniform& U, bool moveVel) {
   LAUNCH(s, "scan", launch_scan(U, s->A, s->stream));
   LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, fp16_form(s), s->stream));
   s->ofsValid = fp16_form(s);
-  s->cullValid = true;
   s->driftCount = 0;
   s->parity ^= 1;
   s->predCur = s->A.pred[s->parity];
@@ -339,7 +328,7 @@ int run_finish(p4a_sim* s, const Uniform& U) {
   const bool tension = s->params.surface_tension_k > 0;
   if (!ex)
     LAUNCH(s, "prep_finish", launch_prep_finish(U, s->A, s->predCur, s->A.pos[par], s->stream));
-  const Drift d = drift_read(s);
+  const Drift d;
   LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, d,
                                 s->stream));
   if (tension)
@@ -354,7 +343,6 @@ int run_finish(p4a_sim* s, const Uniform& U) {
 // One substep after the optional drag: predict .. finalize.
 int run_substep_body(p4a_sim* s, const Uniform& U) {
   LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
-  s->cullValid = false;
   int rc = run_grid(s, U, false);
   if (rc) return rc;
   const int par = s->parity;
@@ -903,7 +891,6 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
         LAUNCH(s, "body_drag", launch_body_drag(U, s->A, s->A.vel[s->parity], D, s->stream));
       }
       LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
-      s->cullValid = false;
       break;
     case P4A_STAGE_GRID:
       if ((rc = run_grid(s, U, true))) return rc;
@@ -942,7 +929,6 @@ P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
   if (rc) return rc;
   if (!box) return P4A_ERR_ARG;
   s->ofsValid = false;
-  s->cullValid = false;
   const double oldBox[3] = {s->params.box[0], s->params.box[1], s->params.box[2]};
   bool moved = false;
   for (int a = 0; a < 3; a++)
@@ -989,7 +975,6 @@ P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, i
   if (rc) return rc;
   if (appended) *appended = 0;
   s->ofsValid = false;
-  s->cullValid = false;
   if (count <= 0) return P4A_OK;
   if (!pos3 || !vel3) return P4A_ERR_ARG;
   const int room = std::max(0, s->cap - s->n);
@@ -1104,7 +1089,6 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   if (rc) return rc;
   if (!src && bytes) return P4A_ERR_ARG;
   s->ofsValid = false;
-  s->cullValid = false;
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
       id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");
@@ -1369,8 +1353,8 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
     for (int v = 0; v < tile_variants(group); v++) {
       TileCfg c{v};
       auto run = [&]() -> hipError_t {
-        if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, drift_read(s), s->stream);
-        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, drift_read(s),
+        if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, Drift{}, s->stream);
+        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, Drift{},
                            s->stream);
       };
       HIP_TRY(s, run());

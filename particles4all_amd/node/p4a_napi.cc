@@ -508,6 +508,78 @@ napi_value Sync(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// aniso(h, {radiusScale, lambda, kr, kn, ks, stretch, minExtent, neps,
+// fp64Fallback, gapTol}) -> number of FP64 re-solves (missing keys keep
+// p4a_aniso_params_default)
+napi_value Aniso(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 2);
+  p4a_sim* h = handle(env, a[0]);
+  p4a_aniso_params q;
+  p4a_aniso_params_default(&q);
+  q.radius_scale = pnum(env, a[1], "radiusScale", q.radius_scale);
+  q.lambda = pnum(env, a[1], "lambda", q.lambda);
+  q.kr = pnum(env, a[1], "kr", q.kr);
+  q.kn = pnum(env, a[1], "kn", q.kn);
+  q.ks = pnum(env, a[1], "ks", q.ks);
+  q.stretch = pnum(env, a[1], "stretch", q.stretch);
+  q.min_extent = pnum(env, a[1], "minExtent", q.min_extent);
+  q.neps = (int32_t)pnum(env, a[1], "neps", q.neps);
+  q.fp64_fallback = (int32_t)pnum(env, a[1], "fp64Fallback", q.fp64_fallback);
+  q.gap_tol = pnum(env, a[1], "gapTol", q.gap_tol);
+  int32_t solves = 0;
+  P4A_CALL(h, p4a_aniso(h, &q, &solves), "p4a_aniso");
+  napi_value v;
+  napi_create_int32(env, solves, &v);
+  return v;
+}
+
+// fieldConfigure(h, boxMin[3], boxMax[3], resolution)
+napi_value FieldConfigure(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 4);
+  p4a_sim* h = handle(env, a[0]);
+  double lo[3], hi[3];
+  vec3(env, a[1], lo);
+  vec3(env, a[2], hi);
+  P4A_CALL(h, p4a_field_configure(h, lo, hi, (int32_t)num(env, a[3])), "p4a_field_configure");
+  return nullptr;
+}
+
+// fieldGenerate(h, anisotropic, skipBodies)
+napi_value FieldGenerate(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 3);
+  p4a_sim* h = handle(env, a[0]);
+  P4A_CALL(h, p4a_field_generate(h, num(env, a[1]) != 0 ? P4A_FIELD_ANISOTROPIC
+                                                          : P4A_FIELD_ISOTROPIC,
+                                 (int32_t)num(env, a[2], -1)),
+           "p4a_field_generate");
+  return nullptr;
+}
+
+// fieldInfo(h) -> {vertDim, voxel, origin, nVerts}
+napi_value FieldInfo(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 1);
+  p4a_sim* h = handle(env, a[0]);
+  p4a_field_info f;
+  P4A_CALL(h, p4a_field_get_info(h, &f), "p4a_field_get_info");
+  napi_value o, vd, org, v;
+  napi_create_object(env, &o);
+  napi_create_array_with_length(env, 3, &vd);
+  napi_create_array_with_length(env, 3, &org);
+  for (int k = 0; k < 3; k++) {
+    napi_create_int32(env, f.vert_dim[k], &v);
+    napi_set_element(env, vd, k, v);
+    napi_create_double(env, f.origin[k], &v);
+    napi_set_element(env, org, k, v);
+  }
+  napi_set_named_property(env, o, "vertDim", vd);
+  napi_set_named_property(env, o, "origin", org);
+  napi_create_double(env, f.voxel, &v);
+  napi_set_named_property(env, o, "voxel", v);
+  napi_create_double(env, (double)f.n_verts, &v);
+  napi_set_named_property(env, o, "nVerts", v);
+  return o;
+}
+
 napi_value Version(napi_env env, napi_callback_info) {
   napi_value s;
   napi_create_string_utf8(env, p4a_version(), NAPI_AUTO_LENGTH, &s);
@@ -528,7 +600,9 @@ napi_value Init(napi_env env, napi_value exports) {
              {"info", Info},             {"read", Read},
              {"devicePtr", DevicePtr},   {"stats", Stats},
              {"bodyPose", BodyPose},     {"runStage", RunStage},
-             {"sync", Sync},             {"version", Version}};
+             {"sync", Sync},             {"version", Version},
+             {"aniso", Aniso},           {"fieldConfigure", FieldConfigure},
+             {"fieldGenerate", FieldGenerate}, {"fieldInfo", FieldInfo}};
   for (auto& f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

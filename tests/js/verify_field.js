@@ -1,0 +1,73 @@
+'use strict';
+// GPU check of the JS SurfaceMesh surface (node/mesh.js): configure, the
+// anisotropy pass, the isotropic field against a brute-force double sum
+// over all particles at sampled vertices, and the anisotropic field's
+// total against the isotropic one (both integrate each particle's volume).
+// Prints one JSON line.
+const path = require('path');
+const root = path.join(__dirname, '..', '..', 'particles4all_amd', 'node');
+const { Sim } = require(path.join(root, 'sim.js'));
+const { SurfaceMesh } = require(path.join(root, 'mesh.js'));
+const { loadPreset } = require(path.join(root, 'settingsini.js'));
+
+const out = {};
+const { params } = loadPreset('small', 'particles=12000');
+const sim = new Sim(0);
+sim.reset(params);
+for (let f = 0; f < 10; f++) sim.step(1 / 60);
+// a grid built from the live positions (predict with dt = 0 keeps them), so
+// the gather sees every particle within h
+sim.runStage('predict', 0);
+sim.runStage('grid', 0);
+const mesh = new SurfaceMesh();
+// four voxels per h, so the lattice sums approximate the fields' integrals
+const res = Math.ceil(4 * Math.max(...sim.params.box) / sim.h);
+mesh.configure(sim, [0, 0, 0], sim.params.box, res);
+out.resolution = res;
+out.vertDim = mesh.vertDim;
+out.nVerts = mesh.nVerts;
+out.fp64Solves = mesh.buildAnisotropy(sim);
+mesh.generateField(sim);
+const iso = mesh.field(sim);
+const pos = sim.read('pos'), dens = sim.read('density'), body = sim.read('body');
+const n = sim.n, h = sim.h, h2 = h * h;
+const coef = 315 / (64 * Math.PI * Math.pow(h, 9));
+const mass = sim.scene.mass, rho0 = sim.params.restDensity;
+const skip = sim.nBodyParts > 0;
+let worst = 0, checked = 0, seed = 7;
+const vd = mesh.vertDim;
+for (let s = 0; s < 4000 && checked < 60; s++) {
+  seed = (seed * 1103515245 + 12345) & 0x7fffffff;
+  const idx = seed % mesh.nVerts;
+  if (!(iso[idx] > 1e-3)) continue;
+  const x = idx % vd[0], y = Math.floor(idx / vd[0]) % vd[1], z = Math.floor(idx / (vd[0] * vd[1]));
+  const p = [mesh.origin[0] + mesh.voxel * x, mesh.origin[1] + mesh.voxel * y,
+             mesh.origin[2] + mesh.voxel * z];
+  let acc = 0;
+  for (let j = 0; j < n; j++) {
+    if (skip && body[4 * j] !== 0) continue;
+    const dx = p[0] - pos[4 * j], dy = p[1] - pos[4 * j + 1], dz = p[2] - pos[4 * j + 2];
+    const r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 >= h2) continue;
+    const t = h2 - r2;
+    acc += mass / Math.max(dens[j], 0.5 * rho0) * coef * t * t * t;
+  }
+  worst = Math.max(worst, Math.abs(iso[idx] - acc) / acc);
+  checked++;
+}
+out.checked = checked;
+out.isoWorstRelError = worst;
+let sumIso = 0, nonFinite = 0;
+for (let i = 0; i < iso.length; i++) { sumIso += iso[i]; if (!Number.isFinite(iso[i])) nonFinite++; }
+mesh.generateField(sim, { anisotropic: true });
+const an = mesh.field(sim);
+let sumAn = 0, maxAn = 0;
+for (let i = 0; i < an.length; i++) {
+  sumAn += an[i]; maxAn = Math.max(maxAn, an[i]); if (!Number.isFinite(an[i])) nonFinite++;
+}
+const v3 = Math.pow(mesh.voxel, 3);
+out.isoIntegral = sumIso * v3;
+out.anisoIntegral = sumAn * v3;
+out.anisoMax = maxAn;
+out.nonFinite = nonFinite;
+console.log(JSON.stringify(out));

@@ -27,3 +27,16 @@ def test_js_sim_verify_harness_and_interactions():
     assert abs(out["boxX"] - 1.5) > 0.05 and out["gen"] >= 2
     assert out["impulsed"] > 0
     assert 0.9 < out["stats"]["avgRho"] < 1.1
+
+
+def test_js_surface_mesh_field():
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "verify_field.js")],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    print(out)
+    assert out["nVerts"] == out["vertDim"][0] * out["vertDim"][1] * out["vertDim"][2]
+    assert out["checked"] >= 30 and out["isoWorstRelError"] < 1e-4
+    assert out["nonFinite"] == 0 and out["anisoMax"] > 0
+    # both fields integrate each particle's volume m / max(rho, rho0/2)
+    assert abs(out["anisoIntegral"] / out["isoIntegral"] - 1) < 0.05

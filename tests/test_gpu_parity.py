@@ -337,3 +337,33 @@ def test_every_tile_variant_bit_exact(variant, monkeypatch):
     assert np.array_equal(rd(sim, "VEL"), vel)
     assert np.array_equal(rd(sim, "POS"), pos2)
     sim.close()
+
+
+def test_drift_slots_hold_the_largest_displacement():
+    """Each delta and shape-matching pass records into its own slot exactly
+    the largest per-axis displacement of any particle (a wave maximum and
+    one atomicMax per wave); predict clears the slots."""
+    sim, p, sc = scene_sim(math=1)
+    dt = 1 / 120
+    _stage_inputs(sim, p, sc, dt)
+    assert not sim.read("DRIFT").any()
+    want = []
+    sim.run_stage("BODY_PRE", dt)
+    for _ in range(p.iterations):
+        sim.run_stage("LAMBDA", dt)
+        before = sim.read("PRED")
+        sim.run_stage("DELTA", dt)
+        after = sim.read("PRED")
+        # float32 differences, as the kernels form them
+        want.append(np.abs(after[:, :3] - before[:, :3]).max())
+        sim.run_stage("BODY_SOLVE", dt)
+        moved = sim.read("PRED")
+        want.append(np.abs(moved[:, :3] - after[:, :3]).max())
+    got = sim.read("DRIFT")
+    print("drift slots", got[:len(want)])
+    assert np.array_equal(got[:len(want)], np.array(want, np.float32))
+    assert not got[len(want):].any()
+    sim.run_stage("FINISH", dt)
+    sim.run_stage("PREDICT", dt)
+    assert not sim.read("DRIFT").any()
+    sim.close()

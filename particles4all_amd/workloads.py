@@ -39,6 +39,18 @@ def dam_break_32(scale=1.0, seed=1, bodies=True, **kw):
     return p, sc, 1.0 / 60.0
 
 
+def dam_break_2m(scale=1.0, seed=5, **kw):
+    """The dam break recipe scaled up for the anisotropy and field pass:
+    a jittered 128^3 lattice (2,097,152 particles) at the corner of a 256^3
+    box, substeps 2, I = 4, XSPH 0.05, no tension, no bodies (the scaled
+    recipe is fluid only so the particle count is exactly 128^3)."""
+    L = max(4, int(round(128 * scale)))
+    B = 2 * L
+    p = _params([B, B, B], substeps=2, iterations=4, xsph=0.05, tension=0.0, **kw)
+    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed)
+    return p, sc, 1.0 / 60.0
+
+
 def block_1m(scale=1.0, seed=2, **kw):
     L = max(4, int(round(128 * scale)))
     p = _params([2 * L, L, L], substeps=2, iterations=4, xsph=0.01, tension=0.5, **kw)
@@ -71,4 +83,4 @@ def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
 
 
 WORKLOADS = {"dam_break_32": dam_break_32, "block_1m": block_1m,
-             "block_4m_bodies": block_4m_bodies}
+             "block_4m_bodies": block_4m_bodies, "dam_break_2m": dam_break_2m}

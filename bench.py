@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--settle", type=int, default=600,
                     help="substeps of settling before the field measurement")
     ap.add_argument("--resolution", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=0, metavar="SCENES",
+                    help="instead of the headline line: SCENES independent scenes batched "
+                         "against the same scenes stepped one by one (one JSON line)")
     return ap.parse_args()
 
 
@@ -269,6 +272,59 @@ def field_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args):
+    """SCENES scenes of 2e3-2e5 particles: one batched graph per substep
+    against each scene's own graph replayed in turn (synchronised after each
+    scene, so scenes do not overlap), the same substeps of the same scenes."""
+    from particles4all_amd import Batch, Sim, workloads
+    t0 = time.time()
+    items = workloads.batch_scenes(n_scenes=args.batch)
+    n_total = sum(sc.n for _, sc in items)
+    log(f"{args.batch} scenes, {n_total} particles, built in {time.time() - t0:.1f}s")
+    dt = 1.0 / 120.0
+    K = args.steps
+    b = Batch(items)
+    b.substeps(dt, args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.substeps(dt, K)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    log(f"batched: {K} substeps in {t_batch:.3f}s")
+    sims = []
+    for p, sc in items:
+        s = Sim(0)
+        s.reset(p, sc)
+        sims.append(s)
+    for s in sims:
+        s.substeps(dt, args.warmup)
+        s.sync()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.substeps(dt, K)
+        s.sync()
+    t_seq = time.perf_counter() - t0
+    log(f"one by one: {K} substeps per scene in {t_seq:.3f}s")
+    for s in sims:
+        s.close()
+    out = {
+        "metric": "batched scenes: particle-substeps/s, speed-up over scene-by-scene launches",
+        "value": n_total * K / t_batch,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": t_batch / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch_scenes: {args.batch} scenes, particle counts "
+                               "log-uniform in [2e3, 2e5], boxes U[32,128] d, 0-4 cubes, I = 4",
+                   "particles": n_total, "scenes": args.batch},
+        "sequential": {"value": n_total * K / t_seq, "ms_per_step": t_seq / K * 1e3},
+        "speedup": t_seq / t_batch,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -280,6 +336,9 @@ def main():
 
     if args.field:
         field_bench(args)
+        return
+    if args.batch:
+        batch_bench(args)
         return
     if args.copy_probe:
         nbytes = args.copy_probe << 20

@@ -405,6 +405,32 @@ P4A_API const char* p4a_tile_variant_name(int32_t group, int32_t v);
 /* Library version string. */
 P4A_API const char* p4a_version(void);
 
+/* ---------------------------------------------------------------------
+ * Batched scenes (no counterpart in the reference, which runs one scene
+ * per Sim): many independent scenes stepped together.  Every stage of a
+ * substep is one launch over all scenes (particles, cells and wall samples
+ * of all scenes concatenated; each workgroup lies in one scene and reads
+ * that scene's parameter block), and the substep is one hipGraph.  Scenes
+ * keep their own box, h, mass, gravity, XSPH, SOR and solver constants;
+ * they must share the iteration count, tension on/off and the math mode,
+ * and run the FP32 form.  Scene arrays are copied; no pour capacity.
+ * ------------------------------------------------------------------- */
+typedef struct p4a_batch p4a_batch;
+P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params* params,
+                             const p4a_scene* scenes, p4a_batch** out);
+P4A_API int p4a_batch_destroy(p4a_batch* b);
+P4A_API const char* p4a_batch_last_error(p4a_batch* b);
+/* count substeps of dt for every scene, then the per-scene stats */
+P4A_API int p4a_batch_substeps(p4a_batch* b, double dt, int32_t count);
+P4A_API int p4a_batch_sync(p4a_batch* b);
+P4A_API int p4a_batch_scene_size(p4a_batch* b, int32_t scene, int32_t* n);
+/* One scene's slice (live order) of P4A_BUF_POS, VEL, PRED, DENSITY, LAMBDA
+ * or PID (ids local to the scene). */
+P4A_API int p4a_batch_read(p4a_batch* b, int32_t scene, int32_t buf, void* dst, size_t offset,
+                           size_t bytes);
+/* p4a_stats of one scene after the last p4a_batch_substeps. */
+P4A_API int p4a_batch_stats(p4a_batch* b, int32_t scene, double out[4]);
+
 #ifdef __cplusplus
 }
 #endif

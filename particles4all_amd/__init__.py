@@ -6,7 +6,7 @@ addon); this Python package carries the ctypes plumbing used by the tests
 and the benchmark, plus the synthetic workload definitions.
 """
 from . import _lib  # noqa: F401
-from ._lib import (Sim, Scene, default_params, lattice_scene, preset_scene,  # noqa: F401
+from ._lib import (Sim, Batch, Scene, default_params, lattice_scene, preset_scene,  # noqa: F401
                    lib, BUF, STAGE)
 
-__all__ = ["Sim", "Scene", "default_params", "lattice_scene", "preset_scene", "lib"]
+__all__ = ["Sim", "Batch", "Scene", "default_params", "lattice_scene", "preset_scene", "lib"]

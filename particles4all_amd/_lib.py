@@ -98,7 +98,9 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
            "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso",
            "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name", "p4a_field_configure",
-           "p4a_field_generate", "p4a_field_get_info"]
+           "p4a_field_generate", "p4a_field_get_info", "p4a_batch_create", "p4a_batch_destroy",
+           "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_sync",
+           "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
 
 
 def lib():
@@ -137,6 +139,16 @@ def lib():
                                       C.c_int32]
     L.p4a_field_generate.argtypes = [vp, C.c_int32, C.c_int32]
     L.p4a_field_get_info.argtypes = [vp, C.POINTER(FieldInfo)]
+    L.p4a_batch_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(Params),
+                                   C.POINTER(SceneView), C.POINTER(vp)]
+    L.p4a_batch_destroy.argtypes = [vp]
+    L.p4a_batch_last_error.argtypes = [vp]
+    L.p4a_batch_last_error.restype = C.c_char_p
+    L.p4a_batch_substeps.argtypes = [vp, C.c_double, C.c_int32]
+    L.p4a_batch_sync.argtypes = [vp]
+    L.p4a_batch_scene_size.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
+    L.p4a_batch_read.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t, C.c_size_t]
+    L.p4a_batch_stats.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
     L.p4a_write.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
     L.p4a_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.p4a_get_info.argtypes = [vp, C.POINTER(Info)]
@@ -449,3 +461,61 @@ class Sim:
         b = C.c_size_t()
         self._check(lib().p4a_device_ptr(self.h, BUF[name], C.byref(p), C.byref(b)), "device_ptr")
         return p.value, b.value
+
+
+class Batch:
+    """Many independent scenes stepped together (p4a_batch_*): every stage
+    is one launch over all scenes and the substep is one hipGraph."""
+
+    def __init__(self, items, device=0):
+        """items: list of (Params, Scene)."""
+        L = lib()
+        n = len(items)
+        self._scenes = [sc for _, sc in items]  # keep the arrays alive
+        P = (Params * n)(*[p for p, _ in items])
+        V = (SceneView * n)(*[sc.view for _, sc in items])
+        h = C.c_void_p()
+        rc = L.p4a_batch_create(device, n, P, V, C.byref(h))
+        self.h = h
+        self.n_scenes = n
+        if rc:
+            msg = L.p4a_batch_last_error(h).decode() if h else ""
+            self.close()
+            raise RuntimeError(f"p4a_batch_create: {ERRORS.get(rc, rc)}: {msg}")
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().p4a_batch_last_error(self.h).decode()
+            raise RuntimeError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+    def close(self):
+        if self.h:
+            lib().p4a_batch_destroy(self.h)
+            self.h = None
+
+    def substeps(self, dt, count):
+        self._check(lib().p4a_batch_substeps(self.h, dt, int(count)), "batch_substeps")
+
+    def sync(self):
+        self._check(lib().p4a_batch_sync(self.h), "batch_sync")
+
+    def scene_size(self, s):
+        n = C.c_int32()
+        self._check(lib().p4a_batch_scene_size(self.h, s, C.byref(n)), "batch_scene_size")
+        return n.value
+
+    def read(self, s, name):
+        n = self.scene_size(s)
+        dt, c = {"POS": (np.float32, 4), "VEL": (np.float32, 4), "PRED": (np.float32, 4),
+                 "DENSITY": (np.float32, 1), "LAMBDA": (np.float32, 1),
+                 "PID": (np.uint32, 1)}[name]
+        out = np.empty((n, c) if c > 1 else (n,), dtype=dt)
+        self._check(lib().p4a_batch_read(self.h, s, BUF[name], _ptr(out), 0, out.nbytes),
+                    f"batch_read {name}")
+        return out
+
+    def stats(self, s):
+        out = (C.c_double * 4)()
+        self._check(lib().p4a_batch_stats(self.h, s, out), "batch_stats")
+        return list(out)
+

@@ -146,14 +146,10 @@ __device__ __forceinline__ Run wave_runs(uint32_t key, bool ok) {
 // v = vel + dt*(0,-g,0); pred = (pos + dt*v, 1).  The updated v itself is not
 // stored: after the grid build the reference overwrites vel with
 // (pred - pos)/dt (velFromPos, wgsl.js:72-83) before anything reads it.
-__global__ void __launch_bounds__(WG)
-k_predict_count(Uniform U, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                float4* __restrict__ pred, uint32_t* __restrict__ key,
-                uint32_t* __restrict__ count, uint32_t* __restrict__ drift) {
-  const uint32_t i = blockIdx.x * WG + threadIdx.x;
-  static_assert(DRIFT_SLOTS <= WG, "block 0 clears the drift slots");
-  if (blockIdx.x == 0 && threadIdx.x < DRIFT_SLOTS) drift[threadIdx.x] = 0u;
-  const bool ok = i < U.n;
+__device__ __forceinline__ void
+predict_body(const Uniform& U, uint32_t i, bool ok, const float4* __restrict__ pos,
+             const float4* __restrict__ vel, float4* __restrict__ pred,
+             uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
   uint32_t c = 0xffffffffu;
   if (ok) {
     float4 p = pos[i], v = vel[i];
@@ -161,11 +157,33 @@ k_predict_count(Uniform U, const float4* __restrict__ pos, const float4* __restr
     float vx = v.x + gx, vy = v.y + gy, vz = v.z + gz;
     float qx = p.x + U.dt * vx, qy = p.y + U.dt * vy, qz = p.z + U.dt * vz;
     pred[i] = make_float4(qx, qy, qz, 1.0f);
-    c = cell_index(U, cell_of(U, qx, qy, qz));
+    c = U.cBase + cell_index(U, cell_of(U, qx, qy, qz));
     key[i] = c;
   }
   Run r = wave_runs(c, ok);
   if (r.head) atomicAdd(&count[c], (uint32_t)r.len);
+}
+
+__global__ void __launch_bounds__(WG)
+k_predict_count(Uniform U, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                float4* __restrict__ pred, uint32_t* __restrict__ key,
+                uint32_t* __restrict__ count, uint32_t* __restrict__ drift) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  static_assert(DRIFT_SLOTS <= WG, "block 0 clears the drift slots");
+  if (blockIdx.x == 0 && threadIdx.x < DRIFT_SLOTS) drift[threadIdx.x] = 0u;
+  predict_body(U, i, i < U.n, pos, vel, pred, key, count);
+}
+
+// Batched: one 256-chunk of one scene per workgroup.
+__global__ void __launch_bounds__(WG)
+k_predict_count_batch(BatchTab B, const float4* __restrict__ pos,
+                      const float4* __restrict__ vel, float4* __restrict__ pred,
+                      uint32_t* __restrict__ key, uint32_t* __restrict__ count,
+                      uint32_t* __restrict__ drift) {
+  if (blockIdx.x == 0 && threadIdx.x < DRIFT_SLOTS) drift[threadIdx.x] = 0u;
+  const Uniform& U = B.U[B.chunkScene[blockIdx.x]];
+  const uint32_t local = blockIdx.x * WG - U.vBase + threadIdx.x;
+  predict_body(U, U.pBase + local, local < U.n, pos, vel, pred, key, count);
 }
 
 // Exclusive scan of cellCount into cellStart (wgsl.js:364-423), two passes:
@@ -478,11 +496,11 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // exist only in cells with some index 0 or gridDim-1, so a particle whose
 // stencil cannot reach such a cell skips the wall loops.
 template <class Op, int NT, int CR>
-__global__ void __launch_bounds__(NT)
-k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
-         const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
-  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= U.n) return;
+__device__ __forceinline__ void
+direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restrict__ cellStart,
+            const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  cellStart += U.cBase;
+  bcellStart += U.cBase;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
   typename Op::Target t;
   op.target(U, i, t);
@@ -528,6 +546,31 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
     }
   }
   op.finish(U, i, t);
+}
+
+template <class Op, int NT, int CR>
+__global__ void __launch_bounds__(NT)
+k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+         const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  direct_walk<Op, NT, CR>(U, op, i, cellStart, bcellStart, bpos);
+}
+
+// Batched scenes: each workgroup lies in one scene's 256-aligned slice of
+// the launch index space; the scene's parameter block is read from the
+// table, and its particle and cell slices are offsets into the
+// concatenated arrays (cellStart holds global particle indices).
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_direct_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
+               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  static_assert(256 % NT == 0, "a workgroup must not span two scenes");
+  const uint32_t v0 = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT;
+  const Uniform& U = B.U[B.chunkScene[v0 >> 8]];
+  const uint32_t local = v0 - U.vBase + threadIdx.x;
+  if (local >= U.n) return;
+  direct_walk<Op, NT, 1>(U, op, U.pBase + local, cellStart, bcellStart, bpos);
 }
 
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
@@ -1098,14 +1141,29 @@ struct TensionFast : FastBase {
 
 // velFromPos (wgsl.js:72-83) and the xsph weight factor, once per
 // particle: velq = ((pred - pos) / dt, m / max(rho, rho0/2)).
+__device__ __forceinline__ void prep_finish_one(const Uniform& U, uint32_t i,
+                                                const float4* __restrict__ pred,
+                                                const float4* __restrict__ pos,
+                                                const float* __restrict__ density,
+                                                float4* __restrict__ velq) {
+  const float4 p = pred[i], q = pos[i];
+  velq[i] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt,
+                        U.mass / fmaxf(density[i], 0.5f * U.rho0));
+}
 __global__ void __launch_bounds__(WG)
 k_prep_finish(Uniform U, const float4* __restrict__ pred, const float4* __restrict__ pos,
               const float* __restrict__ density, float4* __restrict__ velq) {
   const uint32_t i = blockIdx.x * WG + threadIdx.x;
   if (i >= U.n) return;
-  const float4 p = pred[i], q = pos[i];
-  velq[i] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt,
-                        U.mass / fmaxf(density[i], 0.5f * U.rho0));
+  prep_finish_one(U, i, pred, pos, density, velq);
+}
+__global__ void __launch_bounds__(WG)
+k_prep_finish_batch(BatchTab B, const float4* __restrict__ pred, const float4* __restrict__ pos,
+                    const float* __restrict__ density, float4* __restrict__ velq) {
+  const Uniform& U = B.U[B.chunkScene[blockIdx.x]];
+  const uint32_t local = blockIdx.x * WG - U.vBase + threadIdx.x;
+  if (local >= U.n) return;
+  prep_finish_one(U, U.pBase + local, pred, pos, density, velq);
 }
 
 // --- FP16-offset form (solver iterations) -----------------------------------
@@ -1451,15 +1509,30 @@ struct AnisoOp : OpBase {
 
 // finalize (wgsl.js:337-350) with velFromPos (72-83) folded in, for the
 // no-tension path.
+__device__ __forceinline__ void finalize_one(const Uniform& U, uint32_t i,
+                                             const float4* __restrict__ pred,
+                                             float4* __restrict__ pos,
+                                             const float4* __restrict__ corr,
+                                             float4* __restrict__ vel) {
+  const float4 p = pred[i], q = pos[i], c = corr[i];
+  const float vx = (p.x - q.x) * U.invDt, vy = (p.y - q.y) * U.invDt, vz = (p.z - q.z) * U.invDt;
+  vel[i] = make_float4(vx + c.x, vy + c.y, vz + c.z, 0.0f);
+  pos[i] = p;
+}
 __global__ void __launch_bounds__(WG)
 k_finalize(Uniform U, const float4* __restrict__ pred, float4* __restrict__ pos,
            const float4* __restrict__ corr, float4* __restrict__ vel) {
   const uint32_t i = blockIdx.x * WG + threadIdx.x;
   if (i >= U.n) return;
-  const float4 p = pred[i], q = pos[i], c = corr[i];
-  const float vx = (p.x - q.x) * U.invDt, vy = (p.y - q.y) * U.invDt, vz = (p.z - q.z) * U.invDt;
-  vel[i] = make_float4(vx + c.x, vy + c.y, vz + c.z, 0.0f);
-  pos[i] = p;
+  finalize_one(U, i, pred, pos, corr, vel);
+}
+__global__ void __launch_bounds__(WG)
+k_finalize_batch(BatchTab B, const float4* __restrict__ pred, float4* __restrict__ pos,
+                 const float4* __restrict__ corr, float4* __restrict__ vel) {
+  const Uniform& U = B.U[B.chunkScene[blockIdx.x]];
+  const uint32_t local = blockIdx.x * WG - U.vBase + threadIdx.x;
+  if (local >= U.n) return;
+  finalize_one(U, U.pBase + local, pred, pos, corr, vel);
 }
 
 // ---------------------------------------------------------------------------
@@ -1769,6 +1842,74 @@ k_stats_final(const double* __restrict__ part, int nb, uint32_t n, double* __res
   }
 }
 
+// Batched stats: one 256-chunk per workgroup into a partial, then one
+// workgroup per scene folds its chunks' partials in chunk order.
+__global__ void __launch_bounds__(WG)
+k_stats_partial_batch(BatchTab B, const float* __restrict__ density,
+                      const float4* __restrict__ vel, double* __restrict__ part) {
+  __shared__ double sh[WG / 64][4];
+  const Uniform& U = B.U[B.chunkScene[blockIdx.x]];
+  const uint32_t local = blockIdx.x * WG - U.vBase + threadIdx.x;
+  double sr = 0, mr = 0, mv = 0, ke = 0;
+  if (local < U.n) {
+    const uint32_t i = U.pBase + local;
+    const float rho = density[i] * U.invRho0;
+    const float4 v = vel[i];
+    const float v2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    sr = rho;
+    mr = (double)rho;
+    mv = (double)sqrtf(v2);
+    ke = (double)(0.5f * U.mass * v2);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    ke += __shfl_xor(ke, off, 64);
+    mr = fmax(mr, __shfl_xor(mr, off, 64));
+    mv = fmax(mv, __shfl_xor(mv, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sh[wave][0] = sr; sh[wave][1] = mr; sh[wave][2] = mv; sh[wave][3] = ke; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int w = 0; w < WG / 64; w++) {
+      a += sh[w][0]; b = fmax(b, sh[w][1]); c = fmax(c, sh[w][2]); d += sh[w][3];
+    }
+    part[blockIdx.x * 4] = a; part[blockIdx.x * 4 + 1] = b;
+    part[blockIdx.x * 4 + 2] = c; part[blockIdx.x * 4 + 3] = d;
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+k_stats_final_batch(BatchTab B, const double* __restrict__ part, double* __restrict__ out) {
+  const Uniform& U = B.U[blockIdx.x];
+  const uint32_t c0 = U.vBase >> 8, nc = (U.n + 255u) >> 8;
+  __shared__ double sh[WG / 64][4];
+  double sr = 0, mr = 0, mv = 0, ke = 0;
+  for (uint32_t c = threadIdx.x; c < nc; c += WG) {
+    const double* q = part + (size_t)(c0 + c) * 4;
+    sr += q[0]; mr = fmax(mr, q[1]); mv = fmax(mv, q[2]); ke += q[3];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    ke += __shfl_xor(ke, off, 64);
+    mr = fmax(mr, __shfl_xor(mr, off, 64));
+    mv = fmax(mv, __shfl_xor(mv, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { sh[wave][0] = sr; sh[wave][1] = mr; sh[wave][2] = mv; sh[wave][3] = ke; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int w = 0; w < WG / 64; w++) {
+      a += sh[w][0]; b = fmax(b, sh[w][1]); c = fmax(c, sh[w][2]); d += sh[w][3];
+    }
+    double* o = out + (size_t)blockIdx.x * 4;
+    o[0] = a / (double)(U.n > 0 ? U.n : 1u);
+    o[1] = b; o[2] = c; o[3] = d;
+  }
+}
+
 __global__ void __launch_bounds__(WG) k_copy4(const float4* __restrict__ a, float4* __restrict__ b,
                                                size_t n) {
   const size_t stride = (size_t)gridDim.x * WG;
@@ -1790,6 +1931,14 @@ hipError_t run_direct(const Uniform& U, const Arrays& A, const Op& op, hipStream
   return hipGetLastError();
 }
 
+template <class Op, int NT>
+hipError_t run_direct_batch(const BatchTab& B, const Arrays& A, const Op& op, hipStream_t s) {
+  if (B.chunks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_direct_batch<Op, NT>), dim3(B.chunks * (256u / NT)), dim3(NT), 0, s, B,
+                     op, A.cellStart, A.bcellStart, A.bpos);
+  return hipGetLastError();
+}
+
 template <class Op, int TX, int NT, int MAXREC, int CR = 1>
 hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
   const int gx = U.gridDim[0];
@@ -1807,7 +1956,10 @@ hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t
 // the variants on the live scene and keeps the fastest.
 template <class Op>
 hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
-                     hipStream_t s) {
+                     hipStream_t s, const BatchTab* B = nullptr) {
+  if (B) return t.variant == 0 ? run_direct_batch<Op, 256>(*B, A, op, s)
+                : t.variant == 1 ? run_direct_batch<Op, 128>(*B, A, op, s)
+                                 : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
     case 2: return run_direct<Op, 64>(U, A, op, s);
@@ -1818,7 +1970,10 @@ hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const Tile
 }
 template <class Op>
 hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
-                     hipStream_t s) {
+                     hipStream_t s, const BatchTab* B = nullptr) {
+  if (B) return t.variant == 0 ? run_direct_batch<Op, 256>(*B, A, op, s)
+                : t.variant == 1 ? run_direct_batch<Op, 128>(*B, A, op, s)
+                                 : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
     case 2: return run_direct<Op, 64>(U, A, op, s);
@@ -2015,7 +2170,14 @@ const char* tile_variant_name(int group, int v) {
   return group == 1 ? g1[v] : g2[v];
 }
 
-hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s) {
+hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s,
+                                const BatchTab* B) {
+  if (B) {
+    if (B->chunks > 0)
+      hipLaunchKernelGGL(k_predict_count_batch, dim3(B->chunks), dim3(WG), 0, s, *B, A.pos[par],
+                         A.vel[par], A.pred[par], A.key, A.cellCount, A.drift);
+    return hipGetLastError();
+  }
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_predict_count, dim3(blocks(U.n)), dim3(WG), 0, s, U, A.pos[par],
                      A.vel[par], A.pred[par], A.key, A.cellCount, A.drift);
@@ -2053,15 +2215,18 @@ Op drifted(Op op, const Drift& d) {
 }
 
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, bool exact,
-                         const TileCfg& t, const Drift& d, hipStream_t s) {
-  if (exact) return run_tile1(U, A, drifted(LambdaOp<true>{{}, pred, A.density, A.lambda}, d), t, s);
-  return run_tile1(U, A, drifted(LambdaFast{{}, pred, A.density, A.lambda, (float*)pred}, d), t, s);
+                         const TileCfg& t, const Drift& d, hipStream_t s, const BatchTab* B) {
+  if (exact)
+    return run_tile1(U, A, drifted(LambdaOp<true>{{}, pred, A.density, A.lambda}, d), t, s, B);
+  return run_tile1(U, A, drifted(LambdaFast{{}, pred, A.density, A.lambda, (float*)pred}, d), t, s,
+                   B);
 }
 
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
-                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s) {
-  if (exact) return run_tile1(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, s);
-  return run_tile1(U, A, drifted(DeltaFast{{}, pred, A.lambda, out}, d), t, s);
+                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
+                        const BatchTab* B) {
+  if (exact) return run_tile1(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, s, B);
+  return run_tile1(U, A, drifted(DeltaFast{{}, pred, A.lambda, out}, d), t, s, B);
 }
 
 template <class Op>
@@ -2121,42 +2286,64 @@ hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HR
 
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
-                       hipStream_t s) {
+                       hipStream_t s, const BatchTab* B) {
   float4* p = const_cast<float4*>(pos);
   if (normals) {
     if (exact)
-      return run_tile2(U, A, drifted(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s);
-    return run_tile2(U, A, drifted(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s);
+      return run_tile2(U, A, drifted(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
+    return run_tile2(U, A, drifted(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s, B);
   }
   if (exact)
-    return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s);
-  return run_tile2(U, A, drifted(XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s);
+    return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
+  return run_tile2(U, A, drifted(XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s, B);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, const Drift& d,
-                          hipStream_t s) {
+                          hipStream_t s, const BatchTab* B) {
   if (exact)
-    return run_tile2(U, A, drifted(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, d), t, s);
-  return run_tile2(U, A, drifted(TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, d), t, s);
+    return run_tile2(U, A, drifted(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, d), t, s, B);
+  return run_tile2(U, A, drifted(TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, d), t, s, B);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
-                           float4* vel, hipStream_t s) {
+                           float4* vel, hipStream_t s, const BatchTab* B) {
+  if (B) {
+    if (B->chunks > 0)
+      hipLaunchKernelGGL(k_finalize_batch, dim3(B->chunks), dim3(WG), 0, s, *B, pred, pos,
+                         A.corr, vel);
+    return hipGetLastError();
+  }
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_finalize, dim3(blocks(U.n)), dim3(WG), 0, s, U, pred, pos, A.corr, vel);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
-                              const float4* pos, hipStream_t s) {
+                              const float4* pos, hipStream_t s, const BatchTab* B) {
+  if (B) {
+    if (B->chunks > 0)
+      hipLaunchKernelGGL(k_prep_finish_batch, dim3(B->chunks), dim3(WG), 0, s, *B, pred, pos,
+                         A.density, A.velq);
+    return hipGetLastError();
+  }
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_prep_finish, dim3(blocks(U.n)), dim3(WG), 0, s, U, pred, pos, A.density,
                      A.velq);
   return hipGetLastError();
 }
 
-hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s) {
+hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s,
+                        const BatchTab* B) {
+  if (B) {
+    if (B->chunks > 0)
+      hipLaunchKernelGGL(k_stats_partial_batch, dim3(B->chunks), dim3(WG), 0, s, *B, A.density,
+                         vel, A.statsPart);
+    if (B->nScenes > 0)
+      hipLaunchKernelGGL(k_stats_final_batch, dim3(B->nScenes), dim3(WG), 0, s, *B, A.statsPart,
+                         A.statsOut);
+    return hipGetLastError();
+  }
   const int nb = stats_blocks(U.n);
   if (nb > 0)
     hipLaunchKernelGGL(k_stats_partial, dim3(nb), dim3(WG), 0, s, U, A.density, vel,

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <cstdint>
+#include <vector>
+
+#include "../../include/p4a.h"
 
 namespace p4a {
 
@@ -36,6 +39,20 @@ struct Uniform {
   uint32_t sorAverage;
   float invDt, volume, tension, cohes, cohesTerm;
   float minusOne;  // -1 as a run-time operand (keeps fused half-to-float FMAs)
+  // Batched scenes: where this scene's slice starts in the concatenated
+  // particle and cell arrays, and in the launch index space (each scene
+  // padded to a multiple of 256 so no workgroup spans two scenes).  Zero for
+  // a single scene.
+  uint32_t pBase, cBase, vBase;
+};
+
+// Scene table of a batched launch: the parameter block of every scene and
+// the scene of every 256-wide chunk of the launch index space.
+struct BatchTab {
+  const Uniform* U = nullptr;
+  const uint32_t* chunkScene = nullptr;
+  uint32_t chunks = 0;  // launch index space = chunks * 256
+  uint32_t nScenes = 0;
 };
 
 // Drift record of the stale grid: every position update after a grid build
@@ -156,7 +173,8 @@ struct ResizeArgs {
 
 // Kernel launchers (kernels.hip).  All enqueue on `s` and return the
 // hipError_t of the launch.
-hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s);
+hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipStream_t s,
+                                const BatchTab* B = nullptr);
 hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s);
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
                           bool fp16, hipStream_t s);
@@ -166,24 +184,27 @@ hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
                           const Drift& d, hipStream_t s);
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
-                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s);
+                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
+                         const BatchTab* B = nullptr);
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
-                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s);
+                        bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
+                        const BatchTab* B = nullptr);
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s);
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
                              uint32_t* driftOut, hipStream_t s);
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
-                       hipStream_t s);
+                       hipStream_t s, const BatchTab* B = nullptr);
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, const Drift& d,
-                          hipStream_t s);
+                          hipStream_t s, const BatchTab* B = nullptr);
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
-                           float4* vel, hipStream_t s);
+                           float4* vel, hipStream_t s, const BatchTab* B = nullptr);
 hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
-                              const float4* pos, hipStream_t s);
-hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s);
+                              const float4* pos, hipStream_t s, const BatchTab* B = nullptr);
+hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s,
+                        const BatchTab* B = nullptr);
 hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,
                             hipStream_t s);
 hipError_t launch_impulse(const float4* pos, float4* vel, const RayArgs& r, hipStream_t s);
@@ -204,5 +225,19 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s);
 
 int stats_blocks(uint32_t n);
+
+// Host-side restatements shared by the single-scene and batched handles.
+struct SceneConsts {
+  p4a_params params;
+  double h = 0, mass = 0, denomRest = 0, bodyScale = 0, covScale = 0;
+  int gridDim[3] = {1, 1, 1};
+  int n = 0, nCells = 0, nBoundary = 0, nBodyParts = 0, nBodies = 0;
+};
+// Sim.uploadParams (sim.js:371-413): FP64 host arithmetic, then f32.
+Uniform make_uniform_from(const SceneConsts& s, double dt);
+// sortBoundary (sim.js:255-286) into a grid of the given dimensions.
+void sort_boundary_grid(double h, const int* dim, int nCells, const float* pts, const float* psi,
+                        int nb, std::vector<float>& sorted, std::vector<float>& sortedPsi,
+                        std::vector<uint32_t>& start);
 
 }  // namespace p4a

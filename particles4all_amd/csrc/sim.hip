@@ -35,6 +35,95 @@ struct TimedLaunch {
 
 }  // namespace
 
+namespace p4a {
+
+const double kPiU = 3.14159265358979323846;
+
+// Sim.uploadParams (sim.js:371-413): FP64 host arithmetic, then f32.
+Uniform make_uniform_from(const SceneConsts& s, double dt) {
+  const p4a_params& p = s.params;
+  const double h = s.h, d = p.spacing;
+  Uniform U;
+  std::memset(&U, 0, sizeof(U));
+  U.minusOne = -1.0f;
+  for (int k = 0; k < 3; k++) {
+    U.boxMin[k] = 0.0f;
+    U.boxMax[k] = (float)p.box[k];
+    U.gridDim[k] = s.gridDim[k];
+    U.clampMin[k] = (float)(0.5 * d);
+    U.clampMax[k] = (float)(p.box[k] - 0.5 * d);
+  }
+  U.dt = (float)dt;
+  U.h = (float)h;
+  U.h2 = (float)(h * h);
+  const double poly6 = 315.0 / (64.0 * kPiU * std::pow(h, 9));
+  U.poly6 = (float)poly6;
+  U.spikyGrad = (float)(-45.0 / (kPiU * std::pow(h, 6)));
+  U.gravity = (float)p.gravity;
+  U.mass = (float)s.mass;
+  U.rho0 = (float)p.rest_density;
+  U.invRho0 = (float)(1.0 / p.rest_density);
+  U.cfmEps = (float)std::max(1e-9, p.cfm_epsilon_rel * s.denomRest);
+  U.sCorrK = (float)(p.scorr_k / (s.denomRest * std::max(1, p.iterations)));
+  const double rq = p.scorr_dq * h;
+  const double tq = h * h - rq * rq;
+  const double wq = poly6 * tq * tq * tq;
+  U.sCorrWq = (float)(wq > 0 ? 1.0 / wq : 0.0);
+  U.xsphC = (float)p.xsph_c;
+  U.n = (uint32_t)s.n;
+  U.nCells = (uint32_t)s.nCells;
+  U.nBoundary = p.no_boundary ? 0u : (uint32_t)s.nBoundary;
+  U.nBodyParts = (uint32_t)s.nBodyParts;
+  U.nBodies = (uint32_t)s.nBodies;
+  U.bodyScale = (float)s.bodyScale;
+  U.covScale = (float)s.covScale;
+  U.omega = (float)p.omega;
+  U.sorAverage = p.sor_average ? 1u : 0u;
+  U.invDt = (float)(dt > 0 ? 1.0 / dt : 0.0);
+  U.volume = (float)(s.mass / p.rest_density);
+  U.tension = (float)p.surface_tension_k;
+  U.cohes = (float)(32.0 / (kPiU * std::pow(h, 9)));
+  U.cohesTerm = (float)(std::pow(h, 6) / 64.0);
+  return U;
+}
+
+// sortBoundary (sim.js:255-286): stable counting sort of the wall samples
+// into a simulation grid, in host FP64 arithmetic on the float samples.
+void sort_boundary_grid(double h, const int* dim, int nCells, const float* pts, const float* psi,
+                        int nb, std::vector<float>& sorted, std::vector<float>& sortedPsi,
+                        std::vector<uint32_t>& start) {
+  std::vector<uint32_t> counts(nCells + 2, 0), cell(std::max(nb, 1));
+  for (int i = 0; i < nb; i++) {
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+      double f = std::floor((double)pts[i * 3 + k] / h);
+      c[k] = (int)std::min((double)dim[k] - 1, std::max(0.0, f));
+    }
+    cell[i] = (uint32_t)((c[2] * dim[1] + c[1]) * dim[0] + c[0]);
+    counts[cell[i]]++;
+  }
+  start.assign(nCells + 2, 0);
+  uint32_t run = 0;
+  for (int c = 0; c <= nCells; c++) {
+    start[c] = run;
+    run += counts[c];
+  }
+  start[nCells + 1] = run;
+  std::vector<uint32_t> cursor(start);
+  sorted.assign((size_t)std::max(1, nb) * 4, 0.0f);
+  sortedPsi.assign((size_t)std::max(1, nb), 0.0f);
+  for (int i = 0; i < nb; i++) {
+    const uint32_t k = cursor[cell[i]]++;
+    sorted[k * 4 + 0] = pts[i * 3 + 0];
+    sorted[k * 4 + 1] = pts[i * 3 + 1];
+    sorted[k * 4 + 2] = pts[i * 3 + 2];
+    sorted[k * 4 + 3] = psi[i];
+    sortedPsi[k] = psi[i];
+  }
+}
+
+}  // namespace p4a
+
 struct p4a_sim {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -149,52 +238,21 @@ void drop_graphs(p4a_sim* s) {
   }
 }
 
-// Sim.uploadParams (sim.js:371-413): FP64 host arithmetic, then f32.
 Uniform make_uniform(const p4a_sim* s, double dt) {
-  const p4a_params& p = s->params;
-  const double h = s->h, d = p.spacing;
-  Uniform U;
-  std::memset(&U, 0, sizeof(U));
-  U.minusOne = -1.0f;
-  for (int k = 0; k < 3; k++) {
-    U.boxMin[k] = 0.0f;
-    U.boxMax[k] = (float)p.box[k];
-    U.gridDim[k] = s->gridDim[k];
-    U.clampMin[k] = (float)(0.5 * d);
-    U.clampMax[k] = (float)(p.box[k] - 0.5 * d);
-  }
-  U.dt = (float)dt;
-  U.h = (float)h;
-  U.h2 = (float)(h * h);
-  const double poly6 = 315.0 / (64.0 * kPi * std::pow(h, 9));
-  U.poly6 = (float)poly6;
-  U.spikyGrad = (float)(-45.0 / (kPi * std::pow(h, 6)));
-  U.gravity = (float)p.gravity;
-  U.mass = (float)s->mass;
-  U.rho0 = (float)p.rest_density;
-  U.invRho0 = (float)(1.0 / p.rest_density);
-  U.cfmEps = (float)std::max(1e-9, p.cfm_epsilon_rel * s->denomRest);
-  U.sCorrK = (float)(p.scorr_k / (s->denomRest * std::max(1, p.iterations)));
-  const double rq = p.scorr_dq * h;
-  const double tq = h * h - rq * rq;
-  const double wq = poly6 * tq * tq * tq;
-  U.sCorrWq = (float)(wq > 0 ? 1.0 / wq : 0.0);
-  U.xsphC = (float)p.xsph_c;
-  U.n = (uint32_t)s->n;
-  U.nCells = (uint32_t)s->nCells;
-  U.nBoundary = p.no_boundary ? 0u : (uint32_t)s->nBoundary;
-  U.nBodyParts = (uint32_t)s->nBodyParts;
-  U.nBodies = (uint32_t)s->nBodies;
-  U.bodyScale = (float)s->bodyScale;
-  U.covScale = (float)s->covScale;
-  U.omega = (float)p.omega;
-  U.sorAverage = p.sor_average ? 1u : 0u;
-  U.invDt = (float)(dt > 0 ? 1.0 / dt : 0.0);
-  U.volume = (float)(s->mass / p.rest_density);
-  U.tension = (float)p.surface_tension_k;
-  U.cohes = (float)(32.0 / (kPi * std::pow(h, 9)));
-  U.cohesTerm = (float)(std::pow(h, 6) / 64.0);
-  return U;
+  SceneConsts c;
+  c.params = s->params;
+  c.h = s->h;
+  c.mass = s->mass;
+  c.denomRest = s->denomRest;
+  c.bodyScale = s->bodyScale;
+  c.covScale = s->covScale;
+  for (int k = 0; k < 3; k++) c.gridDim[k] = s->gridDim[k];
+  c.n = s->n;
+  c.nCells = s->nCells;
+  c.nBoundary = s->nBoundary;
+  c.nBodyParts = s->nBodyParts;
+  c.nBodies = s->nBodies;
+  return make_uniform_from(c, dt);
 }
 
 hipEvent_t take_event(p4a_sim* s) {
@@ -425,42 +483,10 @@ int run_stats(p4a_sim* s) {
   return P4A_OK;
 }
 
-// sortBoundary (sim.js:255-286): stable counting sort of the wall samples
-// into the simulation grid, in host FP64 arithmetic on the float samples.
 void sort_boundary(const p4a_sim* s, const float* pts, const float* psi, int nb,
                    std::vector<float>& sorted, std::vector<float>& sortedPsi,
                    std::vector<uint32_t>& start) {
-  const int* dim = s->gridDim;
-  const double h = s->h;
-  const int nCells = s->nCells;
-  std::vector<uint32_t> counts(nCells + 2, 0), cell(std::max(nb, 1));
-  for (int i = 0; i < nb; i++) {
-    int c[3];
-    for (int k = 0; k < 3; k++) {
-      double f = std::floor((double)pts[i * 3 + k] / h);
-      c[k] = (int)std::min((double)dim[k] - 1, std::max(0.0, f));
-    }
-    cell[i] = (uint32_t)((c[2] * dim[1] + c[1]) * dim[0] + c[0]);
-    counts[cell[i]]++;
-  }
-  start.assign(nCells + 2, 0);
-  uint32_t run = 0;
-  for (int c = 0; c <= nCells; c++) {
-    start[c] = run;
-    run += counts[c];
-  }
-  start[nCells + 1] = run;
-  std::vector<uint32_t> cursor(start);
-  sorted.assign((size_t)std::max(1, nb) * 4, 0.0f);
-  sortedPsi.assign((size_t)std::max(1, nb), 0.0f);
-  for (int i = 0; i < nb; i++) {
-    const uint32_t k = cursor[cell[i]]++;
-    sorted[k * 4 + 0] = pts[i * 3 + 0];
-    sorted[k * 4 + 1] = pts[i * 3 + 1];
-    sorted[k * 4 + 2] = pts[i * 3 + 2];
-    sorted[k * 4 + 3] = psi[i];
-    sortedPsi[k] = psi[i];
-  }
+  sort_boundary_grid(s->h, s->gridDim, s->nCells, pts, psi, nb, sorted, sortedPsi, start);
 }
 
 int set_grid(p4a_sim* s) {

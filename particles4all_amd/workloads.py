@@ -82,5 +82,37 @@ def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
     return p, sc, 1.0 / 120.0
 
 
+def batch_scenes(n_scenes=256, seed=4, n_min=2000, n_max=200000, math=0, **kw):
+    """Many independent scenes of uneven size: particle counts log-uniform
+    in [n_min, n_max], box extents U[32, 128] d per axis (each axis at least
+    the block plus 8 d of headroom), 0-4 lattice cubes of 4-8 particle
+    edges above the block, densities log-uniform in [0.3, 2]; I = 4, FP32,
+    XSPH 0.05, no tension, substep 1/120 s.  Returns [(params, scene)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        n = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        side = n ** (1.0 / 3.0)
+        bx = int(np.clip(round(side * rng.uniform(0.7, 1.4)), 4, 110))
+        bz = int(np.clip(round(side * rng.uniform(0.7, 1.4)), 4, 110))
+        by = int(np.clip(round(n / (bx * bz)), 4, 110))
+        box = [float(max(rng.integers(32, 129), b + 8)) for b in (bx, by, bz)]
+        box = [min(128.0, v) for v in box]
+        bx, by, bz = (int(min(b, v - 8)) for b, v in zip((bx, by, bz), box))
+        p = _params(box, substeps=2, iterations=4, xsph=0.05, tension=0.0, math=math, **kw)
+        bl = []
+        for _ in range(int(rng.integers(0, 5))):
+            e = int(rng.integers(4, 9))
+            y = min(box[1] - e, by + 0.5 * (box[1] - by))
+            cx = rng.uniform(e, box[0] - e)
+            cz = rng.uniform(e, box[2] - e)
+            dens = float(np.exp(rng.uniform(np.log(0.3), np.log(2.0))))
+            bl.append((SHAPE_LATTICE_CUBE, e, dens, (cx, y, cz)))
+        sc = lattice_scene(p, (bx, by, bz), jitter=0.05, seed=int(rng.integers(1, 1 << 30)),
+                           bodies=bl)
+        out.append((p, sc))
+    return out
+
+
 WORKLOADS = {"dam_break_32": dam_break_32, "block_1m": block_1m,
              "block_4m_bodies": block_4m_bodies, "dam_break_2m": dam_break_2m}

@@ -445,14 +445,16 @@ class Sim:
     def tune(self):
         """Autotune the tile shapes; returns (chosen names, {name: ms})."""
         ch = (C.c_int32 * 2)()
-        ms = (C.c_double * 16)()
-        self._check(lib().p4a_tune(self.h, ch, ms, 16), "tune")
+        ms = (C.c_double * 32)()
+        self._check(lib().p4a_tune(self.h, ch, ms, 32), "tune")
         L = lib()
         times, k = {}, 0
         for g in (1, 2):
-            for v in range(5):
+            v = 0
+            while L.p4a_tile_variant_name(g, v).decode() != "?":
                 times[f"g{g}:" + L.p4a_tile_variant_name(g, v).decode()] = ms[k]
                 k += 1
+                v += 1
         return ([L.p4a_tile_variant_name(1, ch[0]).decode(),
                  L.p4a_tile_variant_name(2, ch[1]).decode()], times)
 

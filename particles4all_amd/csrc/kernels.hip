@@ -1,30 +1,35 @@
 // gfx950 kernels of the Particles4All simulation step.
 //
-// Reference kernels (src/wgsl.js) and their counterparts here:
+// Reference kernels (src/wgsl.js unless named) and their counterparts here:
 //   predict 56-70 + count 352-362        -> k_predict_count (fused)
 //   scanBlock/scanBlocks/scanAdd 364-423 -> k_scan_reduce + k_scan_add
 //   scatterSlot 425-438 (atomic slot)    -> k_scatter_tmp + k_sort_move
 //   scatterMove/scatterPhase 440-475        (stable order, fused move)
-//   lambda 477-543                       -> k_tile<LambdaOp>
-//   delta 545-612                        -> k_tile<DeltaOp>
+//   lambda 477-543                       -> k_direct<LambdaOp/LambdaFast>
+//   delta 545-612                        -> k_direct<DeltaOp/DeltaFast>
 //   bodyCentre/bodySeed/bodyRefLoad      -> k_body_pre      (FP64)
 //   bodyCentre/Seed/Cov/Resolve/Project  -> k_body_solve    (FP64)
 //   velFromPos 72-83, xsph 85-123,
-//   normals 250-288                      -> k_tile<XsphNormalsOp> / <XsphOp>
-//   tension 290-335 + finalize 337-350   -> k_tile<TensionFinalizeOp>
+//   normals 250-288                      -> k_prep_finish + k_direct<XsphOp/XsphFast>
+//   tension 290-335 + finalize 337-350   -> k_direct<TensionOp/TensionFast>
 //   finalize 337-350                     -> k_finalize (no tension)
 //   stats 785-832                        -> k_stats_partial + k_stats_final
 //   bodyDrag 187-214, impulse 216-248, resize 125-157 -> k_body_drag,
 //                                           k_impulse, k_resize
+//   anisoWGSL (aniso_wgsl.js:3-159)      -> k_tile<AnisoOp> (5x5x5 cells)
+//   meshDensityWGSL (mesh_wgsl.js:23-69) -> k_field_records + k_field_iso
+// Forms of this build without a reference kernel: the FP16-offset solver
+// iterations (k_direct16<Lambda16/Delta16>), the anisotropic field splat
+// (k_field_aniso_prep + k_field_splat) and the batched-scene wrappers
+// (*_batch).
 //
-// Neighbour kernels work on row tiles: a workgroup owns TX consecutive
-// cells of one grid row (y, z), stages the particle records of the 3x3
-// neighbouring rows (cells x0-1 .. x0+TX) in LDS once, and then every
-// particle sorted into the tile walks its 27-cell stencil out of LDS.  A
-// particle whose current cell has left the tile's row (the grid is stale
-// during the solver iterations) walks the same stencil out of global
-// memory instead, so both paths visit the same candidates in the same
-// order.
+// Neighbour kernels: the direct walk (k_direct) gives each target particle
+// one thread, in cell-sorted order, and walks its 27-cell stencil row by
+// row straight out of global memory; consecutive lanes share stencils, so
+// the gathers are served from L1/L2.  The LDS row-tile form (k_tile: a
+// workgroup stages the records of the 3x3 neighbouring rows of TX cells
+// once) is kept as an autotuner candidate and runs the anisotropy pass.
+// Both visit the same candidates in the same order.
 //
 // Arithmetic: with EXACT the float operations are those of the reference
 // kernels, one IEEE-rounded operation each (the file is compiled with
@@ -495,7 +500,14 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // each XCD walks a contiguous range of the sorted particles.  Wall samples
 // exist only in cells with some index 0 or gridDim-1, so a particle whose
 // stencil cannot reach such a cell skips the wall loops.
-template <class Op, int NT, int CR>
+// PH = 1 (fast ops): two phases per chunk of up to 32 consecutive
+// candidates of a row.  Phase 1 only forms r^2 (the same operations as the
+// pair term) and sets one bit per candidate inside h; phase 2 runs the pair
+// term for the set bits, in the same order.  About 15% of the candidates
+// lie inside h, and the full term costs ~25 issue slots against ~10 for
+// the test, so the walk issues far fewer instructions; the terms dropped
+// are the ones the branchless pair term reduces to zero.
+template <class Op, int NT, int CR, int PH = 0>
 __device__ __forceinline__ void
 direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restrict__ cellStart,
             const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
@@ -518,7 +530,60 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       if (op.skip_row(U, t, z, y)) continue;
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
-      if constexpr (Op::FAST) {
+      if constexpr (Op::FAST && PH >= 1) {
+        for (uint32_t c0 = b; c0 < e; c0 += 32u) {
+          const uint32_t len = min(e - c0, 32u);
+          uint32_t m = 0u, k = 0u;
+          for (; k + 3u < len; k += 4u) {
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) q[u] = op.pred[c0 + k + u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const float r2 = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z);
+              m |= (r2 < U.h2 ? 1u : 0u) << (k + (uint32_t)u);
+            }
+          }
+          for (; k < len; k++) {
+            const float4 q = op.pred[c0 + k];
+            const float r2 = r2_of(t.px - q.x, t.py - q.y, t.pz - q.z);
+            m |= (r2 < U.h2 ? 1u : 0u) << k;
+          }
+          if constexpr (PH == 1) {
+            while (m) {
+              const uint32_t bit = (uint32_t)__builtin_ctz(m);
+              m &= m - 1u;
+              float4 v[Op::NREC];
+              op.load(U, c0 + bit, v);
+              op.fpair(U, t, v);
+            }
+          } else {
+            // four hits per trip with their loads in flight together; a
+            // missing hit is the far sentinel, which every fast pair term
+            // reduces to zero
+            while (m) {
+              uint32_t jj[4];
+              bool hit[4];
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                hit[u] = m != 0u;
+                jj[u] = c0 + (hit[u] ? (uint32_t)__builtin_ctz(m) : 0u);
+                m &= m - 1u;
+              }
+              float4 v[4][Op::NREC];
+#pragma unroll
+              for (int u = 0; u < 4; u++) op.load(U, jj[u], v[u]);
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+#pragma unroll
+                for (int r = 0; r < Op::NREC; r++)
+                  if (!hit[u]) v[u][r] = Op::sentinel(r);
+                op.fpair(U, t, v[u]);
+              }
+            }
+          }
+        }
+      } else if constexpr (Op::FAST) {
         uint32_t j = b;
         for (; j + 3 < e; j += 4) {
           float4 v[4][Op::NREC];
@@ -548,20 +613,20 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
   op.finish(U, i, t);
 }
 
-template <class Op, int NT, int CR>
+template <class Op, int NT, int CR, int PH = 0>
 __global__ void __launch_bounds__(NT)
 k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= U.n) return;
-  direct_walk<Op, NT, CR>(U, op, i, cellStart, bcellStart, bpos);
+  direct_walk<Op, NT, CR, PH>(U, op, i, cellStart, bcellStart, bpos);
 }
 
 // Batched scenes: each workgroup lies in one scene's 256-aligned slice of
 // the launch index space; the scene's parameter block is read from the
 // table, and its particle and cell slices are offsets into the
 // concatenated arrays (cellStart holds global particle indices).
-template <class Op, int NT>
+template <class Op, int NT, int PH = 0>
 __global__ void __launch_bounds__(NT)
 k_direct_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
                const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
@@ -570,7 +635,7 @@ k_direct_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
   const Uniform& U = B.U[B.chunkScene[v0 >> 8]];
   const uint32_t local = v0 - U.vBase + threadIdx.x;
   if (local >= U.n) return;
-  direct_walk<Op, NT, 1>(U, op, U.pBase + local, cellStart, bcellStart, bpos);
+  direct_walk<Op, NT, 1, PH>(U, op, U.pBase + local, cellStart, bcellStart, bpos);
 }
 
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
@@ -1923,18 +1988,18 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + WG - 1) / WG); }
 
-template <class Op, int NT, int CR = 1>
+template <class Op, int NT, int CR = 1, int PH = 0>
 hipError_t run_direct(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_direct<Op, NT, CR>), dim3((U.n + NT - 1) / NT), dim3(NT), 0, s, U, op,
+  hipLaunchKernelGGL((k_direct<Op, NT, CR, PH>), dim3((U.n + NT - 1) / NT), dim3(NT), 0, s, U, op,
                      A.cellStart, A.bcellStart, A.bpos);
   return hipGetLastError();
 }
 
-template <class Op, int NT>
+template <class Op, int NT, int PH = 0>
 hipError_t run_direct_batch(const BatchTab& B, const Arrays& A, const Op& op, hipStream_t s) {
   if (B.chunks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_direct_batch<Op, NT>), dim3(B.chunks * (256u / NT)), dim3(NT), 0, s, B,
+  hipLaunchKernelGGL((k_direct_batch<Op, NT, PH>), dim3(B.chunks * (256u / NT)), dim3(NT), 0, s, B,
                      op, A.cellStart, A.bcellStart, A.bpos);
   return hipGetLastError();
 }
@@ -1957,28 +2022,44 @@ hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t
 template <class Op>
 hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s, const BatchTab* B = nullptr) {
-  if (B) return t.variant == 0 ? run_direct_batch<Op, 256>(*B, A, op, s)
+  if (B) return t.variant == 0   ? run_direct_batch<Op, 256>(*B, A, op, s)
                 : t.variant == 1 ? run_direct_batch<Op, 128>(*B, A, op, s)
+                : t.variant == 5 ? run_direct_batch<Op, 64, 1>(*B, A, op, s)
+                : t.variant == 6 ? run_direct_batch<Op, 128, 1>(*B, A, op, s)
+                : t.variant == 7 ? run_direct_batch<Op, 64, 2>(*B, A, op, s)
+                : t.variant == 8 ? run_direct_batch<Op, 128, 2>(*B, A, op, s)
                                  : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
     case 2: return run_direct<Op, 64>(U, A, op, s);
     case 3: return run_tile<Op, 32, 256, 3072>(U, A, op, s);
     case 4: return run_tile<Op, 16, 128, 2048>(U, A, op, s);
+    case 5: return run_direct<Op, 64, 1, 1>(U, A, op, s);
+    case 6: return run_direct<Op, 128, 1, 1>(U, A, op, s);
+    case 7: return run_direct<Op, 64, 1, 2>(U, A, op, s);
+    case 8: return run_direct<Op, 128, 1, 2>(U, A, op, s);
     default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
 template <class Op>
 hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
                      hipStream_t s, const BatchTab* B = nullptr) {
-  if (B) return t.variant == 0 ? run_direct_batch<Op, 256>(*B, A, op, s)
+  if (B) return t.variant == 0   ? run_direct_batch<Op, 256>(*B, A, op, s)
                 : t.variant == 1 ? run_direct_batch<Op, 128>(*B, A, op, s)
+                : t.variant == 5 ? run_direct_batch<Op, 64, 1>(*B, A, op, s)
+                : t.variant == 6 ? run_direct_batch<Op, 128, 1>(*B, A, op, s)
+                : t.variant == 7 ? run_direct_batch<Op, 64, 2>(*B, A, op, s)
+                : t.variant == 8 ? run_direct_batch<Op, 128, 2>(*B, A, op, s)
                                  : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
     case 2: return run_direct<Op, 64>(U, A, op, s);
     case 3: return run_tile<Op, 16, 128, 1536>(U, A, op, s);
     case 4: return run_tile<Op, 16, 256, 1536>(U, A, op, s);
+    case 5: return run_direct<Op, 64, 1, 1>(U, A, op, s);
+    case 6: return run_direct<Op, 128, 1, 1>(U, A, op, s);
+    case 7: return run_direct<Op, 64, 1, 2>(U, A, op, s);
+    case 8: return run_direct<Op, 128, 1, 2>(U, A, op, s);
     default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
@@ -2159,14 +2240,16 @@ k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
 
 int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
 
-int tile_variants(int group) { return group == 1 ? 5 : 5; }
+int tile_variants(int) { return 9; }
 
 const char* tile_variant_name(int group, int v) {
   static const char* g1[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx32/t256/lds3072",
-                             "tile/tx16/t128/lds2048"};
+                             "tile/tx16/t128/lds2048", "direct2p/t64", "direct2p/t128",
+                             "direct2p4/t64", "direct2p4/t128"};
   static const char* g2[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx16/t128/lds1536",
-                             "tile/tx16/t256/lds1536"};
-  if (v < 0 || v >= 5) return "?";
+                             "tile/tx16/t256/lds1536", "direct2p/t64", "direct2p/t128",
+                             "direct2p4/t64", "direct2p4/t128"};
+  if (v < 0 || v >= 9) return "?";
   return group == 1 ? g1[v] : g2[v];
 }
 

@@ -310,7 +310,7 @@ def test_aniso_against_oracle(math):
     sim.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_every_tile_variant_bit_exact(variant, monkeypatch):
     """Each traversal shape the autotuner may pick (direct per-particle
     gathers or LDS row tiles) gives the oracle's bits in exact math."""
@@ -337,6 +337,44 @@ def test_every_tile_variant_bit_exact(variant, monkeypatch):
     assert np.array_equal(rd(sim, "VEL"), vel)
     assert np.array_equal(rd(sim, "POS"), pos2)
     sim.close()
+
+
+def _fast_run(variant, monkeypatch):
+    monkeypatch.setenv("P4A_TILE1", str(variant))
+    monkeypatch.setenv("P4A_TILE2", str(variant))
+    p, sc, _ = workloads.block_1m(scale=0.125, math=0)
+    sim = Sim(0)
+    sim.reset(p, sc)
+    dt = 1 / 120
+    sim.substeps(dt, 2)
+    out = {}
+    sim.run_stage("PREDICT", dt)
+    sim.run_stage("GRID", dt)
+    sim.run_stage("LAMBDA", dt)
+    out["lambda"] = rd(sim, "LAMBDA")
+    out["density"] = rd(sim, "DENSITY")
+    sim.run_stage("DELTA", dt)
+    out["pred"] = rd(sim, "PRED")
+    sim.run_stage("FINISH", dt)
+    out["vel"] = rd(sim, "VEL")
+    out["normal"] = rd(sim, "NORMAL")
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("variant", [5, 6, 7, 8])
+def test_two_phase_walk_matches_single_phase_in_fast_math(variant, monkeypatch):
+    """The two-phase walk (test r^2 < h^2, then the pair term for the hits)
+    sums the same non-zero terms in the same order as the single-phase
+    branchless walk; the terms it skips are the branchless form's zeros and
+    its ~1e-13 gradient residue at r just above h (hardware rsq)."""
+    a = _fast_run(2, monkeypatch)
+    b = _fast_run(variant, monkeypatch)
+    for k in a:
+        scale = max(np.abs(a[k]).max(), 1e-30)
+        err = np.abs(a[k] - b[k]).max() / scale
+        print(f"{k}: max |diff| / max|value| = {err:.2e}")
+        assert err <= 1e-6, k
 
 
 def test_drift_slots_hold_the_largest_displacement():

@@ -2172,53 +2172,126 @@ k_field_aniso_prep(FieldArgs F, uint32_t n, const float4* __restrict__ smoothPos
   recB[3 * j + 2] = make_float4(ex, ey, ez, 0.0f);
 }
 
-// One thread per particle (threads of a wave take particles n/64 apart, so
-// their atomics rarely meet): every voxel vertex in the bounding box with
-// s^2 = r^T M r < 1 receives rint(w (1 - s^2)^3 * 2^40) by a 64-bit
-// integer atomic add.
-__global__ void __launch_bounds__(WG)
-k_field_splat(FieldArgs F, uint32_t n, const float4* __restrict__ recA,
-              const float4* __restrict__ recB, unsigned long long* __restrict__ acc) {
-  const uint32_t t = blockIdx.x * WG + threadIdx.x;
-  const uint32_t nth = gridDim.x * WG;
-  if (t >= nth) return;
-  const uint32_t stride = (n + 63u) / 64u;
-  const uint32_t j = (t % 64u) * stride + t / 64u;
-  if (j >= n || t / 64u >= stride) return;
-  const float4 A = recA[j];
-  if (!(A.w > 0.0f)) return;
-  const float4 B0 = recB[3 * j], B1 = recB[3 * j + 1], E = recB[3 * j + 2];
-  // a non-finite centre or extent would make the box span the whole field
-  if (!isfinite(A.x) || !isfinite(A.y) || !isfinite(A.z) || !isfinite(E.x) ||
-      !isfinite(E.y) || !isfinite(E.z))
-    return;
-  const float inv = 1.0f / F.voxel;
+// Vertex index box of particle j's ellipsoid (empty: lo > hi), and whether
+// the particle contributes at all.  A non-finite centre or extent would make
+// the box span the whole field, so such a particle contributes nothing.
+struct SplatBox {
   int lo[3], hi[3];
+};
+__device__ __forceinline__ bool splat_box(const FieldArgs& F, const float4& A, const float4& E,
+                                          SplatBox& bx) {
+  if (!(A.w > 0.0f) || !isfinite(A.x) || !isfinite(A.y) || !isfinite(A.z) || !isfinite(E.x) ||
+      !isfinite(E.y) || !isfinite(E.z))
+    return false;
+  const float inv = 1.0f / F.voxel;
   const float c[3] = {A.x, A.y, A.z}, ext[3] = {E.x, E.y, E.z};
   for (int k = 0; k < 3; k++) {
     const float a = (c[k] - ext[k] - F.origin[k]) * inv, b = (c[k] + ext[k] - F.origin[k]) * inv;
-    lo[k] = max((int)ceilf(fmaxf(a, -1.0f)), 0);
-    hi[k] = min((int)floorf(fminf(b, (float)F.vertDim[k])), F.vertDim[k] - 1);
+    bx.lo[k] = max((int)ceilf(fmaxf(a, -1.0f)), 0);
+    bx.hi[k] = min((int)floorf(fminf(b, (float)F.vertDim[k])), F.vertDim[k] - 1);
   }
-  for (int z = lo[2]; z <= hi[2]; z++) {
-    const float rz = (F.origin[2] + F.voxel * (float)z) - A.z;
-    for (int y = lo[1]; y <= hi[1]; y++) {
-      const float ry = (F.origin[1] + F.voxel * (float)y) - A.y;
-      const uint32_t rowBase = ((uint32_t)z * (uint32_t)F.vertDim[1] + (uint32_t)y) *
-                               (uint32_t)F.vertDim[0];
-      for (int x = lo[0]; x <= hi[0]; x++) {
-        const float rx = (F.origin[0] + F.voxel * (float)x) - A.x;
-        // s^2 = M00 x^2 + M11 y^2 + M22 z^2 + 2 (M01 xy + M02 xz + M12 yz)
-        const float s2 = B0.x * (rx * rx) + B0.y * (ry * ry) + B0.z * (rz * rz) +
-                         2.0f * (B0.w * (rx * ry) + B1.x * (rx * rz) + B1.y * (ry * rz));
-        if (s2 < 1.0f) {
-          const float u = 1.0f - s2;
-          const float term = ((A.w * u) * u) * u;
-          const long long q = __double2ll_rn((double)term * FIELD_FIXED_SCALE);
-          atomicAdd(&acc[rowBase + (uint32_t)x], (unsigned long long)q);
+  return bx.lo[0] <= bx.hi[0] && bx.lo[1] <= bx.hi[1] && bx.lo[2] <= bx.hi[2];
+}
+
+// The term of vertex (x, y, z) for particle record (A, B0, B1), as a
+// multiple of 2^-40, or -1 outside the ellipsoid.
+__device__ __forceinline__ long long splat_term(const FieldArgs& F, const float4& A,
+                                                const float4& B0, const float4& B1, int x,
+                                                float ry, float rz) {
+  const float rx = (F.origin[0] + F.voxel * (float)x) - A.x;
+  // s^2 = M00 x^2 + M11 y^2 + M22 z^2 + 2 (M01 xy + M02 xz + M12 yz)
+  const float s2 = B0.x * (rx * rx) + B0.y * (ry * ry) + B0.z * (rz * rz) +
+                   2.0f * (B0.w * (rx * ry) + B1.x * (rx * rz) + B1.y * (ry * rz));
+  if (!(s2 < 1.0f)) return -1;
+  const float u = 1.0f - s2;
+  const float term = ((A.w * u) * u) * u;
+  return __double2ll_rn((double)term * FIELD_FIXED_SCALE);
+}
+
+// One workgroup per NT consecutive particles of the cell-sorted order (a
+// spatially compact chunk: 64 particles are about 8 cells of one row).  The terms are first summed in an LDS tile
+// covering the union of the chunk's vertex boxes, and each non-zero tile
+// entry is then added to the field with one 64-bit integer atomic: the
+// global atomics drop by the overlap of the chunk's ellipsoids.  A chunk
+// whose union does not fit the tile adds its terms to the field directly.
+// Integer sums, so the result does not depend on the order of either kind
+// of atomic.
+template <int NT, int CAP>
+__global__ void __launch_bounds__(NT)
+k_field_splat(FieldArgs F, uint32_t n, const float4* __restrict__ recA,
+              const float4* __restrict__ recB, unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long tile[CAP];
+  __shared__ int red[NT / 64][6];
+  __shared__ int box[6];
+  const uint32_t j = blockIdx.x * NT + threadIdx.x;
+  float4 A = make_float4(0.0f, 0.0f, 0.0f, 0.0f), B0 = A, B1 = A, E = A;
+  SplatBox bx;
+  bool act = false;
+  if (j < n) {
+    A = recA[j];
+    B0 = recB[3 * j];
+    B1 = recB[3 * j + 1];
+    E = recB[3 * j + 2];
+    act = splat_box(F, A, E, bx);
+  }
+  // union of the boxes (empty boxes contribute nothing)
+  int v[6];
+  for (int k = 0; k < 3; k++) {
+    v[k] = act ? bx.lo[k] : INT_MAX;
+    v[3 + k] = act ? -bx.hi[k] : INT_MAX;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int m = __ockl_wfred_min_i32(v[k]);
+    if (lane == 0) red[wave][k] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int m = red[0][threadIdx.x];
+    for (int w = 1; w < NT / 64; w++) m = min(m, red[w][threadIdx.x]);
+    box[threadIdx.x] = m;
+  }
+  __syncthreads();
+  const int ox = box[0], oy = box[1], oz = box[2];
+  const long long sx = (long long)-box[3] - ox + 1, sy = (long long)-box[4] - oy + 1,
+                  sz = (long long)-box[5] - oz + 1;
+  const bool any = box[0] != INT_MAX;
+  const bool useTile = any && sx * sy * sz <= (long long)CAP;
+  const int tx = (int)sx, ty = (int)sy;
+  if (useTile) {
+    const int cnt = (int)(sx * sy * sz);
+    for (int k = threadIdx.x; k < cnt; k += NT) tile[k] = 0ull;
+  }
+  __syncthreads();
+  if (act) {
+    for (int z = bx.lo[2]; z <= bx.hi[2]; z++) {
+      const float rz = (F.origin[2] + F.voxel * (float)z) - A.z;
+      for (int y = bx.lo[1]; y <= bx.hi[1]; y++) {
+        const float ry = (F.origin[1] + F.voxel * (float)y) - A.y;
+        const uint32_t rowBase = ((uint32_t)z * (uint32_t)F.vertDim[1] + (uint32_t)y) *
+                                 (uint32_t)F.vertDim[0];
+        const int tileRow = ((z - oz) * ty + (y - oy)) * tx - ox;
+        for (int x = bx.lo[0]; x <= bx.hi[0]; x++) {
+          const long long q = splat_term(F, A, B0, B1, x, ry, rz);
+          if (q < 0) continue;
+          if (useTile)
+            atomicAdd(&tile[tileRow + x], (unsigned long long)q);
+          else
+            atomicAdd(&acc[rowBase + (uint32_t)x], (unsigned long long)q);
         }
       }
     }
+  }
+  if (!useTile) return;
+  __syncthreads();
+  const int cnt = (int)(sx * sy * sz);
+  for (int k = threadIdx.x; k < cnt; k += NT) {
+    const unsigned long long t = tile[k];
+    if (t == 0ull) continue;
+    const int x = ox + k % tx, y = oy + (k / tx) % ty, z = oz + k / (tx * ty);
+    atomicAdd(&acc[((uint32_t)z * (uint32_t)F.vertDim[1] + (uint32_t)y) *
+                       (uint32_t)F.vertDim[0] + (uint32_t)x], t);
   }
 }
 
@@ -2496,8 +2569,8 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
   if (U.n > 0) {
     hipLaunchKernelGGL(k_field_aniso_prep, dim3(blocks(U.n)), dim3(WG), 0, s, F, U.n, smoothPos,
                        g, A.density, pid, A.bodyOfPid, recA, recB);
-    const uint32_t stride = (U.n + 63u) / 64u;
-    hipLaunchKernelGGL(k_field_splat, dim3(blocks((size_t)stride * 64u)), dim3(WG), 0, s, F,
+    constexpr int NT = 64, CAP = 4096;  // 8 cells of particles, 32 KB of LDS
+    hipLaunchKernelGGL((k_field_splat<NT, CAP>), dim3((U.n + NT - 1) / NT), dim3(NT), 0, s, F,
                        U.n, recA, recB, acc);
   }
   hipLaunchKernelGGL(k_field_fixed_to_float, dim3(blocks(F.nVerts)), dim3(WG), 0, s, F.nVerts,

@@ -244,6 +244,19 @@ def field_bench(args):
     counts = g[:, 1, 3]
     sim.field_generate(anisotropic=True)
     field = sim.read("FIELD").astype(np.float64)
+    # the surface of generate: normal field (2 passes, the default of the
+    # reference's UI) and the marching tetrahedra at meshIso 0.4 (one untimed
+    # round first)
+    sim.mesh_normal_field(2)
+    sim.mesh_march(0.4)
+    sim.sync()
+    sim.set_timing(True)
+    for _ in range(reps):
+        sim.mesh_normal_field(2)
+        sim.mesh_march(0.4)
+    ktm = sim.timing()
+    sim.set_timing(False)
+    tri_total, tri_written = sim.mesh_triangles()
     rho = sim.read("DENSITY").astype(np.float64)
     vol = float(sc.mass) / np.maximum(rho, 0.5 * p.rest_density)
     integral = field.sum() * fi.voxel ** 3
@@ -262,7 +275,9 @@ def field_bench(args):
                    "particles": n, "field_vert_dim": list(fi.vert_dim), "voxel": fi.voxel,
                    "aniso": "R = 2h, lambda 0.9, kr 4, ks 1, stretch 2, kn 1, neps 25, "
                             "min extent 1.25 voxel/h, FP64 re-solve below a 1e-4 eigen-gap"},
-        "ms": {"aniso": a_ms, "field_anisotropic": f_ms, "field_isotropic": i_ms},
+        "ms": {"aniso": a_ms, "field_anisotropic": f_ms, "field_isotropic": i_ms,
+               "normal_field_2_passes": ktm["normal field"][0], "march": ktm["march"][0]},
+        "surface": {"iso": 0.4, "triangles": tri_total, "written": tri_written},
         "fp64_solves": solves,
         "checks": {"det_G_positive": det_ok, "min_count": float(counts.min()),
                    "mean_count": float(counts.mean()),

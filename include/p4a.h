@@ -256,7 +256,13 @@ enum p4a_buf {
   P4A_BUF_DRIFT = 23,      /* float 256: per position update since the last grid
                               build (delta, shape matching, in launch order),
                               the largest per-axis displacement of a particle */
-  P4A_BUF_FIELD = 24,      /* float n_verts: the last p4a_field_generate */
+  P4A_BUF_FIELD = 24,      /* float n_verts: the last p4a_field_generate,
+                              after any p4a_field_smooth */
+  P4A_BUF_FIELD_NORMAL = 25, /* float n_verts: the last p4a_mesh_normal_field */
+  P4A_BUF_MESH_VERTS = 26, /* uint2 3*max_triangles: packed triangle vertices
+                              of the last p4a_mesh_march */
+  P4A_BUF_MESH_INDIRECT = 27, /* uint 4: draw-indirect arguments of the last
+                              march (vertex count, 1, 0, 0) */
   P4A_BUF_COUNT_
 };
 
@@ -322,6 +328,8 @@ P4A_API int p4a_aniso(p4a_sim* h, const p4a_aniso_params* a, int32_t* fp64_solve
  * voxel = longest(box_max - box_min) / resolution, vert_dim = ceil(size /
  * voxel) + 3 per axis, origin = box_min - voxel; vertex v sits at origin +
  * voxel * v; flat index (v.z * vert_dim.y + v.y) * vert_dim.x + v.x.
+ * max_triangles is the march's triangle budget (<= 0: the reference's
+ * default 3,000,000; at most 2^30).
  *
  * p4a_field_generate = SurfaceMesh.generateField (src/mesh.js:263-271):
  *  P4A_FIELD_ISOTROPIC: meshDensityWGSL (src/mesh_wgsl.js:23-69), per
@@ -347,11 +355,45 @@ typedef struct p4a_field_info {
   double voxel;
   double origin[3];
   int64_t n_verts;
+  int64_t max_triangles;
 } p4a_field_info;
 P4A_API int p4a_field_configure(p4a_sim* h, const double box_min[3], const double box_max[3],
-                                int32_t resolution);
+                                int32_t resolution, int64_t max_triangles);
 P4A_API int p4a_field_generate(p4a_sim* h, int32_t kind, int32_t skip_bodies);
 P4A_API int p4a_field_get_info(p4a_sim* h, p4a_field_info* out);
+
+/* ---------------------------------------------------------------------
+ * Surface extraction over the field lattice.
+ *
+ * p4a_field_smooth = SurfaceMesh.smoothField (src/mesh.js:273-291): `passes`
+ * rounds of the separable blur of meshSmoothWGSL (mesh_wgsl.js:70-91),
+ * dst(v) = 0.25 S(v - a) + 0.5 S(v) + 0.25 S(v + a) along a = x, y, z, S
+ * clamping v into the lattice; in place on P4A_BUF_FIELD.
+ *
+ * p4a_mesh_normal_field = SurfaceMesh.buildNormalField (mesh.js:293-313):
+ * the same blur of the field into P4A_BUF_FIELD_NORMAL, which the march
+ * then differentiates for normals; passes <= 0 makes the march use the
+ * field itself (normalFieldReady = false).
+ *
+ * p4a_mesh_march = the march of SurfaceMesh.generate (mesh.js:320-331,
+ * meshMarchWGSL mesh_wgsl.js:93-233, meshIndirectWGSL 235-244): marching
+ * tetrahedra (six per voxel around its 0-7 diagonal), corners inside when
+ * field > iso, vertices on the crossed edges by linear interpolation,
+ * normals -grad of the normal field by central differences.  Each vertex
+ * is packed into two words as packVertex does: x | y << 16, z | (octahedral
+ * normal, 8 + 8 bits) << 16, positions in lattice units / (vert_dim - 1)
+ * quantised to 16 bits.  Triangles are numbered in voxel order (flat voxel
+ * index, then tetrahedron, then triangle), so the vertex buffer is the
+ * same on every run (the reference appends with an atomic counter, in no
+ * fixed order); triangles numbered >= max_triangles are dropped.
+ * Asynchronous; p4a_mesh_triangles waits and returns the total count
+ * (dropped triangles included, as the reference's counter) and the number
+ * written.
+ * ------------------------------------------------------------------- */
+P4A_API int p4a_field_smooth(p4a_sim* h, int32_t passes);
+P4A_API int p4a_mesh_normal_field(p4a_sim* h, int32_t passes);
+P4A_API int p4a_mesh_march(p4a_sim* h, float iso);
+P4A_API int p4a_mesh_triangles(p4a_sim* h, int64_t* total, int64_t* emitted);
 
 /* Wait for all queued GPU work of the handle. */
 P4A_API int p4a_sync(p4a_sim* h);

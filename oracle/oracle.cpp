@@ -853,4 +853,175 @@ void orc_field_aniso(const int32_t* vertDim, double voxel, const double* origin,
   for (long i = 0; i < nv; i++) field[i] = (float)((double)acc[i] * (1.0 / scale));
 }
 
+// One axis step of meshSmoothWGSL (mesh_wgsl.js:70-91): dst(v) = 0.25 S(v -
+// axis) + 0.5 S(v) + 0.25 S(v + axis), S clamping the vertex into the
+// lattice, evaluated left to right in float.
+static void mesh_blur_axis(const int32_t* vd, int axis, const float* src, float* dst) {
+  const long sx = 1, sy = vd[0], sz = (long)vd[0] * vd[1];
+  const long stride = axis == 0 ? sx : axis == 1 ? sy : sz;
+  for (int z = 0; z < vd[2]; z++)
+    for (int y = 0; y < vd[1]; y++)
+      for (int x = 0; x < vd[0]; x++) {
+        const int c = axis == 0 ? x : axis == 1 ? y : z;
+        const long i = x * sx + y * sy + z * sz;
+        const long lo = c > 0 ? i - stride : i;
+        const long hi = c < vd[axis] - 1 ? i + stride : i;
+        const float a = 0.25f * src[lo];
+        const float b = 0.5f * src[i];
+        const float t = a + b;
+        const float d = 0.25f * src[hi];
+        dst[i] = t + d;
+      }
+}
+
+// SurfaceMesh.smoothField / buildNormalField (mesh.js:273-313): `passes`
+// rounds of the x, y and z steps applied to src; the result in dst.
+void orc_mesh_smooth(const int32_t* vertDim, int passes, const float* src, float* dst) {
+  const long nv = (long)vertDim[0] * vertDim[1] * vertDim[2];
+  std::vector<float> a(src, src + nv), b(nv);
+  for (int p = 0; p < passes; p++)
+    for (int axis = 0; axis < 3; axis++) {
+      mesh_blur_axis(vertDim, axis, a.data(), b.data());
+      a.swap(b);
+    }
+  std::memcpy(dst, a.data(), nv * sizeof(float));
+}
+
+namespace {
+
+struct MCorner {
+  float d;
+  float p[3];
+  float g[3];
+};
+
+// packVertex (mesh_wgsl.js:112-124): grid position quantised to 16 bits per
+// axis over [0, vertDim-1]; normal octahedral-encoded to 8+8 bits.
+void mesh_pack(const int32_t* vd, const float* pg, const float* nIn, uint32_t* out) {
+  uint32_t q[3];
+  for (int k = 0; k < 3; k++) {
+    const float pn = std::fmin(std::fmax(pg[k] / (float)(vd[k] - 1), 0.0f), 1.0f);
+    q[k] = (uint32_t)(pn * 65535.0f + 0.5f);
+  }
+  const float l1 = std::fmax(std::fabs(nIn[0]) + std::fabs(nIn[1]) + std::fabs(nIn[2]), 1.0e-12f);
+  const float nx = nIn[0] / l1, ny = nIn[1] / l1, nz = nIn[2] / l1;
+  float e[2] = {nx, ny};
+  if (nz < 0.0f) {
+    e[0] = (1.0f - std::fabs(ny)) * (nx >= 0.0f ? 1.0f : -1.0f);
+    e[1] = (1.0f - std::fabs(nx)) * (ny >= 0.0f ? 1.0f : -1.0f);
+  }
+  uint32_t eo[2];
+  for (int k = 0; k < 2; k++)
+    eo[k] = (uint32_t)(std::fmin(std::fmax(e[k] * 0.5f + 0.5f, 0.0f), 1.0f) * 255.0f + 0.5f);
+  out[0] = q[0] | (q[1] << 16);
+  out[1] = q[2] | ((eo[0] | (eo[1] << 8)) << 16);
+}
+
+// edgeVertex (mesh_wgsl.js:126-132): t = (iso - da) / (db - da); position
+// and gradient by mix(a, b, t) = a * (1 - t) + b * t; the normal is -g.
+void mesh_edge(const int32_t* vd, float iso, const MCorner& a, const MCorner& b, uint32_t* out) {
+  const float t = (iso - a.d) / (b.d - a.d);
+  const float u = 1.0f - t;
+  float p[3], n[3];
+  for (int k = 0; k < 3; k++) {
+    p[k] = a.p[k] * u + b.p[k] * t;
+    n[k] = -(a.g[k] * u + b.g[k] * t);
+  }
+  mesh_pack(vd, p, n, out);
+}
+
+struct MeshOut {
+  const int32_t* vd;
+  float iso;
+  uint32_t maxTris;
+  uint32_t* verts;  // 6 words per triangle
+  uint32_t count;
+  void tri(const MCorner& a0, const MCorner& a1, const MCorner& b0, const MCorner& b1,
+           const MCorner& c0, const MCorner& c1) {
+    const uint32_t k = count++;
+    if (k >= maxTris) return;
+    mesh_edge(vd, iso, a0, a1, verts + 6 * (size_t)k);
+    mesh_edge(vd, iso, b0, b1, verts + 6 * (size_t)k + 2);
+    mesh_edge(vd, iso, c0, c1, verts + 6 * (size_t)k + 4);
+  }
+};
+
+// emitTet (mesh_wgsl.js:166-188) with emitLone / emitQuad (143-164).
+void mesh_tet(MeshOut& o, const MCorner& a, const MCorner& b, const MCorner& c,
+              const MCorner& d) {
+  const bool ia = a.d > o.iso, ib = b.d > o.iso, ic = c.d > o.iso, id = d.d > o.iso;
+  const int nin = (int)ia + (int)ib + (int)ic + (int)id;
+  if (nin == 0 || nin == 4) return;
+  if (nin == 1 || nin == 3) {
+    const bool look = nin == 1;
+    const MCorner *l, *p1, *p2, *p3;
+    if (ia == look) { l = &a; p1 = &b; p2 = &c; p3 = &d; }
+    else if (ib == look) { l = &b; p1 = &a; p2 = &c; p3 = &d; }
+    else if (ic == look) { l = &c; p1 = &a; p2 = &b; p3 = &d; }
+    else { l = &d; p1 = &a; p2 = &b; p3 = &c; }
+    o.tri(*l, *p1, *l, *p2, *l, *p3);
+    return;
+  }
+  const MCorner *qa, *qb, *qc, *qd;
+  if (ia && ib) { qa = &a; qb = &b; qc = &c; qd = &d; }
+  else if (ia && ic) { qa = &a; qb = &c; qc = &b; qd = &d; }
+  else if (ia && id) { qa = &a; qb = &d; qc = &b; qd = &c; }
+  else if (ib && ic) { qa = &b; qb = &c; qc = &a; qd = &d; }
+  else if (ib && id) { qa = &b; qb = &d; qc = &a; qd = &c; }
+  else { qa = &c; qb = &d; qc = &a; qd = &b; }
+  // q0 = (a,c), q1 = (a,d), q2 = (b,d), q3 = (b,c); triangles (q0,q1,q2), (q0,q2,q3)
+  o.tri(*qa, *qc, *qa, *qd, *qb, *qd);
+  o.tri(*qa, *qc, *qb, *qd, *qb, *qc);
+}
+
+}  // namespace
+
+// meshMarchWGSL (mesh_wgsl.js:190-233): marching tetrahedra over the voxels
+// of the lattice, six tetrahedra per voxel around the 0-7 diagonal, corners
+// inside when d > iso, gradients by central differences of nfield (clamped
+// at the lattice faces).  Triangles are numbered in voxel order (flat voxel
+// index, then tetrahedron, then triangle); those numbered >= maxTris are
+// dropped (emitTri, mesh_wgsl.js:134-141).  verts receives 6 words per kept
+// triangle; returns the total count including the dropped ones.
+uint32_t orc_mesh_march(const int32_t* vd, float iso, uint32_t maxTris, const float* field,
+                        const float* nfield, uint32_t* verts) {
+  const int nx = vd[0] - 1, ny = vd[1] - 1, nz = vd[2] - 1;
+  auto at = [&](const float* f, int x, int y, int z) {
+    x = std::min(std::max(x, 0), vd[0] - 1);
+    y = std::min(std::max(y, 0), vd[1] - 1);
+    z = std::min(std::max(z, 0), vd[2] - 1);
+    return f[((long)z * vd[1] + y) * vd[0] + x];
+  };
+  MeshOut o{vd, iso, maxTris, verts, 0u};
+  static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  for (int z = 0; z < nz; z++)
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) {
+        MCorner c[8];
+        int mask = 0;
+        for (int k = 0; k < 8; k++) {
+          c[k].d = at(field, x + off[k][0], y + off[k][1], z + off[k][2]);
+          if (c[k].d > iso) mask |= 1 << k;
+        }
+        if (mask == 0 || mask == 255) continue;
+        for (int k = 0; k < 8; k++) {
+          const int X = x + off[k][0], Y = y + off[k][1], Z = z + off[k][2];
+          c[k].p[0] = (float)X;
+          c[k].p[1] = (float)Y;
+          c[k].p[2] = (float)Z;
+          c[k].g[0] = at(nfield, X + 1, Y, Z) - at(nfield, X - 1, Y, Z);
+          c[k].g[1] = at(nfield, X, Y + 1, Z) - at(nfield, X, Y - 1, Z);
+          c[k].g[2] = at(nfield, X, Y, Z + 1) - at(nfield, X, Y, Z - 1);
+        }
+        mesh_tet(o, c[0], c[1], c[3], c[7]);
+        mesh_tet(o, c[0], c[1], c[5], c[7]);
+        mesh_tet(o, c[0], c[2], c[3], c[7]);
+        mesh_tet(o, c[0], c[2], c[6], c[7]);
+        mesh_tet(o, c[0], c[4], c[5], c[7]);
+        mesh_tet(o, c[0], c[4], c[6], c[7]);
+      }
+  return o.count;
+}
+
 }  // extern "C"

@@ -254,6 +254,49 @@ def field_aniso(lattice, mass, rho0, smooth_pos, g, density, body, skip_bodies):
     return out
 
 
+def mesh_smooth(vd, passes, field):
+    """`passes` rounds of meshSmoothWGSL's x, y, z steps (mesh_wgsl.js:70-91),
+    as SurfaceMesh.smoothField / buildNormalField (mesh.js:273-313)."""
+    src = np.ascontiguousarray(field, np.float32)
+    out = np.zeros_like(src)
+    lib().orc_mesh_smooth(_p(np.array(vd, np.int32)), int(passes), _p(src), _p(out))
+    return out
+
+
+def mesh_march(vd, iso, max_tris, field, nfield=None):
+    """meshMarchWGSL (mesh_wgsl.js:190-233) with triangles numbered in voxel
+    order: returns (kept triangles as uint32 (k, 6), total count)."""
+    vdc = np.array(vd, np.int32)
+    f = np.ascontiguousarray(field, np.float32)
+    nf = f if nfield is None else np.ascontiguousarray(nfield, np.float32)
+    total = lib().orc_mesh_march(_p(vdc), float(iso), 0, _p(f), _p(nf), None)
+    keep = min(int(total), int(max_tris))
+    verts = np.zeros((max(keep, 1), 6), np.uint32)
+    lib().orc_mesh_march(_p(vdc), float(iso), keep, _p(f), _p(nf), _p(verts))
+    return verts[:keep], int(total)
+
+
+def mesh_decode(verts, lattice):
+    """The mesh vertex shader's unpacking (mesh_wgsl.js:271-283), in FP64:
+    world positions (m, 3) and unit normals (m, 3) of packed vertices."""
+    vd, voxel, origin = lattice
+    v = np.asarray(verts, np.uint32).reshape(-1, 2).astype(np.int64)
+    q = np.stack([v[:, 0] & 0xFFFF, v[:, 0] >> 16, v[:, 1] & 0xFFFF], axis=1) / 65535.0
+    pos = np.array(origin) + q * (np.array(vd) - 1) * voxel
+    oct_ = v[:, 1] >> 16
+    e = np.stack([oct_ & 0xFF, oct_ >> 8], axis=1) / 255.0 * 2.0 - 1.0
+    n = np.stack([e[:, 0], e[:, 1], 1.0 - np.abs(e[:, 0]) - np.abs(e[:, 1])], axis=1)
+    neg = n[:, 2] < 0
+    sx = np.where(n[:, 0] >= 0, 1.0, -1.0)
+    sy = np.where(n[:, 1] >= 0, 1.0, -1.0)
+    tx = (1.0 - np.abs(n[:, 1])) * sx
+    ty = (1.0 - np.abs(n[:, 0])) * sy
+    n[neg, 0] = tx[neg]
+    n[neg, 1] = ty[neg]
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return pos, n
+
+
 def xsph(mode, P, pred, pos, density, cs, normals):
     pred = np.ascontiguousarray(pred, np.float64)
     pos = np.ascontiguousarray(pos, np.float64)
@@ -366,6 +409,9 @@ def _setup(L):
                                 vp, vp, C.c_int, vp, vp]
     L.orc_field_aniso.argtypes = [vp, C.c_double, vp, C.c_double, C.c_double, C.c_int, vp, vp,
                                   vp, vp, C.c_int, vp]
+    L.orc_mesh_smooth.argtypes = [vp, C.c_int, vp, vp]
+    L.orc_mesh_march.argtypes = [vp, C.c_float, C.c_uint32, vp, vp, vp]
+    L.orc_mesh_march.restype = C.c_uint32
     L.orc_xsph.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp]
     L.orc_tension.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
     L.orc_finalize.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]

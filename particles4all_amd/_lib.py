@@ -25,7 +25,8 @@ SCENE_PRESET, SCENE_LATTICE = 0, 1
 BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CORR=8,
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
            PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20,
-           OFS=21, CELL_OF_SLOT=22, DRIFT=23, FIELD=24)
+           OFS=21, CELL_OF_SLOT=22, DRIFT=23, FIELD=24, FIELD_NORMAL=25, MESH_VERTS=26,
+           MESH_INDIRECT=27)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -74,7 +75,8 @@ class AnisoParams(C.Structure):
 
 class FieldInfo(C.Structure):
     _fields_ = [("vert_dim", C.c_int32 * 3), ("pad_", C.c_int32), ("voxel", C.c_double),
-                ("origin", C.c_double * 3), ("n_verts", C.c_int64)]
+                ("origin", C.c_double * 3), ("n_verts", C.c_int64),
+                ("max_triangles", C.c_int64)]
 
 
 class Info(C.Structure):
@@ -98,7 +100,9 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_sync", "p4a_run_stage", "p4a_neighbour_census", "p4a_set_timing",
            "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso",
            "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name", "p4a_field_configure",
-           "p4a_field_generate", "p4a_field_get_info", "p4a_batch_create", "p4a_batch_destroy",
+           "p4a_field_generate", "p4a_field_get_info", "p4a_field_smooth",
+           "p4a_mesh_normal_field", "p4a_mesh_march", "p4a_mesh_triangles",
+           "p4a_batch_create", "p4a_batch_destroy",
            "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_sync",
            "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
 
@@ -136,7 +140,11 @@ def lib():
     L.p4a_release_body.argtypes = [vp]
     L.p4a_read.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
     L.p4a_field_configure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                      C.c_int32]
+                                      C.c_int32, C.c_int64]
+    L.p4a_field_smooth.argtypes = [vp, C.c_int32]
+    L.p4a_mesh_normal_field.argtypes = [vp, C.c_int32]
+    L.p4a_mesh_march.argtypes = [vp, C.c_float]
+    L.p4a_mesh_triangles.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.p4a_field_generate.argtypes = [vp, C.c_int32, C.c_int32]
     L.p4a_field_get_info.argtypes = [vp, C.POINTER(FieldInfo)]
     L.p4a_batch_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(Params),
@@ -344,7 +352,10 @@ class Sim:
                 "BPSI": (np.float32, i.n_boundary, 1), "SMOOTH_POS": (np.float32, n, 4),
                 "ANISO_G": (np.float32, 2 * n, 4), "OFS": (np.uint32, n, 2),
                 "CELL_OF_SLOT": (np.uint32, n, 1), "DRIFT": (np.float32, 256, 1),
-                "FIELD": (np.float32, self.field_info().n_verts, 1)}[name]
+                "FIELD": (np.float32, self.field_info().n_verts, 1),
+                "FIELD_NORMAL": (np.float32, self.field_info().n_verts, 1),
+                "MESH_VERTS": (np.uint32, 3 * self.field_info().max_triangles, 2),
+                "MESH_INDIRECT": (np.uint32, 4, 1)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")
@@ -419,11 +430,40 @@ class Sim:
         self._check(lib().p4a_aniso(self.h, C.byref(a), C.byref(cnt)), "aniso")
         return cnt.value
 
-    def field_configure(self, box_min, box_max, resolution):
-        """SurfaceMesh.configure (mesh.js:96-128)."""
+    def field_configure(self, box_min, box_max, resolution, max_triangles=0):
+        """SurfaceMesh.configure (mesh.js:96-128); max_triangles <= 0 is the
+        reference's 3,000,000."""
         self._check(lib().p4a_field_configure(self.h, (C.c_double * 3)(*box_min),
-                                              (C.c_double * 3)(*box_max), int(resolution)),
+                                              (C.c_double * 3)(*box_max), int(resolution),
+                                              int(max_triangles)),
                     "field_configure")
+
+    def field_smooth(self, passes):
+        """SurfaceMesh.smoothField (mesh.js:273-291), in place."""
+        self._check(lib().p4a_field_smooth(self.h, int(passes)), "field_smooth")
+
+    def mesh_normal_field(self, passes):
+        """SurfaceMesh.buildNormalField (mesh.js:293-313)."""
+        self._check(lib().p4a_mesh_normal_field(self.h, int(passes)), "mesh_normal_field")
+
+    def mesh_march(self, iso):
+        """The march of SurfaceMesh.generate (mesh.js:320-331); asynchronous."""
+        self._check(lib().p4a_mesh_march(self.h, float(iso)), "mesh_march")
+
+    def mesh_triangles(self):
+        """(total triangles including dropped ones, triangles written)."""
+        t, e = C.c_int64(), C.c_int64()
+        self._check(lib().p4a_mesh_triangles(self.h, C.byref(t), C.byref(e)), "mesh_triangles")
+        return t.value, e.value
+
+    def mesh(self):
+        """Written triangles of the last march as uint32 (k, 6) and the total."""
+        total, emitted = self.mesh_triangles()
+        out = np.empty((max(emitted, 0), 6), np.uint32)
+        if emitted:
+            self._check(lib().p4a_read(self.h, BUF["MESH_VERTS"], _ptr(out), 0, out.nbytes),
+                        "read MESH_VERTS")
+        return out, total
 
     def field_generate(self, anisotropic=False, skip_bodies=-1):
         """SurfaceMesh.generateField (mesh.js:263-271); anisotropic uses the

@@ -46,7 +46,7 @@ def build_lib(verbose=False):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     jobs = []
     objs = []
-    for src in ("kernels.hip", "sim.hip", "batch.hip"):
+    for src in ("kernels.hip", "sim.hip", "batch.hip", "mesh.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:

@@ -2350,6 +2350,16 @@ hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s) {
   return hipGetLastError();
 }
 
+int scan_blocks(uint32_t n) { return (int)((n + SCAN_CH) / SCAN_CH); }
+
+hipError_t launch_exclusive_scan(uint32_t* count, uint32_t n, uint32_t* blockSum, uint32_t* out,
+                                 hipStream_t s) {
+  const unsigned nb = (unsigned)scan_blocks(n);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, count, n, blockSum);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(WG), 0, s, count, n, blockSum, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
                           bool fp16, hipStream_t s) {
   if (U.n == 0) return hipSuccess;

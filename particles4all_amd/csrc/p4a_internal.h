@@ -223,6 +223,49 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
                               float4* recA, float4* recB, unsigned long long* acc,
                               float* field, hipStream_t s);
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s);
+// Exclusive scan of count[0..n) into out[0..n], out[n] = the total;
+// consumed counts are zeroed.  blockSum holds scan_blocks(n) entries.
+int scan_blocks(uint32_t n);
+hipError_t launch_exclusive_scan(uint32_t* count, uint32_t n, uint32_t* blockSum, uint32_t* out,
+                                 hipStream_t s);
+
+// Unsigned 32-bit division by an invariant divisor d >= 1 with one high
+// multiply (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, figure 4.1): q = (t + ((n - t) >> s1)) >> s2,
+// t = umulhi(m, n), exact for every 32-bit n.
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+  FastDiv() : d(1), m(1), s1(0), s2(0) {}
+  explicit FastDiv(uint32_t div) : d(div) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < div) l++;
+    m = (uint32_t)((((1ull << 32) * ((1ull << l) - div)) / div) + 1);
+    s1 = l < 1 ? l : 1;
+    s2 = l > 1 ? l - 1 : 0;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
+// Surface extraction on the field lattice (SurfaceMesh.smoothField,
+// buildNormalField, generate's march; mesh_wgsl.js:70-244).
+struct MeshArgs {
+  int vertDim[3];
+  float iso;
+  FastDiv divVx, divVy;    // vertices per row, rows per plane
+  FastDiv divNx, divNy;    // voxels per row, voxel rows per plane
+  uint32_t nVerts, nVox;
+  uint32_t maxTris;
+  uint32_t nBlocks;        // march workgroups (256 voxels each)
+};
+constexpr int MESH_WG = 256;
+hipError_t launch_mesh_blur(const MeshArgs& M, int passes, const float* src, float* tmp,
+                            float* dst, hipStream_t s);
+hipError_t launch_mesh_march(const MeshArgs& M, const float* field, const float* nfield,
+                             uint32_t* blockCount, uint32_t* blockSum, uint32_t* blockStart,
+                             uint2* verts, uint32_t* counter, uint32_t* indirect, hipStream_t s);
 
 int stats_blocks(uint32_t n);
 

@@ -187,6 +187,18 @@ struct p4a_sim {
   float4* fieldRecA = nullptr;
   float4* fieldRecB = nullptr;
   std::vector<void*> fieldRecAllocs;
+  // surface extraction (smoothField, buildNormalField, march)
+  float* field2 = nullptr;
+  float* nfield = nullptr;
+  uint32_t* meshBlockCount = nullptr;
+  uint32_t* meshBlockSum = nullptr;
+  uint32_t* meshBlockStart = nullptr;
+  uint32_t* meshCounter = nullptr;  // [0] total triangles, [4..7] draw-indirect arguments
+  bool normalReady = false;
+  uint32_t meshMaxTris = 0;
+  size_t meshVertCap = 0;  // triangles the vertex buffer holds
+  uint2* meshVerts = nullptr;
+  std::vector<void*> meshVertAllocs;
 };
 
 namespace {
@@ -559,6 +571,10 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_CELL_OF_SLOT: return {s->A.cellOfSlot, n * 4};
     case P4A_BUF_DRIFT: return {s->A.drift, (size_t)DRIFT_SLOTS * 4};
     case P4A_BUF_FIELD: return {s->field, s->field ? s->fieldVerts * 4 : 0};
+    case P4A_BUF_FIELD_NORMAL: return {s->nfield, s->nfield ? s->fieldVerts * 4 : 0};
+    case P4A_BUF_MESH_VERTS: return {s->meshVerts, s->meshVerts ? (size_t)s->meshMaxTris * 24 : 0};
+    case P4A_BUF_MESH_INDIRECT: return {s->meshCounter ? s->meshCounter + 4 : nullptr,
+                                        s->meshCounter ? (size_t)16 : (size_t)0};
     default: return {nullptr, 0};
   }
 }
@@ -648,6 +664,7 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->anisoAllocs);
   free_list(s->fieldAllocs);
   free_list(s->fieldRecAllocs);
+  free_list(s->meshVertAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
   hipStreamDestroy(s->stream);
@@ -1239,10 +1256,12 @@ P4A_API int p4a_aniso(p4a_sim* s, const p4a_aniso_params* a, int32_t* fp64_solve
 }
 
 P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const double box_max[3],
-                                int32_t resolution) {
+                                int32_t resolution, int64_t max_triangles) {
   int rc = check(s);
   if (rc) return rc;
   if (!box_min || !box_max || resolution < 1) return P4A_ERR_ARG;
+  if (max_triangles <= 0) max_triangles = 3000000;  // SurfaceMesh.maxTriangles (mesh.js:20)
+  if (max_triangles > (int64_t)1 << 30) return fail(s, P4A_ERR_RANGE, "max_triangles > 2^30");
   // SurfaceMesh.configure (mesh.js:96-128), JavaScript double arithmetic
   double size[3], longest = 0.0;
   for (int k = 0; k < 3; k++) {
@@ -1264,13 +1283,33 @@ P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const doubl
   if (nv > s->fieldVertCap) {
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     free_list(s->fieldAllocs);
-    s->field = nullptr;
+    s->field = s->field2 = s->nfield = nullptr;
     s->fieldAcc = nullptr;
+    s->meshBlockCount = s->meshBlockSum = s->meshBlockStart = s->meshCounter = nullptr;
+    s->normalReady = false;
     s->fieldVertCap = 0;
+    // voxels <= vertices, so nv / 256 + 1 march workgroups always suffice
+    const size_t nb = nv / MESH_WG + 1;
     if ((rc = dalloc(s, s->fieldAllocs, &s->field, nv))) return rc;
     if ((rc = dalloc(s, s->fieldAllocs, &s->fieldAcc, nv))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->field2, nv))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->nfield, nv))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->meshBlockCount, nb))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->meshBlockSum, (size_t)scan_blocks((uint32_t)nb))))
+      return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->meshBlockStart, nb + 1))) return rc;
+    if ((rc = dalloc(s, s->fieldAllocs, &s->meshCounter, 8))) return rc;
     s->fieldVertCap = nv;
   }
+  if ((size_t)max_triangles != s->meshVertCap) {
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    free_list(s->meshVertAllocs);
+    s->meshVerts = nullptr;
+    s->meshVertCap = 0;
+    if ((rc = dalloc(s, s->meshVertAllocs, &s->meshVerts, (size_t)max_triangles * 3))) return rc;
+    s->meshVertCap = (size_t)max_triangles;
+  }
+  s->meshMaxTris = (uint32_t)max_triangles;
   for (int k = 0; k < 3; k++) {
     s->fieldVertDim[k] = vd[k];
     s->fieldOrigin[k] = box_min[k] - voxel;
@@ -1334,6 +1373,76 @@ P4A_API int p4a_field_get_info(p4a_sim* s, p4a_field_info* out) {
   }
   out->voxel = s->fieldVoxel;
   out->n_verts = (int64_t)s->fieldVerts;
+  out->max_triangles = (int64_t)s->meshMaxTris;
+  return P4A_OK;
+}
+
+namespace {
+
+MeshArgs mesh_args(const p4a_sim* s, float iso) {
+  MeshArgs M;
+  for (int k = 0; k < 3; k++) M.vertDim[k] = s->fieldVertDim[k];
+  M.iso = iso;
+  M.divVx = FastDiv((uint32_t)M.vertDim[0]);
+  M.divVy = FastDiv((uint32_t)M.vertDim[1]);
+  M.divNx = FastDiv((uint32_t)(M.vertDim[0] - 1));
+  M.divNy = FastDiv((uint32_t)(M.vertDim[1] - 1));
+  M.nVerts = (uint32_t)s->fieldVerts;
+  M.nVox = (uint32_t)((size_t)(M.vertDim[0] - 1) * (M.vertDim[1] - 1) * (M.vertDim[2] - 1));
+  M.maxTris = s->meshMaxTris;
+  M.nBlocks = (M.nVox + MESH_WG - 1) / MESH_WG;
+  return M;
+}
+
+}  // namespace
+
+P4A_API int p4a_field_smooth(p4a_sim* s, int32_t passes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!s->fieldConfigured) return fail(s, P4A_ERR_ARG, "p4a_field_configure first");
+  if (passes <= 0) return P4A_OK;
+  HIP_TRY(s, hipSetDevice(s->device));
+  LAUNCH(s, "field smooth", launch_mesh_blur(mesh_args(s, 0.0f), passes, s->field, s->field2,
+                                             s->field, s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_mesh_normal_field(p4a_sim* s, int32_t passes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!s->fieldConfigured) return fail(s, P4A_ERR_ARG, "p4a_field_configure first");
+  s->normalReady = false;
+  if (passes <= 0) return P4A_OK;
+  HIP_TRY(s, hipSetDevice(s->device));
+  LAUNCH(s, "normal field", launch_mesh_blur(mesh_args(s, 0.0f), passes, s->field, s->field2,
+                                             s->nfield, s->stream));
+  s->normalReady = true;
+  return P4A_OK;
+}
+
+P4A_API int p4a_mesh_march(p4a_sim* s, float iso) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!s->fieldConfigured) return fail(s, P4A_ERR_ARG, "p4a_field_configure first");
+  HIP_TRY(s, hipSetDevice(s->device));
+  const MeshArgs M = mesh_args(s, iso);
+  LAUNCH(s, "march", launch_mesh_march(M, s->field, s->normalReady ? s->nfield : s->field,
+                                       s->meshBlockCount, s->meshBlockSum, s->meshBlockStart,
+                                       s->meshVerts, s->meshCounter, s->meshCounter + 4,
+                                       s->stream));
+  return P4A_OK;
+}
+
+P4A_API int p4a_mesh_triangles(p4a_sim* s, int64_t* total, int64_t* emitted) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!s->meshCounter) return fail(s, P4A_ERR_ARG, "p4a_field_configure first");
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  uint32_t c[8];
+  HIP_TRY(s, hipMemcpy(c, s->meshCounter, sizeof(c), hipMemcpyDeviceToHost));
+  if (total) *total = (int64_t)c[0];
+  if (emitted) *emitted = (int64_t)(c[4] / 3u);
   return P4A_OK;
 }
 

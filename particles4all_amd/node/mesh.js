@@ -1,13 +1,19 @@
 'use strict';
-// The simulation-side part of the reference's SurfaceMesh (src/mesh.js):
-// the anisotropy pass (buildAnisotropy, mesh.js:207-261) and the density
-// field (configure 96-128, generateField 263-271).  The marching-tetrahedra
-// mesher and every draw call stay with the renderer.
+// The compute part of the reference's SurfaceMesh (src/mesh.js): the
+// anisotropy pass (buildAnisotropy, mesh.js:207-261), the density field
+// (configure 96-128, generateField 263-271), its blur (smoothField 273-291,
+// buildNormalField 293-313) and the marching-tetrahedra surface (generate
+// 315-331, pollTriangles 333-345).  Draw calls stay with the renderer: the
+// packed vertices and the draw-indirect arguments are device buffers
+// (`verts`, `indirect`: {ptr, bytes} descriptors) it can read in place.
 const native = require('./native');
 const { BUF } = require('./sim');
 
 class SurfaceMesh {
   constructor() {
+    this.maxTriangles = 3000000;     // mesh.js:20-21
+    this.normalSmoothPasses = 0;
+    this.lastTriangles = 0;
     // mesh.js:81-90 defaults
     this.anisoRatio = 4.0;
     this.anisoKs = 1.0;
@@ -24,15 +30,19 @@ class SurfaceMesh {
     this.lastFp64Solves = 0;
   }
 
-  // configure(boxMin, boxMax, resolution): the voxel-vertex lattice
-  configure(sim, boxMin, boxMax, resolution) {
-    native.fieldConfigure(sim.handle, boxMin, boxMax, resolution);
+  // configure(boxMin, boxMax, resolution, maxTriangles): the voxel-vertex
+  // lattice and the triangle budget
+  configure(sim, boxMin, boxMax, resolution, maxTriangles = this.maxTriangles) {
+    native.fieldConfigure(sim.handle, boxMin, boxMax, resolution, maxTriangles);
     const f = native.fieldInfo(sim.handle);
     this.voxel = f.voxel;
     this.vertDim = f.vertDim;
     this.nVerts = f.nVerts;
     this.origin = f.origin;
     this.boxMin = boxMin;
+    this.vertBudget = f.maxTriangles;
+    this.verts = native.devicePtr(sim.handle, BUF.meshVerts);
+    this.indirect = native.devicePtr(sim.handle, BUF.meshIndirect);
   }
 
   // buildAnisotropy(enc, sim): smoothed centres and G; minExtent follows
@@ -55,6 +65,44 @@ class SurfaceMesh {
   generateField(sim, opts = {}) {
     native.fieldGenerate(sim.handle, opts.anisotropic ? 1 : 0,
                          opts.skipBodies === undefined ? -1 : (opts.skipBodies ? 1 : 0));
+  }
+
+  // smoothField(enc, sim, iso, passes): blur the field in place
+  smoothField(sim, iso, passes) {
+    if (passes > 0) native.fieldSmooth(sim.handle, passes);
+  }
+
+  // buildNormalField(enc, sim, iso, passes): the blurred copy the march
+  // differentiates for normals (none when passes <= 0)
+  buildNormalField(sim, iso, passes) {
+    this.normalFieldReady = passes > 0;
+    native.meshNormalField(sim.handle, passes);
+  }
+
+  // generate(enc, sim, iso, smoothPasses): field, blur, normal field and
+  // the marching-tetrahedra surface into `verts`; opts.anisotropic splats
+  // the last buildAnisotropy instead of the isotropic field
+  generate(sim, iso, smoothPasses = 0, opts = {}) {
+    this.generateField(sim, opts);
+    this.smoothField(sim, iso, smoothPasses);
+    this.buildNormalField(sim, iso, this.normalSmoothPasses);
+    native.meshMarch(sim.handle, iso);
+  }
+
+  // pollTriangles(): the triangle counter of the last generate (waits for
+  // it; the reference reads it back a few frames late)
+  pollTriangles(sim) {
+    const t = native.meshTriangles(sim.handle);
+    this.lastTriangles = t.total;
+    return t;
+  }
+
+  // packed vertices of the written triangles (2 words each, 3 per
+  // triangle) as a Uint32Array
+  vertices(sim) {
+    const t = native.meshTriangles(sim.handle);
+    if (t.emitted <= 0) return new Uint32Array(0);
+    return native.read(sim.handle, BUF.meshVerts, 'u32', t.emitted * 6);
   }
 
   // the field as a Float32Array, x fastest

@@ -533,15 +533,56 @@ napi_value Aniso(napi_env env, napi_callback_info info) {
   return v;
 }
 
-// fieldConfigure(h, boxMin[3], boxMax[3], resolution)
+// fieldConfigure(h, boxMin[3], boxMax[3], resolution, maxTriangles)
 napi_value FieldConfigure(napi_env env, napi_callback_info info) {
-  auto a = args(env, info, 4);
+  auto a = args(env, info, 5);
   p4a_sim* h = handle(env, a[0]);
   double lo[3], hi[3];
   vec3(env, a[1], lo);
   vec3(env, a[2], hi);
-  P4A_CALL(h, p4a_field_configure(h, lo, hi, (int32_t)num(env, a[3])), "p4a_field_configure");
+  P4A_CALL(h, p4a_field_configure(h, lo, hi, (int32_t)num(env, a[3]),
+                                  (int64_t)num(env, a[4], 0)),
+           "p4a_field_configure");
   return nullptr;
+}
+
+// fieldSmooth(h, passes)
+napi_value FieldSmooth(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 2);
+  p4a_sim* h = handle(env, a[0]);
+  P4A_CALL(h, p4a_field_smooth(h, (int32_t)num(env, a[1], 0)), "p4a_field_smooth");
+  return nullptr;
+}
+
+// meshNormalField(h, passes)
+napi_value MeshNormalField(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 2);
+  p4a_sim* h = handle(env, a[0]);
+  P4A_CALL(h, p4a_mesh_normal_field(h, (int32_t)num(env, a[1], 0)), "p4a_mesh_normal_field");
+  return nullptr;
+}
+
+// meshMarch(h, iso)
+napi_value MeshMarch(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 2);
+  p4a_sim* h = handle(env, a[0]);
+  P4A_CALL(h, p4a_mesh_march(h, (float)num(env, a[1])), "p4a_mesh_march");
+  return nullptr;
+}
+
+// meshTriangles(h) -> {total, emitted}
+napi_value MeshTriangles(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 1);
+  p4a_sim* h = handle(env, a[0]);
+  int64_t total = 0, emitted = 0;
+  P4A_CALL(h, p4a_mesh_triangles(h, &total, &emitted), "p4a_mesh_triangles");
+  napi_value o, v;
+  napi_create_object(env, &o);
+  napi_create_double(env, (double)total, &v);
+  napi_set_named_property(env, o, "total", v);
+  napi_create_double(env, (double)emitted, &v);
+  napi_set_named_property(env, o, "emitted", v);
+  return o;
 }
 
 // fieldGenerate(h, anisotropic, skipBodies)
@@ -555,7 +596,7 @@ napi_value FieldGenerate(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-// fieldInfo(h) -> {vertDim, voxel, origin, nVerts}
+// fieldInfo(h) -> {vertDim, voxel, origin, nVerts, maxTriangles}
 napi_value FieldInfo(napi_env env, napi_callback_info info) {
   auto a = args(env, info, 1);
   p4a_sim* h = handle(env, a[0]);
@@ -577,6 +618,8 @@ napi_value FieldInfo(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "voxel", v);
   napi_create_double(env, (double)f.n_verts, &v);
   napi_set_named_property(env, o, "nVerts", v);
+  napi_create_double(env, (double)f.max_triangles, &v);
+  napi_set_named_property(env, o, "maxTriangles", v);
   return o;
 }
 
@@ -602,7 +645,9 @@ napi_value Init(napi_env env, napi_value exports) {
              {"bodyPose", BodyPose},     {"runStage", RunStage},
              {"sync", Sync},             {"version", Version},
              {"aniso", Aniso},           {"fieldConfigure", FieldConfigure},
-             {"fieldGenerate", FieldGenerate}, {"fieldInfo", FieldInfo}};
+             {"fieldGenerate", FieldGenerate}, {"fieldInfo", FieldInfo},
+             {"fieldSmooth", FieldSmooth}, {"meshNormalField", MeshNormalField},
+             {"meshMarch", MeshMarch},   {"meshTriangles", MeshTriangles}};
   for (auto& f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

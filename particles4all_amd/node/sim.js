@@ -12,7 +12,7 @@ const BUF = {
   pos: 0, vel: 1, pred: 2, body: 3, rest: 4, lambda: 5, density: 6, slot: 7, corr: 8,
   normal: 9, cellStart: 10, bpos: 11, bcellStart: 12, bodyCentre: 13, bodyRot: 14,
   pid: 15, key: 16, order: 17, bpsi: 18, smoothPos: 19, anisoG: 20, ofs: 21, cellOfSlot: 22,
-  drift: 23, field: 24,
+  drift: 23, field: 24, fieldNormal: 25, meshVerts: 26, meshIndirect: 27,
 };
 const STAGE = { predict: 0, grid: 1, bodyPre: 2, lambda: 3, delta: 4, bodySolve: 5,
                 finish: 6, stats: 7 };

@@ -1,8 +1,9 @@
 'use strict';
 // GPU check of the JS SurfaceMesh surface (node/mesh.js): configure, the
 // anisotropy pass, the isotropic field against a brute-force double sum
-// over all particles at sampled vertices, and the anisotropic field's
-// total against the isotropic one (both integrate each particle's volume).
+// over all particles at sampled vertices, the anisotropic field's total
+// against the isotropic one (both integrate each particle's volume), and
+// generate's marching-tetrahedra surface (counter, vertex buffer, unpacking).
 // Prints one JSON line.
 const path = require('path');
 const root = path.join(__dirname, '..', '..', 'particles4all_amd', 'node');
@@ -70,4 +71,29 @@ out.isoIntegral = sumIso * v3;
 out.anisoIntegral = sumAn * v3;
 out.anisoMax = maxAn;
 out.nonFinite = nonFinite;
+// generate(sim, iso, smoothPasses): field, blur, normal field, march
+mesh.normalSmoothPasses = 2;
+mesh.generate(sim, 0.4, 1);
+const tri = mesh.pollTriangles(sim);
+out.triangles = tri.total;
+out.emitted = tri.emitted;
+out.lastTriangles = mesh.lastTriangles;
+out.vertsBytes = mesh.verts.bytes;
+out.vertsPtrNonZero = mesh.verts.ptr !== 0n;
+const vw = mesh.vertices(sim);
+out.vertWords = vw.length;
+// unpack as the mesh vertex shader does (mesh_wgsl.js:271-283)
+let outside = 0, badNormal = 0;
+for (let k = 0; k < vw.length; k += 2) {
+  const q = [vw[k] & 0xffff, vw[k] >>> 16, vw[k + 1] & 0xffff];
+  for (let a = 0; a < 3; a++) {
+    const w = mesh.origin[a] + q[a] / 65535 * (vd[a] - 1) * mesh.voxel;
+    if (w < -mesh.voxel - 1e-6 || w > sim.params.box[a] + 2 * mesh.voxel + 1e-6) outside++;
+  }
+  const oct = vw[k + 1] >>> 16;
+  const e = [(oct & 0xff) / 255 * 2 - 1, (oct >>> 8) / 255 * 2 - 1];
+  if (Math.abs(e[0]) > 1 || Math.abs(e[1]) > 1) badNormal++;
+}
+out.vertsOutside = outside;
+out.badNormal = badNormal;
 console.log(JSON.stringify(out));

@@ -40,3 +40,10 @@ def test_js_surface_mesh_field():
     assert out["nonFinite"] == 0 and out["anisoMax"] > 0
     # both fields integrate each particle's volume m / max(rho, rho0/2)
     assert abs(out["anisoIntegral"] / out["isoIntegral"] - 1) < 0.05
+    # generate: a surface within the budget, its counter, packed vertices
+    # inside the lattice
+    assert out["triangles"] > 1000 and out["emitted"] == out["triangles"]
+    assert out["lastTriangles"] == out["triangles"]
+    assert out["vertsPtrNonZero"] and out["vertsBytes"] == 3000000 * 24
+    assert out["vertWords"] == 6 * out["emitted"]
+    assert out["vertsOutside"] == 0 and out["badNormal"] == 0

@@ -231,7 +231,11 @@ enum p4a_buf {
   P4A_BUF_POS = 0,         /* float4 n   (sim.buf.posA/B, livePos) */
   P4A_BUF_VEL = 1,         /* float4 n   (liveVel) */
   P4A_BUF_PRED = 2,        /* float4 n   (pred of the live parity) */
-  P4A_BUF_BODY = 3,        /* uint4 n, x = body id (liveBody) */
+  P4A_BUF_BODY = 3,        /* uint4 n, x = body id (liveBody); p4a_read builds
+                              it on the host; p4a_device_ptr makes it a device
+                              buffer kept current after every step, stage and
+                              append from then on (the SSFR renderer binds it
+                              with P4A_BUF_SMOOTH_POS and P4A_BUF_ANISO_G) */
   P4A_BUF_REST = 4,        /* float4 n   (restA/B) */
   P4A_BUF_LAMBDA = 5,      /* float n */
   P4A_BUF_DENSITY = 6,     /* float n */
@@ -263,6 +267,7 @@ enum p4a_buf {
                               of the last p4a_mesh_march */
   P4A_BUF_MESH_INDIRECT = 27, /* uint 4: draw-indirect arguments of the last
                               march (vertex count, 1, 0, 0) */
+  P4A_BUF_BODY_PACKED = 28, /* float4 6*n_bodies: the last p4a_body_pack */
   P4A_BUF_COUNT_
 };
 
@@ -394,6 +399,17 @@ P4A_API int p4a_field_smooth(p4a_sim* h, int32_t passes);
 P4A_API int p4a_mesh_normal_field(p4a_sim* h, int32_t passes);
 P4A_API int p4a_mesh_march(p4a_sim* h, float iso);
 P4A_API int p4a_mesh_triangles(p4a_sim* h, int64_t* total, int64_t* emitted);
+
+/* ---------------------------------------------------------------------
+ * Body poses for the solid renderer: bodyPackWGSL (src/solid_wgsl.js:3-26)
+ * as Solids.build / Solids.pack run it (src/solids.js:107-161).  stat holds
+ * 8 floats per body as Solids.build fills it (half extents xyz, shape id;
+ * colour rgb, 0); null keeps the last upload.  count must equal n_bodies.
+ * Writes P4A_BUF_BODY_PACKED from the current poses, per body 6 float4:
+ * (centre, shape id), (half extents, 0), the three columns of R (each
+ * (R0k, R1k, R2k, 0)), (colour, 0).  Asynchronous.
+ * ------------------------------------------------------------------- */
+P4A_API int p4a_body_pack(p4a_sim* h, const float* stat, int32_t count);
 
 /* Wait for all queued GPU work of the handle. */
 P4A_API int p4a_sync(p4a_sim* h);

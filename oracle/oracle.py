@@ -276,6 +276,34 @@ def mesh_march(vd, iso, max_tris, field, nfield=None):
     return verts[:keep], int(total)
 
 
+def body_pack(centre, rot, stat):
+    """bodyPackWGSL (solid_wgsl.js:3-26): per body 6 float4 = (centre, shape
+    id), (half extents, 0), the columns of R (rot holds 3 rows per body),
+    (colour, 0)."""
+    centre = np.asarray(centre, np.float32).reshape(-1, 4)
+    rot = np.asarray(rot, np.float32).reshape(-1, 3, 4)
+    stat = np.asarray(stat, np.float32).reshape(-1, 2, 4)
+    nb = len(centre)
+    out = np.zeros((nb, 6, 4), np.float32)
+    out[:, 0, :3] = centre[:, :3]
+    out[:, 0, 3] = stat[:, 0, 3]
+    out[:, 1, :3] = stat[:, 0, :3]
+    for k in range(3):
+        out[:, 2 + k, :3] = rot[:, :, k]
+    out[:, 5, :3] = stat[:, 1, :3]
+    return out.reshape(nb * 6, 4)
+
+
+def ssfr_kernel_g(g):
+    """The SSFR splat's reading of the packed G (ssfr_wgsl.js:21-28): the
+    symmetric matrix of each particle's two float4."""
+    g = np.asarray(g, np.float64).reshape(-1, 2, 4)
+    a, b = g[:, 0], g[:, 1]
+    return np.stack([np.stack([a[:, 0], a[:, 1], a[:, 2]], -1),
+                     np.stack([a[:, 1], a[:, 3], b[:, 0]], -1),
+                     np.stack([a[:, 2], b[:, 0], b[:, 1]], -1)], 1)
+
+
 def mesh_decode(verts, lattice):
     """The mesh vertex shader's unpacking (mesh_wgsl.js:271-283), in FP64:
     world positions (m, 3) and unit normals (m, 3) of packed vertices."""

@@ -26,7 +26,7 @@ BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CO
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
            PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20,
            OFS=21, CELL_OF_SLOT=22, DRIFT=23, FIELD=24, FIELD_NORMAL=25, MESH_VERTS=26,
-           MESH_INDIRECT=27)
+           MESH_INDIRECT=27, BODY_PACKED=28)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -101,7 +101,7 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_timing", "p4a_version", "p4a_aniso_params_default", "p4a_aniso",
            "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name", "p4a_field_configure",
            "p4a_field_generate", "p4a_field_get_info", "p4a_field_smooth",
-           "p4a_mesh_normal_field", "p4a_mesh_march", "p4a_mesh_triangles",
+           "p4a_mesh_normal_field", "p4a_mesh_march", "p4a_mesh_triangles", "p4a_body_pack",
            "p4a_batch_create", "p4a_batch_destroy",
            "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_sync",
            "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
@@ -142,6 +142,7 @@ def lib():
     L.p4a_field_configure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.c_int32, C.c_int64]
     L.p4a_field_smooth.argtypes = [vp, C.c_int32]
+    L.p4a_body_pack.argtypes = [vp, vp, C.c_int32]
     L.p4a_mesh_normal_field.argtypes = [vp, C.c_int32]
     L.p4a_mesh_march.argtypes = [vp, C.c_float]
     L.p4a_mesh_triangles.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -355,7 +356,8 @@ class Sim:
                 "FIELD": (np.float32, self.field_info().n_verts, 1),
                 "FIELD_NORMAL": (np.float32, self.field_info().n_verts, 1),
                 "MESH_VERTS": (np.uint32, 3 * self.field_info().max_triangles, 2),
-                "MESH_INDIRECT": (np.uint32, 4, 1)}[name]
+                "MESH_INDIRECT": (np.uint32, 4, 1),
+                "BODY_PACKED": (np.float32, 6 * i.n_bodies, 4)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")
@@ -437,6 +439,13 @@ class Sim:
                                               (C.c_double * 3)(*box_max), int(resolution),
                                               int(max_triangles)),
                     "field_configure")
+
+    def body_pack(self, stat=None):
+        """Solids.pack (solids.js:147-161); stat (n_bodies, 8) float32 as
+        Solids.build fills it, None keeps the last upload."""
+        arr = None if stat is None else np.ascontiguousarray(stat, np.float32)
+        self._check(lib().p4a_body_pack(self.h, None if arr is None else _ptr(arr),
+                                        self.info().n_bodies), "body_pack")
 
     def field_smooth(self, passes):
         """SurfaceMesh.smoothField (mesh.js:273-291), in place."""

@@ -18,6 +18,8 @@
 //                                           k_impulse, k_resize
 //   anisoWGSL (aniso_wgsl.js:3-159)      -> k_tile<AnisoOp> (5x5x5 cells)
 //   meshDensityWGSL (mesh_wgsl.js:23-69) -> k_field_records + k_field_iso
+//   scatterPhase's body buffer (wgsl.js:460-475) -> k_export_phase (on request)
+//   bodyPackWGSL (solid_wgsl.js:3-26)    -> k_body_pack
 // Forms of this build without a reference kernel: the FP16-offset solver
 // iterations (k_direct16<Lambda16/Delta16>), the anisotropic field splat
 // (k_field_aniso_prep + k_field_splat) and the batched-scene wrappers
@@ -2303,6 +2305,35 @@ k_field_fixed_to_float(uint32_t nv, unsigned long long* __restrict__ acc,
   field[i] = (float)((double)(long long)acc[i] * (1.0 / FIELD_FIXED_SCALE));
 }
 
+// Per-slot phase in the live order (the reference's body buffer, x = body
+// id, 0 for fluid; scatterPhase keeps it in slot order, wgsl.js:460-475).
+__global__ void __launch_bounds__(WG)
+k_export_phase(uint32_t n, const uint32_t* __restrict__ pid, const uint32_t* __restrict__ bodyOfPid,
+               uint4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= n) return;
+  out[i] = make_uint4(bodyOfPid[pid[i]], 0u, 0u, 0u);
+}
+
+// bodyPackWGSL (solid_wgsl.js:3-26): per body 6 float4 = (centre, shape id),
+// (half extents, 0), the columns of R, (colour, 0).
+__global__ void __launch_bounds__(64)
+k_body_pack(uint32_t count, const float4* __restrict__ centre, const float4* __restrict__ rot,
+            const float4* __restrict__ stat, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= count) return;
+  const float4 s = stat[2 * i], c = stat[2 * i + 1];
+  const float4 r0 = rot[3 * i], r1 = rot[3 * i + 1], r2 = rot[3 * i + 2];
+  const float4 ce = centre[i];
+  float4* o = out + 6 * (size_t)i;
+  o[0] = make_float4(ce.x, ce.y, ce.z, s.w);
+  o[1] = make_float4(s.x, s.y, s.z, 0.0f);
+  o[2] = make_float4(r0.x, r1.x, r2.x, 0.0f);
+  o[3] = make_float4(r0.y, r1.y, r2.y, 0.0f);
+  o[4] = make_float4(r0.z, r1.z, r2.z, 0.0f);
+  o[5] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
 __global__ void __launch_bounds__(WG)
 k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
   const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
@@ -2591,6 +2622,21 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy4, dim3(256 * 32), dim3(WG), 0, s, a, b, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_phase(uint32_t n, const uint32_t* pid, const uint32_t* bodyOfPid,
+                               uint4* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_phase, dim3(blocks(n)), dim3(WG), 0, s, n, pid, bodyOfPid, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_body_pack(uint32_t count, const float4* centre, const float4* rot,
+                            const float4* stat, float4* out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_body_pack, dim3((count + 63) / 64), dim3(64), 0, s, count, centre, rot,
+                     stat, out);
   return hipGetLastError();
 }
 

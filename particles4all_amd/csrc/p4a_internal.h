@@ -223,6 +223,10 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
                               float4* recA, float4* recB, unsigned long long* acc,
                               float* field, hipStream_t s);
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s);
+hipError_t launch_export_phase(uint32_t n, const uint32_t* pid, const uint32_t* bodyOfPid,
+                               uint4* out, hipStream_t s);
+hipError_t launch_body_pack(uint32_t count, const float4* centre, const float4* rot,
+                            const float4* stat, float4* out, hipStream_t s);
 // Exclusive scan of count[0..n) into out[0..n], out[n] = the total;
 // consumed counts are zeroed.  blockSum holds scan_blocks(n) entries.
 int scan_blocks(uint32_t n);

@@ -199,6 +199,14 @@ struct p4a_sim {
   size_t meshVertCap = 0;  // triangles the vertex buffer holds
   uint2* meshVerts = nullptr;
   std::vector<void*> meshVertAllocs;
+  // renderer exports: the per-slot phase (kept current once requested) and
+  // the packed body poses
+  bool exportPhase = false;
+  uint4* phaseOut = nullptr;
+  float4* bodyStat = nullptr;
+  float4* bodyPacked = nullptr;
+  bool bodyStatSet = false;
+  std::vector<void*> exportAllocs;
 };
 
 namespace {
@@ -572,6 +580,9 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_DRIFT: return {s->A.drift, (size_t)DRIFT_SLOTS * 4};
     case P4A_BUF_FIELD: return {s->field, s->field ? s->fieldVerts * 4 : 0};
     case P4A_BUF_FIELD_NORMAL: return {s->nfield, s->nfield ? s->fieldVerts * 4 : 0};
+    case P4A_BUF_BODY: return {s->exportPhase ? s->phaseOut : nullptr, s->exportPhase ? n * 16 : 0};
+    case P4A_BUF_BODY_PACKED:
+      return {s->bodyPacked, s->bodyPacked ? (size_t)s->nBodies * 96 : (size_t)0};
     case P4A_BUF_MESH_VERTS: return {s->meshVerts, s->meshVerts ? (size_t)s->meshMaxTris * 24 : 0};
     case P4A_BUF_MESH_INDIRECT: return {s->meshCounter ? s->meshCounter + 4 : nullptr,
                                         s->meshCounter ? (size_t)16 : (size_t)0};
@@ -582,6 +593,15 @@ BufView buf_view(p4a_sim* s, int id) {
 int check(p4a_sim* s) {
   if (!s) return P4A_ERR_ARG;
   if (!s->ready) return fail(s, P4A_ERR_STATE, "reset() has not been called");
+  return P4A_OK;
+}
+
+// Rewrites the per-slot phase after anything that moves slots, once a caller
+// has asked for its device buffer.
+int refresh_exports(p4a_sim* s) {
+  if (!s->exportPhase) return P4A_OK;
+  LAUNCH(s, "export_phase", launch_export_phase((uint32_t)s->n, s->A.pid[s->parity],
+                                                s->A.bodyOfPid, s->phaseOut, s->stream));
   return P4A_OK;
 }
 
@@ -665,6 +685,7 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->fieldAllocs);
   free_list(s->fieldRecAllocs);
   free_list(s->meshVertAllocs);
+  free_list(s->exportAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
   hipStreamDestroy(s->stream);
@@ -694,6 +715,11 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   free_list(s->bndAllocs);
   free_list(s->anisoAllocs);
   free_list(s->fieldRecAllocs);
+  free_list(s->exportAllocs);
+  s->exportPhase = false;
+  s->phaseOut = nullptr;
+  s->bodyStat = s->bodyPacked = nullptr;
+  s->bodyStatSet = false;
   s->smoothPos = nullptr;
   s->anisoG = nullptr;
   s->anisoFallbacks = nullptr;
@@ -887,8 +913,8 @@ P4A_API int p4a_substeps(p4a_sim* s, double dt, int32_t count) {
   }
   for (int k = 0; k < count; k++)
     if ((rc = run_substep(s, dt))) return rc;
-  if (count > 0) return run_stats(s);
-  return P4A_OK;
+  if (count > 0 && (rc = run_stats(s))) return rc;
+  return count > 0 ? refresh_exports(s) : P4A_OK;
 }
 
 P4A_API int p4a_step(p4a_sim* s, double frame_dt, int32_t* substeps_run) {
@@ -963,6 +989,7 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
     default:
       return fail(s, P4A_ERR_ARG, "unknown stage");
   }
+  if ((rc = refresh_exports(s))) return rc;
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   return P4A_OK;
 }
@@ -1052,7 +1079,7 @@ P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, i
   HIP_TRY(s, hipMemcpy(s->A.slotOf + off, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
   s->n += k;
   if (appended) *appended = k;
-  return P4A_OK;
+  return refresh_exports(s);
 }
 
 P4A_API int p4a_apply_ray_impulse(p4a_sim* s, const double origin[3], const double dir[3],
@@ -1149,6 +1176,15 @@ P4A_API int p4a_device_ptr(p4a_sim* s, int32_t id, void** ptr, size_t* bytes) {
   int rc = check(s);
   if (rc) return rc;
   if (!ptr) return P4A_ERR_ARG;
+  if (id == P4A_BUF_BODY && !s->exportPhase) {
+    // a renderer binds the per-slot phase: keep it on the device from now on
+    HIP_TRY(s, hipSetDevice(s->device));
+    if (!s->phaseOut &&
+        (rc = dalloc(s, s->exportAllocs, &s->phaseOut, (size_t)std::max(1, s->cap))))
+      return rc;
+    s->exportPhase = true;
+    if ((rc = refresh_exports(s))) return rc;
+  }
   BufView v = buf_view(s, id);
   if (!v.ptr) return fail(s, P4A_ERR_ARG, "no device buffer with that id");
   *ptr = v.ptr;
@@ -1395,6 +1431,28 @@ MeshArgs mesh_args(const p4a_sim* s, float iso) {
 }
 
 }  // namespace
+
+P4A_API int p4a_body_pack(p4a_sim* s, const float* stat, int32_t count) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (count != s->nBodies) return fail(s, P4A_ERR_ARG, "count must equal the number of bodies");
+  if (count == 0) return P4A_OK;
+  HIP_TRY(s, hipSetDevice(s->device));
+  if (!s->bodyPacked) {
+    if ((rc = dalloc(s, s->exportAllocs, &s->bodyStat, (size_t)count * 2))) return rc;
+    if ((rc = dalloc(s, s->exportAllocs, &s->bodyPacked, (size_t)count * 6))) return rc;
+  }
+  if (stat) {
+    HIP_TRY(s, hipMemcpyAsync(s->bodyStat, stat, (size_t)count * 32, hipMemcpyHostToDevice,
+                              s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->bodyStatSet = true;
+  }
+  if (!s->bodyStatSet) return fail(s, P4A_ERR_ARG, "no body static data uploaded yet");
+  LAUNCH(s, "body_pack", launch_body_pack((uint32_t)count, s->A.bodyCentre, s->A.bodyRot,
+                                          s->bodyStat, s->bodyPacked, s->stream));
+  return P4A_OK;
+}
 
 P4A_API int p4a_field_smooth(p4a_sim* s, int32_t passes) {
   int rc = check(s);

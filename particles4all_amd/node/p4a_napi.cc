@@ -546,6 +546,21 @@ napi_value FieldConfigure(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// bodyPack(h, stat Float32Array | null, count)
+napi_value BodyPack(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 3);
+  p4a_sim* h = handle(env, a[0]);
+  const float* f = nullptr;
+  size_t len = 0;
+  const int32_t count = (int32_t)num(env, a[2]);
+  if (typed_f32(env, a[1], &f, &len) && len < (size_t)count * 8) {
+    napi_throw_error(env, nullptr, "bodyPack: stat needs 8 floats per body");
+    return nullptr;
+  }
+  P4A_CALL(h, p4a_body_pack(h, f, count), "p4a_body_pack");
+  return nullptr;
+}
+
 // fieldSmooth(h, passes)
 napi_value FieldSmooth(napi_env env, napi_callback_info info) {
   auto a = args(env, info, 2);
@@ -647,7 +662,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"aniso", Aniso},           {"fieldConfigure", FieldConfigure},
              {"fieldGenerate", FieldGenerate}, {"fieldInfo", FieldInfo},
              {"fieldSmooth", FieldSmooth}, {"meshNormalField", MeshNormalField},
-             {"meshMarch", MeshMarch},   {"meshTriangles", MeshTriangles}};
+             {"meshMarch", MeshMarch},   {"meshTriangles", MeshTriangles},
+             {"bodyPack", BodyPack}};
   for (auto& f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

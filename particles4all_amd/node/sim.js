@@ -12,7 +12,7 @@ const BUF = {
   pos: 0, vel: 1, pred: 2, body: 3, rest: 4, lambda: 5, density: 6, slot: 7, corr: 8,
   normal: 9, cellStart: 10, bpos: 11, bcellStart: 12, bodyCentre: 13, bodyRot: 14,
   pid: 15, key: 16, order: 17, bpsi: 18, smoothPos: 19, anisoG: 20, ofs: 21, cellOfSlot: 22,
-  drift: 23, field: 24, fieldNormal: 25, meshVerts: 26, meshIndirect: 27,
+  drift: 23, field: 24, fieldNormal: 25, meshVerts: 26, meshIndirect: 27, bodyPacked: 28,
 };
 const STAGE = { predict: 0, grid: 1, bodyPre: 2, lambda: 3, delta: 4, bodySolve: 5,
                 finish: 6, stats: 7 };
@@ -75,7 +75,9 @@ class Sim {
     this.covScale = i.covScale;
     const buf = {};
     for (const k of Object.keys(BUF)) {
-      if (k === 'body' || k === 'rest') continue;  // synthesised on read
+      if (k === 'rest') continue;  // synthesised on read
+      // buf.body: asking for its device pointer keeps the per-slot phase on
+      // the device, current after every step (what the SSFR splat binds)
       try { buf[k] = native.devicePtr(this.handle, BUF[k]); } catch (e) { /* empty buffer */ }
     }
     this.buf = buf;
@@ -146,7 +148,7 @@ class Sim {
 
   livePos() { return this.buf.pos; }
   liveVel() { return this.buf.vel; }
-  liveBody() { return this.read('body'); }
+  liveBody() { return this.buf.body; }
 
   // readback of a buffer as a typed array (the reference's __readBuf,
   // main.js:810-820); particle arrays are in the live order.

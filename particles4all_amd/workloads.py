@@ -18,10 +18,10 @@ import numpy as np
 from ._lib import (SHAPE_LATTICE_CUBE, SHAPE_LATTICE_SPHERE, default_params, lattice_scene)
 
 
-def _params(box, substeps, iterations, xsph, tension, gravity=9.8, **kw):
+def _params(box, substeps, iterations, xsph, tension, gravity=9.8, pour_budget=0, **kw):
     p = default_params(spacing=1.0, rest_density=1000.0, gravity=gravity, substeps=substeps,
                        iterations=iterations, xsph_c=xsph, surface_tension_k=tension,
-                       box=box, pour_budget=0, **kw)
+                       box=box, pour_budget=pour_budget, **kw)
     return p
 
 

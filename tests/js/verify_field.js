@@ -10,6 +10,7 @@ const root = path.join(__dirname, '..', '..', 'particles4all_amd', 'node');
 const { Sim } = require(path.join(root, 'sim.js'));
 const { SurfaceMesh } = require(path.join(root, 'mesh.js'));
 const { loadPreset } = require(path.join(root, 'settingsini.js'));
+const { Solids, halfExtent, shapeId } = require(path.join(root, 'solids.js'));
 
 const out = {};
 const { params } = loadPreset('small', 'particles=12000');
@@ -96,4 +97,26 @@ for (let k = 0; k < vw.length; k += 2) {
 }
 out.vertsOutside = outside;
 out.badNormal = badNormal;
+// renderer exports: the per-slot phase on the device, and the packed poses
+out.liveBodyBytes = sim.liveBody().bytes;
+out.liveBodyPtrNonZero = sim.liveBody().ptr !== 0n;
+const solids = new Solids();
+solids.build(sim);
+sim.step(1 / 60);
+solids.pack(sim);
+const pk = solids.read(sim);
+const centre = sim.read('bodyCentre'), rot = sim.read('bodyRot');
+let packErr = 0;
+sim.scene.bodies.forEach((b, i) => {
+  const o = i * 24, hx = halfExtent(b);
+  for (let a = 0; a < 3; a++) {
+    packErr += Math.abs(pk[o + a] - centre[i * 4 + a]);
+    packErr += Math.abs(pk[o + 4 + a] - Math.fround(hx[a]));
+    for (let r = 0; r < 3; r++) packErr += Math.abs(pk[o + 8 + 4 * a + r] - rot[r * 4 + a + i * 12]);
+  }
+  packErr += Math.abs(pk[o + 3] - shapeId(b.shape));
+});
+out.solidsCount = solids.count;
+out.packErr = packErr;
+out.packedBytes = solids.packed.bytes;
 console.log(JSON.stringify(out));

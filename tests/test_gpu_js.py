@@ -47,3 +47,7 @@ def test_js_surface_mesh_field():
     assert out["vertsPtrNonZero"] and out["vertsBytes"] == 3000000 * 24
     assert out["vertWords"] == 6 * out["emitted"]
     assert out["vertsOutside"] == 0 and out["badNormal"] == 0
+    # renderer exports: device phase buffer, packed body poses
+    assert out["liveBodyPtrNonZero"] and out["liveBodyBytes"] > 0
+    assert out["solidsCount"] == 3 and out["packedBytes"] == 3 * 96
+    assert out["packErr"] == 0

@@ -419,3 +419,32 @@ def test_fp16_quantize_rounds_offsets_from_cell_centres():
     o = pts[:, 1:3] - O.cell_centres(P, cid)[:, 1:]
     bound = 0.5 * np.spacing(np.abs(o).astype(np.float16)).astype(np.float64)
     assert np.all(np.abs(q[:, 1:3] - pts[:, 1:3]) <= bound + 1e-9)
+
+
+def test_body_pack_closed_form():
+    """bodyPackWGSL restated: a rotation of 90 degrees about z (rows (0,-1,0),
+    (1,0,0), (0,0,1)) packs its columns (0,1,0), (-1,0,0), (0,0,1)."""
+    centre = np.array([[1.0, 2.0, 3.0, 0.0], [4.0, 5.0, 6.0, 0.0]])
+    rot = np.array([[0, -1, 0, 0], [1, 0, 0, 0], [0, 0, 1, 0],
+                    [1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]], np.float64)
+    stat = np.array([[0.9, 0.7, 0.9, 0, 0.55, 0.72, 0.95, 0],
+                     [2.0, 2.0, 2.0, 1, 0.60, 0.88, 0.66, 0]])
+    out = O.body_pack(centre, rot, stat).reshape(2, 6, 4)
+    assert np.array_equal(out[0, 0], np.float32([1, 2, 3, 0]))
+    assert np.array_equal(out[1, 0], np.float32([4, 5, 6, 1]))
+    assert np.array_equal(out[0, 1], np.float32([0.9, 0.7, 0.9, 0]))
+    assert np.array_equal(out[0, 2:5, :3], np.float32([[0, 1, 0], [-1, 0, 0], [0, 0, 1]]))
+    assert np.array_equal(out[1, 2:5, :3], np.eye(3, dtype=np.float32))
+    assert np.array_equal(out[1, 5], np.float32([0.60, 0.88, 0.66, 0]))
+
+
+def test_ssfr_kernel_g_layout():
+    """The packed G (G00, G10, G20, G11), (G21, G22, det, n) read back as the
+    SSFR splat does gives the symmetric matrix it was packed from."""
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(5, 3, 3))
+    G = a @ a.transpose(0, 2, 1) + np.eye(3)
+    g = np.zeros((5, 2, 4))
+    g[:, 0] = np.stack([G[:, 0, 0], G[:, 1, 0], G[:, 2, 0], G[:, 1, 1]], -1)
+    g[:, 1] = np.stack([G[:, 2, 1], G[:, 2, 2], np.linalg.det(G), np.full(5, 30.0)], -1)
+    assert np.allclose(O.ssfr_kernel_g(g), G)

@@ -535,11 +535,12 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       if constexpr (Op::FAST && PH >= 1) {
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
           const uint32_t len = min(e - c0, 32u);
+          const float4* P = op.pred + c0;
           uint32_t m = 0u, k = 0u;
           for (; k + 3u < len; k += 4u) {
             float4 q[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) q[u] = op.pred[c0 + k + u];
+            for (int u = 0; u < 4; u++) q[u] = P[(size_t)k + u];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               const float r2 = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z);
@@ -547,7 +548,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
             }
           }
           for (; k < len; k++) {
-            const float4 q = op.pred[c0 + k];
+            const float4 q = P[k];
             const float r2 = r2_of(t.px - q.x, t.py - q.y, t.pz - q.z);
             m |= (r2 < U.h2 ? 1u : 0u) << k;
           }
@@ -586,15 +587,18 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           }
         }
       } else if constexpr (Op::FAST) {
-        uint32_t j = b;
-        for (; j + 3 < e; j += 4) {
+        // 64-bit candidate index: one address per group of four, the
+        // others as immediate offsets
+        size_t j = b;
+        const size_t eEnd = e;
+        for (; j + 3 < eEnd; j += 4) {
           float4 v[4][Op::NREC];
 #pragma unroll
           for (int u = 0; u < 4; u++) op.load(U, j + u, v[u]);
 #pragma unroll
           for (int u = 0; u < 4; u++) op.fpair(U, t, v[u]);
         }
-        for (; j < e; j++) {
+        for (; j < eEnd; j++) {
           float4 v[Op::NREC];
           op.load(U, j, v);
           op.fpair(U, t, v);
@@ -654,7 +658,7 @@ struct LambdaOp : OpBase {
     uint32_t i;
     float rho, gx, gy, gz, sg2;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void load(const Uniform&, size_t j, float4* v) const { v[0] = pred[j]; }
   __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
     float4 p = pred[i];
     t.px = p.x; t.py = p.y; t.pz = p.z; t.i = i;
@@ -736,7 +740,7 @@ struct DeltaOp : OpBase {
     float px, py, pz, li;
     float dx, dy, dz, bx, by, bz, nc;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+  __device__ void load(const Uniform&, size_t j, float4* v) const {
     float4 p = pred[j];
     v[0] = make_float4(p.x, p.y, p.z, lambda[j]);
   }
@@ -825,7 +829,7 @@ struct XsphOp : OpBase {
     float px, py, pz, vx, vy, vz;
     float ax, ay, az, nx, ny, nz;
   };
-  __device__ void load(const Uniform& U, uint32_t j, float4* v) const {
+  __device__ void load(const Uniform& U, size_t j, float4* v) const {
     float4 p = pred[j], q = pos[j];
     v[0] = make_float4(p.x, p.y, p.z, fmaxf(density[j], 0.5f * U.rho0));
     v[1] = make_float4((p.x - q.x) * U.invDt, (p.y - q.y) * U.invDt, (p.z - q.z) * U.invDt, 0.0f);
@@ -893,7 +897,7 @@ struct TensionOp : OpBase {
     float px, py, pz, nx, ny, nz, rho;
     float ax, ay, az;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+  __device__ void load(const Uniform&, size_t j, float4* v) const {
     float4 p = pred[j];
     v[0] = make_float4(p.x, p.y, p.z, density[j]);
     v[1] = normal[j];
@@ -981,7 +985,7 @@ struct LambdaFast : FastBase {
     float st3, gx, gy, gz, sg2;   // fluid: sum t^3, sum s r, sum s^2 r^2
     float rb, bx, by, bz;         // walls: sum psi t^3, sum psi s r
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void load(const Uniform&, size_t j, float4* v) const { v[0] = pred[j]; }
   __device__ void target(const Uniform&, uint32_t i, Target& t) const {
     const float4 p = pred[i];
     t.px = p.x; t.py = p.y; t.pz = p.z;
@@ -992,10 +996,11 @@ struct LambdaFast : FastBase {
     const float r2 = r2_of(rx, ry, rz);
     const float tt = fmaxf(U.h2 - r2, 0.0f);
     t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
-    const float ir = __builtin_amdgcn_rsqf(r2);
+    // r^2 clamped below at 1e-12: the self pair and coincident particles
+    // (r = 0) contribute s * 0 = 0 without a select
+    const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
     const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-    float s = hr * hr * ir;
-    s = (r2 > 1.0e-12f) ? s : 0.0f;
+    const float s = hr * hr * ir;
     t.gx = __fmaf_rn(s, rx, t.gx);
     t.gy = __fmaf_rn(s, ry, t.gy);
     t.gz = __fmaf_rn(s, rz, t.gz);
@@ -1030,7 +1035,10 @@ struct LambdaFast : FastBase {
   }
 };
 
-struct DeltaFast : FastBase {
+// SOR = false (sorAverage off, the default): the neighbour count delta-p
+// would be divided by is not formed at all.
+template <bool SOR>
+struct DeltaFastT : FastBase {
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* __restrict__ pred;
@@ -1040,7 +1048,7 @@ struct DeltaFast : FastBase {
     float px, py, pz, li;
     float dx, dy, dz, bx, by, bz, nc;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void load(const Uniform&, size_t j, float4* v) const { v[0] = pred[j]; }
   __device__ void target(const Uniform&, uint32_t i, Target& t) const {
     const float4 p = pred[i];
     t.px = p.x; t.py = p.y; t.pz = p.z; t.li = p.w;
@@ -1062,7 +1070,7 @@ struct DeltaFast : FastBase {
     t.dx = __fmaf_rn(f, rx, t.dx);
     t.dy = __fmaf_rn(f, ry, t.dy);
     t.dz = __fmaf_rn(f, rz, t.dz);
-    t.nc += ok ? 1.0f : 0.0f;
+    if constexpr (SOR) t.nc += ok ? 1.0f : 0.0f;
   }
   __device__ void bpair(const Uniform& U, Target& t, float4 b) const {
     const float rx = t.px - b.x, ry = t.py - b.y, rz = t.pz - b.z;
@@ -1079,7 +1087,7 @@ struct DeltaFast : FastBase {
     const float cf = U.spikyGrad * U.volume, cb = U.spikyGrad * U.invRho0;
     const float dx = cf * t.dx + cb * t.bx, dy = cf * t.dy + cb * t.by, dz = cf * t.dz + cb * t.bz;
     float scale = U.omega;
-    if (U.sorAverage == 1u) scale = U.omega / fmaxf(t.nc, 1.0f);
+    if (SOR && U.sorAverage == 1u) scale = U.omega / fmaxf(t.nc, 1.0f);
     float ox = t.px + dx * scale, oy = t.py + dy * scale, oz = t.pz + dz * scale;
     ox = fminf(fmaxf(ox, U.clampMin[0]), U.clampMax[0]);
     oy = fminf(fmaxf(oy, U.clampMin[1]), U.clampMax[1]);
@@ -1105,7 +1113,7 @@ struct XsphFast : FastBase {
     float px, py, pz, vx, vy, vz;
     float ax, ay, az, nx, ny, nz;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+  __device__ void load(const Uniform&, size_t j, float4* v) const {
     v[0] = pred[j];
     v[1] = velq[j];
   }
@@ -1161,7 +1169,7 @@ struct TensionFast : FastBase {
   __device__ static float4 sentinel(int k) {
     return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const {
+  __device__ void load(const Uniform&, size_t j, float4* v) const {
     v[0] = pred[j];
     v[1] = normal[j];
   }
@@ -1325,10 +1333,11 @@ struct Lambda16 : LambdaFast {
     const float r2 = r2_of(rx, ry, rz);
     const float tt = fmaxf(U.h2 - r2, 0.0f);
     t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
-    const float ir = __builtin_amdgcn_rsqf(r2);
+    // r^2 clamped below at 1e-12: the self pair and coincident particles
+    // (r = 0) contribute s * 0 = 0 without a select
+    const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
     const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-    float s = hr * hr * ir;
-    s = (r2 > 1.0e-12f) ? s : 0.0f;
+    const float s = hr * hr * ir;
     t.gx = __fmaf_rn(s, rx, t.gx);
     t.gy = __fmaf_rn(s, ry, t.gy);
     t.gz = __fmaf_rn(s, rz, t.gz);
@@ -1341,6 +1350,8 @@ struct Lambda16 : LambdaFast {
 
 // delta over FP16-offset records (lambda_j from the lambda array); writes
 // the moved FP32 position and its re-encoded record
+using DeltaFast = DeltaFastT<true>;
+
 struct Delta16 : DeltaFast {
   HRec* ofsOut;
   const uint32_t* cellOfSlot;
@@ -1364,7 +1375,7 @@ struct Delta16 : DeltaFast {
     t.dx = __fmaf_rn(f, rx, t.dx);
     t.dy = __fmaf_rn(f, ry, t.dy);
     t.dz = __fmaf_rn(f, rz, t.dz);
-    t.nc += ok ? 1.0f : 0.0f;
+    if (U.sorAverage == 1u) t.nc += ok ? 1.0f : 0.0f;  // uniform: a scalar branch
   }
   __device__ void finish16(const Uniform& U, uint32_t i, Target& t) const {
     DeltaFast::finish(U, i, t);
@@ -1386,7 +1397,7 @@ struct CensusOp : OpBase {
     float px, py, pz;
     uint32_t i, c, a, b;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pred[j]; }
+  __device__ void load(const Uniform&, size_t j, float4* v) const { v[0] = pred[j]; }
   __device__ void target(const Uniform&, uint32_t i, Target& t) const {
     float4 p = pred[i];
     t.px = p.x; t.py = p.y; t.pz = p.z; t.i = i; t.c = t.a = t.b = 0u;
@@ -1487,7 +1498,7 @@ struct AnisoOp : OpBase {
     float sx, sy, sz, m00, m10, m20, m11, m21, m22, wsum;
     int n;
   };
-  __device__ void load(const Uniform&, uint32_t j, float4* v) const { v[0] = pos[j]; }
+  __device__ void load(const Uniform&, size_t j, float4* v) const { v[0] = pos[j]; }
   __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
     const float4 p = pos[i];
     t.px = p.x; t.py = p.y; t.pz = p.z;
@@ -2423,7 +2434,9 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
                         const BatchTab* B) {
   if (exact) return run_tile1(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, s, B);
-  return run_tile1(U, A, drifted(DeltaFast{{}, pred, A.lambda, out}, d), t, s, B);
+  if (U.sorAverage == 1u)
+    return run_tile1(U, A, drifted(DeltaFastT<true>{{}, pred, A.lambda, out}, d), t, s, B);
+  return run_tile1(U, A, drifted(DeltaFastT<false>{{}, pred, A.lambda, out}, d), t, s, B);
 }
 
 template <class Op>

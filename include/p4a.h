@@ -411,6 +411,24 @@ P4A_API int p4a_mesh_triangles(p4a_sim* h, int64_t* total, int64_t* emitted);
  * ------------------------------------------------------------------- */
 P4A_API int p4a_body_pack(p4a_sim* h, const float* stat, int32_t count);
 
+/* ---------------------------------------------------------------------
+ * State snapshot (no reference counterpart: the reference keeps its state
+ * only on the GPU).  For reproducible long runs and regression fixtures:
+ * a sim loaded from a snapshot continues bit for bit as the saved one
+ * would have (same steps, same results), because the snapshot holds the
+ * live particle order, the FP64 body state, the last grid, the time bank
+ * and the parameters.  p4a_snapshot_size gives the bytes save writes;
+ * p4a_snapshot_load resets the handle from a snapshot (any handle: scene,
+ * capacity and box come from the snapshot).  Native byte order; a header
+ * records the layout version and sizes, and load rejects a snapshot of
+ * another version, a truncated one or one whose ids are not a
+ * permutation.  Render-side state (anisotropy, fields, meshes) is not
+ * part of it.
+ * ------------------------------------------------------------------- */
+P4A_API int p4a_snapshot_size(p4a_sim* h, size_t* bytes);
+P4A_API int p4a_snapshot_save(p4a_sim* h, void* dst, size_t bytes);
+P4A_API int p4a_snapshot_load(p4a_sim* h, const void* src, size_t bytes);
+
 /* Wait for all queued GPU work of the handle. */
 P4A_API int p4a_sync(p4a_sim* h);
 

@@ -102,6 +102,7 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_copy_probe", "p4a_tune", "p4a_tile_variant_name", "p4a_field_configure",
            "p4a_field_generate", "p4a_field_get_info", "p4a_field_smooth",
            "p4a_mesh_normal_field", "p4a_mesh_march", "p4a_mesh_triangles", "p4a_body_pack",
+           "p4a_snapshot_size", "p4a_snapshot_save", "p4a_snapshot_load",
            "p4a_batch_create", "p4a_batch_destroy",
            "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_sync",
            "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
@@ -143,6 +144,9 @@ def lib():
                                       C.c_int32, C.c_int64]
     L.p4a_field_smooth.argtypes = [vp, C.c_int32]
     L.p4a_body_pack.argtypes = [vp, vp, C.c_int32]
+    L.p4a_snapshot_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.p4a_snapshot_save.argtypes = [vp, vp, C.c_size_t]
+    L.p4a_snapshot_load.argtypes = [vp, vp, C.c_size_t]
     L.p4a_mesh_normal_field.argtypes = [vp, C.c_int32]
     L.p4a_mesh_march.argtypes = [vp, C.c_float]
     L.p4a_mesh_triangles.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -439,6 +443,27 @@ class Sim:
                                               (C.c_double * 3)(*box_max), int(resolution),
                                               int(max_triangles)),
                     "field_configure")
+
+    def snapshot(self):
+        """The whole simulation state as bytes (p4a_snapshot_save)."""
+        size = C.c_size_t()
+        self._check(lib().p4a_snapshot_size(self.h, C.byref(size)), "snapshot_size")
+        buf = np.empty(size.value, np.uint8)
+        self._check(lib().p4a_snapshot_save(self.h, _ptr(buf), buf.nbytes), "snapshot_save")
+        return buf.tobytes()
+
+    def load_snapshot(self, data):
+        """Reset this handle from snapshot bytes (p4a_snapshot_load)."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        self._check(lib().p4a_snapshot_load(self.h, _ptr(buf), buf.nbytes), "snapshot_load")
+
+    def save_snapshot(self, path):
+        with open(path, "wb") as f:
+            f.write(self.snapshot())
+
+    def load_snapshot_file(self, path):
+        with open(path, "rb") as f:
+            self.load_snapshot(f.read())
 
     def body_pack(self, stat=None):
         """Solids.pack (solids.js:147-161); stat (n_bodies, 8) float32 as

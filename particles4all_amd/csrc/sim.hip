@@ -1504,6 +1504,242 @@ P4A_API int p4a_mesh_triangles(p4a_sim* s, int64_t* total, int64_t* emitted) {
   return P4A_OK;
 }
 
+// ---- state snapshot ------------------------------------------------------
+// Layout (native byte order): SnapHeader, then the arrays in this order:
+//   pos float4[n], vel float4[n], pid u32[n], density f32[n]   (live order)
+//   bodyOfPid u32[n], restOfPid float4[n]                      (by particle id)
+//   cellStart u32[nCells + 2]                                  (last grid build)
+//   bodyCentreD f64[4 nB], bodyRotD f64[9 nB], bodyCentre float4[nB],
+//   bodyRot float4[3 nB], bodyInvRatio f32[nB]
+//   bpos float4[nBoundary]                                     (cell-sorted, w = psi)
+namespace {
+
+constexpr char kSnapMagic[8] = {'P', '4', 'A', 'S', 'N', 'A', 'P', '\0'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, headerBytes, paramsBytes, pad0;
+  uint64_t totalBytes;
+  p4a_params params;
+  double h, d, mass, denomRest, simTime, timeBank, lastAdvanced;
+  double heldTarget[3], heldRate, heldLimit;
+  int32_t n, nFluid, cap, pourBudget, nBodies, nBoundary, nCells, heldBody;
+  int32_t lastSubsteps, tile1, tile2, pad1;
+};
+
+size_t snap_bytes(size_t n, size_t nCells, size_t nB, size_t nBnd) {
+  return sizeof(SnapHeader) + n * (16 + 16 + 4 + 4 + 4 + 16) + (nCells + 2) * 4 +
+         nB * (32 + 72 + 16 + 48 + 4) + nBnd * 16;
+}
+
+struct Cursor {
+  unsigned char* w = nullptr;
+  const unsigned char* r = nullptr;
+  void put(const void* src, size_t bytes) {
+    if (bytes) std::memcpy(w, src, bytes);
+    w += bytes;
+  }
+  void get(void* dst, size_t bytes) {
+    if (bytes) std::memcpy(dst, r, bytes);
+    r += bytes;
+  }
+};
+
+}  // namespace
+
+P4A_API int p4a_snapshot_size(p4a_sim* s, size_t* bytes) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!bytes) return P4A_ERR_ARG;
+  *bytes = snap_bytes((size_t)s->n, (size_t)s->nCells, (size_t)s->nBodies, s->bposCount);
+  return P4A_OK;
+}
+
+P4A_API int p4a_snapshot_save(p4a_sim* s, void* dst, size_t bytes) {
+  int rc = check(s);
+  if (rc) return rc;
+  const size_t n = (size_t)s->n, nc = (size_t)s->nCells, nB = (size_t)s->nBodies;
+  const size_t need = snap_bytes(n, nc, nB, s->bposCount);
+  if (!dst) return P4A_ERR_ARG;
+  if (bytes < need) return fail(s, P4A_ERR_RANGE, "snapshot buffer too small");
+  HIP_TRY(s, hipSetDevice(s->device));
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  SnapHeader H;
+  std::memset(&H, 0, sizeof(H));
+  std::memcpy(H.magic, kSnapMagic, 8);
+  H.version = kSnapVersion;
+  H.headerBytes = (uint32_t)sizeof(SnapHeader);
+  H.paramsBytes = (uint32_t)sizeof(p4a_params);
+  H.totalBytes = need;
+  H.params = s->params;
+  H.h = s->h; H.d = s->d; H.mass = s->mass; H.denomRest = s->denomRest;
+  H.simTime = s->simTime; H.timeBank = s->timeBank; H.lastAdvanced = s->lastAdvanced;
+  for (int k = 0; k < 3; k++) H.heldTarget[k] = s->heldTarget[k];
+  H.heldRate = s->heldRate; H.heldLimit = s->heldLimit;
+  H.n = s->n; H.nFluid = s->nFluid; H.cap = s->cap; H.pourBudget = s->pourBudget;
+  H.nBodies = s->nBodies; H.nBoundary = (int32_t)s->bposCount; H.nCells = s->nCells;
+  H.heldBody = s->heldBody; H.lastSubsteps = s->lastSubsteps;
+  H.tile1 = s->tile1.variant; H.tile2 = s->tile2.variant;
+  Cursor c;
+  c.w = (unsigned char*)dst;
+  c.put(&H, sizeof(H));
+  const int par = s->parity;
+  auto d2h = [&](const void* src, size_t b) -> hipError_t {
+    const hipError_t e = b ? hipMemcpy(c.w, src, b, hipMemcpyDeviceToHost) : hipSuccess;
+    c.w += b;
+    return e;
+  };
+  HIP_TRY(s, d2h(s->A.pos[par], n * 16));
+  HIP_TRY(s, d2h(s->A.vel[par], n * 16));
+  HIP_TRY(s, d2h(s->A.pid[par], n * 4));
+  HIP_TRY(s, d2h(s->A.density, n * 4));
+  c.put(s->bodyOfPidH.data(), n * 4);
+  c.put(s->restOfPidH.data(), n * 16);
+  HIP_TRY(s, d2h(s->A.cellStart, (nc + 2) * 4));
+  HIP_TRY(s, d2h(s->A.bodyCentreD, nB * 32));
+  HIP_TRY(s, d2h(s->A.bodyRotD, nB * 72));
+  HIP_TRY(s, d2h(s->A.bodyCentre, nB * 16));
+  HIP_TRY(s, d2h(s->A.bodyRot, nB * 48));
+  HIP_TRY(s, d2h(s->A.bodyInvRatio, nB * 4));
+  HIP_TRY(s, d2h(s->A.bpos, s->bposCount * 16));
+  return P4A_OK;
+}
+
+P4A_API int p4a_snapshot_load(p4a_sim* s, const void* src, size_t bytes) {
+  if (!s || !src) return P4A_ERR_ARG;
+  if (bytes < sizeof(SnapHeader)) return fail(s, P4A_ERR_RANGE, "snapshot too short");
+  SnapHeader H;
+  std::memcpy(&H, src, sizeof(H));
+  if (std::memcmp(H.magic, kSnapMagic, 8) != 0) return fail(s, P4A_ERR_ARG, "not a snapshot");
+  if (H.version != kSnapVersion || H.headerBytes != sizeof(SnapHeader) ||
+      H.paramsBytes != sizeof(p4a_params))
+    return fail(s, P4A_ERR_ARG, "snapshot of another version or build");
+  if (H.n < 0 || H.cap < H.n || H.nBodies < 0 || H.nBoundary < 0 || H.nCells < 1 ||
+      H.nFluid < 0 || H.nFluid > H.n || H.heldBody > H.nBodies)
+    return fail(s, P4A_ERR_ARG, "snapshot header out of range");
+  const size_t n = (size_t)H.n, nc = (size_t)H.nCells, nB = (size_t)H.nBodies,
+               nBnd = (size_t)H.nBoundary;
+  if (H.totalBytes != snap_bytes(n, nc, nB, nBnd) || bytes < H.totalBytes)
+    return fail(s, P4A_ERR_RANGE, "snapshot size does not match its header");
+  if (H.tile1 < 0 || H.tile1 >= tile_variants(1) || H.tile2 < 0 || H.tile2 >= tile_variants(2))
+    return fail(s, P4A_ERR_ARG, "snapshot names an unknown traversal");
+  Cursor c;
+  c.r = (const unsigned char*)src + sizeof(H);
+  std::vector<float> pos(n * 4), vel(n * 4), density(n), rest(n * 4), bpos(nBnd * 4);
+  std::vector<uint32_t> pid(n), body(n), cellStart(nc + 2);
+  std::vector<double> cD(nB * 4), rD(nB * 9);
+  std::vector<float> cF(nB * 4), rF(nB * 12), inv(nB);
+  c.get(pos.data(), n * 16);
+  c.get(vel.data(), n * 16);
+  c.get(pid.data(), n * 4);
+  c.get(density.data(), n * 4);
+  c.get(body.data(), n * 4);
+  c.get(rest.data(), n * 16);
+  c.get(cellStart.data(), (nc + 2) * 4);
+  c.get(cD.data(), nB * 32);
+  c.get(rD.data(), nB * 72);
+  c.get(cF.data(), nB * 16);
+  c.get(rF.data(), nB * 48);
+  c.get(inv.data(), nB * 4);
+  c.get(bpos.data(), nBnd * 16);
+  // the live order must be a permutation of the particle ids
+  std::vector<uint32_t> slotOf((size_t)H.cap);
+  for (size_t k = 0; k < slotOf.size(); k++) slotOf[k] = (uint32_t)k;
+  {
+    std::vector<char> seen(n, 0);
+    for (size_t i = 0; i < n; i++) {
+      if (pid[i] >= n || seen[pid[i]]) return fail(s, P4A_ERR_ARG, "snapshot ids are not a permutation");
+      seen[pid[i]] = 1;
+      slotOf[pid[i]] = (uint32_t)i;
+    }
+    for (size_t k = 0; k < n; k++)
+      if (body[k] > (uint32_t)nB) return fail(s, P4A_ERR_ARG, "snapshot names a body that does not exist");
+    if (cellStart[nc] != (uint32_t)n) return fail(s, P4A_ERR_ARG, "snapshot grid does not hold n particles");
+  }
+  // a scene in particle-id order, so reset derives the per-id tables
+  std::vector<float> posP(n * 4), velP(n * 4);
+  std::vector<uint32_t> phaseP(n * 4, 0u);
+  for (size_t i = 0; i < n; i++) {
+    std::memcpy(&posP[(size_t)pid[i] * 4], &pos[i * 4], 16);
+    std::memcpy(&velP[(size_t)pid[i] * 4], &vel[i * 4], 16);
+  }
+  for (size_t k = 0; k < n; k++) phaseP[k * 4] = body[k];
+  std::vector<float> pts(nBnd * 3), psi(nBnd);
+  for (size_t k = 0; k < nBnd; k++) {
+    for (int a = 0; a < 3; a++) pts[k * 3 + a] = bpos[k * 4 + a];
+    psi[k] = bpos[k * 4 + 3];
+  }
+  std::vector<double> centre(nB * 3), dens(nB);
+  std::vector<int32_t> count(nB, 0), spec(nB, 0);
+  for (size_t b = 0; b < nB; b++) {
+    for (int a = 0; a < 3; a++) centre[b * 3 + a] = cD[b * 4 + a];
+    dens[b] = inv[b] > 0 ? 1.0 / inv[b] : 1.0;
+    spec[b] = (int32_t)b;
+  }
+  for (size_t k = 0; k < n; k++)
+    if (body[k]) count[body[k] - 1]++;
+  p4a_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  sc.n = H.n;
+  sc.n_fluid = H.nFluid;
+  sc.n_body = H.n - H.nFluid;
+  sc.pos = posP.data();
+  sc.vel = velP.data();
+  sc.rest = rest.data();
+  sc.phase = phaseP.data();
+  sc.n_boundary = H.nBoundary;
+  sc.boundary_pts = pts.data();
+  sc.boundary_psi = psi.data();
+  sc.n_bodies = H.nBodies;
+  sc.body_centre = centre.data();
+  sc.body_density = dens.data();
+  sc.body_count = count.data();
+  sc.body_spec = spec.data();
+  sc.mass = H.mass;
+  sc.h = H.h;
+  sc.d = H.d;
+  sc.denom_rest = H.denomRest;
+  p4a_params q = H.params;
+  q.pour_budget = H.cap - H.n;
+  int rc = p4a_reset(s, &q, &sc);
+  if (rc) return rc;
+  if (s->nCells != H.nCells) return fail(s, P4A_ERR_ARG, "snapshot grid does not match its box");
+  // the live state as it was saved (reset's grid build reordered the scene)
+  s->params = H.params;
+  s->pourBudget = H.pourBudget;
+  const int par = s->parity;
+  HIP_TRY(s, hipMemcpy(s->A.pos[par], pos.data(), n * 16, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.vel[par], vel.data(), n * 16, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.pred[par], pos.data(), n * 16, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.pid[par], pid.data(), n * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.slotOf, slotOf.data(), slotOf.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.density, density.data(), n * 4, hipMemcpyHostToDevice));
+  HIP_TRY(s, hipMemcpy(s->A.cellStart, cellStart.data(), (nc + 2) * 4, hipMemcpyHostToDevice));
+  if (nB) {
+    HIP_TRY(s, hipMemcpy(s->A.bodyCentreD, cD.data(), nB * 32, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(s->A.bodyRotD, rD.data(), nB * 72, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(s->A.bodyCentre, cF.data(), nB * 16, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(s->A.bodyRot, rF.data(), nB * 48, hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(s->A.bodyInvRatio, inv.data(), nB * 4, hipMemcpyHostToDevice));
+  }
+  s->predCur = s->A.pred[par];
+  s->simTime = H.simTime;
+  s->timeBank = H.timeBank;
+  s->lastAdvanced = H.lastAdvanced;
+  s->lastSubsteps = H.lastSubsteps;
+  for (int k = 0; k < 3; k++) s->heldTarget[k] = H.heldTarget[k];
+  s->heldRate = H.heldRate;
+  s->heldLimit = H.heldLimit;
+  s->heldBody = H.heldBody;
+  s->tile1.variant = H.tile1;
+  s->tile2.variant = H.tile2;
+  s->ofsValid = false;
+  s->statsValid = false;
+  drop_graphs(s);
+  return P4A_OK;
+}
+
 P4A_API int p4a_sync(p4a_sim* s) {
   if (!s) return P4A_ERR_ARG;
   HIP_TRY(s, hipStreamSynchronize(s->stream));

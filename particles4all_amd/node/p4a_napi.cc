@@ -546,6 +546,43 @@ napi_value FieldConfigure(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// snapshotSave(h) -> Uint8Array
+napi_value SnapshotSave(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 1);
+  p4a_sim* h = handle(env, a[0]);
+  size_t bytes = 0;
+  P4A_CALL(h, p4a_snapshot_size(h, &bytes), "p4a_snapshot_size");
+  void* data = nullptr;
+  napi_value ab, u8;
+  NAPI_OK(napi_create_arraybuffer(env, bytes, &data, &ab));
+  P4A_CALL(h, p4a_snapshot_save(h, data, bytes), "p4a_snapshot_save");
+  NAPI_OK(napi_create_typedarray(env, napi_uint8_array, bytes, ab, 0, &u8));
+  return u8;
+}
+
+// snapshotLoad(h, Uint8Array | Buffer)
+napi_value SnapshotLoad(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 2);
+  p4a_sim* h = handle(env, a[0]);
+  bool is = false;
+  napi_is_typedarray(env, a[1], &is);
+  if (!is) {
+    napi_throw_type_error(env, nullptr, "snapshotLoad: expected a Uint8Array");
+    return nullptr;
+  }
+  napi_typedarray_type t;
+  size_t len = 0, off = 0;
+  void* d = nullptr;
+  napi_value ab;
+  NAPI_OK(napi_get_typedarray_info(env, a[1], &t, &len, &d, &ab, &off));
+  if (t != napi_uint8_array) {
+    napi_throw_type_error(env, nullptr, "snapshotLoad: expected a Uint8Array");
+    return nullptr;
+  }
+  P4A_CALL(h, p4a_snapshot_load(h, d, len), "p4a_snapshot_load");
+  return nullptr;
+}
+
 // bodyPack(h, stat Float32Array | null, count)
 napi_value BodyPack(napi_env env, napi_callback_info info) {
   auto a = args(env, info, 3);
@@ -663,7 +700,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"fieldGenerate", FieldGenerate}, {"fieldInfo", FieldInfo},
              {"fieldSmooth", FieldSmooth}, {"meshNormalField", MeshNormalField},
              {"meshMarch", MeshMarch},   {"meshTriangles", MeshTriangles},
-             {"bodyPack", BodyPack}};
+             {"bodyPack", BodyPack},     {"snapshotSave", SnapshotSave},
+             {"snapshotLoad", SnapshotLoad}};
   for (auto& f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

@@ -53,6 +53,32 @@ class Sim {
     this.heldBody = -1;
   }
 
+  // The whole simulation state as a Uint8Array (p4a_snapshot_save): a sim
+  // loaded from it continues bit for bit.  params and the scene's body
+  // list are JS-side and are not part of it; keep them beside it.
+  snapshot() { return native.snapshotSave(this.handle); }
+
+  // Reset from snapshot bytes.  `params` and `scene` (the JS objects the
+  // saved sim was reset with) replace this.params / this.scene when given;
+  // step() needs params.
+  loadSnapshot(bytes, params, scene) {
+    native.snapshotLoad(this.handle, bytes);
+    if (params) this.params = params;
+    if (scene) {
+      this.scene = scene;
+      this.bodies = scene.bodies;
+    }
+    this._refresh();
+    this.bodyPose = [];
+    if (this.nBodies) {
+      const pose = native.bodyPose(this.handle, this.nBodies);
+      for (let i = 0; i < this.nBodies; i++)
+        this.bodyPose.push({ centre: Array.from(pose.centre.slice(i * 3, i * 3 + 3)),
+                             rot: Array.from(pose.rot.slice(i * 9, i * 9 + 9)) });
+    }
+    this.heldBody = -1;
+  }
+
   _refresh() {
     const i = native.info(this.handle);
     this.n = i.n;

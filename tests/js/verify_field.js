@@ -119,4 +119,15 @@ sim.scene.bodies.forEach((b, i) => {
 out.solidsCount = solids.count;
 out.packErr = packErr;
 out.packedBytes = solids.packed.bytes;
+// snapshot: a second Sim loaded from it continues bit for bit
+const snap = sim.snapshot();
+const sim2 = new Sim(0);
+sim2.loadSnapshot(snap, sim.params, sim.scene);
+for (let f = 0; f < 3; f++) { sim.step(1 / 60); sim2.step(1 / 60); }
+const pa = sim.read('pos'), pb = sim2.read('pos');
+let snapDiff = pa.length === pb.length ? 0 : -1;
+for (let k = 0; k < pa.length && snapDiff >= 0; k++) if (pa[k] !== pb[k]) snapDiff++;
+out.snapshotBytes = snap.length;
+out.snapshotDiff = snapDiff;
+out.snapshotBodyPose = JSON.stringify(sim.bodyPose) === JSON.stringify(sim2.bodyPose);
 console.log(JSON.stringify(out));

@@ -51,3 +51,5 @@ def test_js_surface_mesh_field():
     assert out["liveBodyPtrNonZero"] and out["liveBodyBytes"] > 0
     assert out["solidsCount"] == 3 and out["packedBytes"] == 3 * 96
     assert out["packErr"] == 0
+    # snapshot through JS: the loaded sim continues bit for bit
+    assert out["snapshotBytes"] > 0 and out["snapshotDiff"] == 0 and out["snapshotBodyPose"]

@@ -536,6 +536,8 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
           const uint32_t len = min(e - c0, 32u);
           const float4* P = op.pred + c0;
+          // m shifts left once per candidate and takes the test as its low
+          // bit (one add with carry): candidate u ends at bit len - 1 - u
           uint32_t m = 0u, k = 0u;
           for (; k + 3u < len; k += 4u) {
             float4 q[4];
@@ -544,21 +546,32 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               const float r2 = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z);
-              m |= (r2 < U.h2 ? 1u : 0u) << (k + (uint32_t)u);
+              m = (m + m) + (r2 < U.h2 ? 1u : 0u);
             }
           }
           for (; k < len; k++) {
             const float4 q = P[k];
             const float r2 = r2_of(t.px - q.x, t.py - q.y, t.pz - q.z);
-            m |= (r2 < U.h2 ? 1u : 0u) << k;
+            m = (m + m) + (r2 < U.h2 ? 1u : 0u);
           }
           if constexpr (PH == 1) {
-            while (m) {
-              const uint32_t bit = (uint32_t)__builtin_ctz(m);
-              m &= m - 1u;
-              float4 v[Op::NREC];
-              op.load(U, c0 + bit, v);
-              op.fpair(U, t, v);
+            // highest set bit first: candidates in their walk order; the next
+            // hit's records are loaded before the current hit's term runs
+            if (m) {
+              uint32_t top = 31u - (uint32_t)__builtin_clz(m);
+              m ^= 1u << top;
+              float4 cur[Op::NREC];
+              op.load(U, c0 + (len - 1u - top), cur);
+              while (m) {
+                top = 31u - (uint32_t)__builtin_clz(m);
+                m ^= 1u << top;
+                float4 nxt[Op::NREC];
+                op.load(U, c0 + (len - 1u - top), nxt);
+                op.fpair(U, t, cur);
+#pragma unroll
+                for (int r = 0; r < Op::NREC; r++) cur[r] = nxt[r];
+              }
+              op.fpair(U, t, cur);
             }
           } else {
             // four hits per trip with their loads in flight together; a
@@ -570,8 +583,9 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
 #pragma unroll
               for (int u = 0; u < 4; u++) {
                 hit[u] = m != 0u;
-                jj[u] = c0 + (hit[u] ? (uint32_t)__builtin_ctz(m) : 0u);
-                m &= m - 1u;
+                const uint32_t top = hit[u] ? 31u - (uint32_t)__builtin_clz(m) : 0u;
+                jj[u] = c0 + (hit[u] ? len - 1u - top : 0u);
+                m = hit[u] ? (m ^ (1u << top)) : 0u;
               }
               float4 v[4][Op::NREC];
 #pragma unroll
@@ -1132,10 +1146,10 @@ struct XsphFast : FastBase {
     t.ay = __fmaf_rn(f, v[1].y - t.vy, t.ay);
     t.az = __fmaf_rn(f, v[1].z - t.vz, t.az);
     if (NORMALS) {
-      const float ir = __builtin_amdgcn_rsqf(r2);
+      // r^2 clamped below at 1e-12: coincident pairs add g * 0 = 0
+      const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
       const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-      float g = hr * hr * ir * v[1].w;
-      g = (r2 > 1.0e-12f) ? g : 0.0f;
+      const float g = hr * hr * ir * v[1].w;
       t.nx = __fmaf_rn(g, rx, t.nx);
       t.ny = __fmaf_rn(g, ry, t.ny);
       t.nz = __fmaf_rn(g, rz, t.nz);

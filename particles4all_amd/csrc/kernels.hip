@@ -351,6 +351,11 @@ struct OpBase {
   __device__ Cell cell(const Uniform& U, const T& t) const { return cell_of(U, t.px, t.py, t.pz); }
   template <class T>
   __device__ bool skip_row(const Uniform&, const T&, int, int) const { return false; }
+  // The walks call this when the target's own record lay in none of the
+  // rows they visited (it moved more than a cell from where it was binned):
+  // a fast form that takes its self term inside the sum adds it here.
+  template <class T>
+  __device__ void self_outside(const Uniform&, T&) const {}
 };
 
 template <class Op, int TX, int NT, int MAXREC, int CR>
@@ -433,6 +438,7 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
     const int xlo = max(c.x - CR, 0), xhi = min(c.x + CR, gx - 1);
     const bool inTile = ldsOk && c.y == ty && c.z == tz && c.x >= x0 && c.x < x1;
     const bool doB = Op::BOUNDARY && U.nBoundary > 0 && (anyB || !inTile);
+    bool seen = false;  // the target's own record lay in a visited row
     for (int dz = -CR; dz <= CR; dz++) {
       const int z = c.z + dz;
       if (z < 0 || z >= gz) continue;
@@ -446,6 +452,7 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
           const int rel = S.off[r] - (int)S.base[r];
           const int b = (int)S.cs[r][xlo - xa] + rel;
           const int e = (int)S.cs[r][xhi + 1 - xa] + rel;
+          seen = seen || ((int)i + rel >= b && (int)i + rel < e);
           if constexpr (Op::FAST) {
             // branchless, four records in flight; the tail pads with
             // records that lie beyond the support and weigh nothing
@@ -479,6 +486,7 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
           }
         } else {
           const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+          seen = seen || (i >= b && i < e);
           for (uint32_t j = b; j < e; j++) {
             float4 v[Op::NREC];
             op.load(U, j, v);
@@ -491,6 +499,9 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
           for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
         }
       }
+    }
+    if constexpr (Op::FAST) {
+      if (!seen) op.self_outside(U, t);
     }
     op.finish(U, i, t);
   }
@@ -523,6 +534,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
   const bool nearWall = c.x <= CR || c.x >= gx - 1 - CR || c.y <= CR || c.y >= gy - 1 - CR ||
                         c.z <= CR || c.z >= gz - 1 - CR;
   const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
+  bool seen = false;  // the target's own record lay in a visited row
   for (int dz = -CR; dz <= CR; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
@@ -532,6 +544,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       if (op.skip_row(U, t, z, y)) continue;
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      seen = seen || (i >= b && i < e);
       if constexpr (Op::FAST && PH >= 1) {
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
           const uint32_t len = min(e - c0, 32u);
@@ -629,6 +642,9 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
       }
     }
+  }
+  if constexpr (Op::FAST) {
+    if (!seen) op.self_outside(U, t);
   }
   op.finish(U, i, t);
 }
@@ -1033,6 +1049,10 @@ struct LambdaFast : FastBase {
     t.by = __fmaf_rn(s, ry, t.by);
     t.bz = __fmaf_rn(s, rz, t.bz);
   }
+  // the self term W(0) the walk adds when it meets the target itself
+  __device__ void self_outside(const Uniform& U, Target& t) const {
+    t.st3 = __fmaf_rn(U.h2 * U.h2, U.h2, t.st3);
+  }
   __device__ void finish(const Uniform& U, uint32_t i, Target& t) const {
     const float rho = U.poly6 * __fmaf_rn(U.mass, t.st3, t.rb);
     density[i] = rho;
@@ -1297,6 +1317,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
                         c.z >= gz - 2;
   const bool doB = U.nBoundary > 0 && nearWall;
+  bool seen = false;  // the target's own record lay in a visited row
   for (int dz = -1; dz <= 1; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
@@ -1307,6 +1328,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
       const float ay = __fmaf_rn((float)(by - y), U.h, oy);
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
+      seen = seen || (i >= b && i < e);
       uint32_t j = b;
       for (; j + 3 < e; j += 4) {
         HRec r[4];
@@ -1332,6 +1354,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
       }
     }
   }
+  if (!seen) op.self_outside(U, t);
   op.finish16(U, i, t);
 }
 

@@ -1257,7 +1257,9 @@ P4A_API int p4a_aniso(p4a_sim* s, const p4a_aniso_params* a, int32_t* fp64_solve
   if (!(a->radius_scale > 0) || a->radius_scale > 2.0)
     return fail(s, P4A_ERR_ARG, "radius_scale must be in (0, 2]: the pass walks +-2 cells");
   HIP_TRY(s, hipSetDevice(s->device));
-  if (s->anisoCap < s->cap) {
+  // allocated on first use even at capacity 0: the fallback counter is
+  // cleared and read back whatever the particle count
+  if (!s->anisoFallbacks || s->anisoCap < s->cap) {
     free_list(s->anisoAllocs);
     if ((rc = dalloc(s, s->anisoAllocs, &s->smoothPos, (size_t)s->cap))) return rc;
     if ((rc = dalloc(s, s->anisoAllocs, &s->anisoG, (size_t)s->cap * 2))) return rc;
@@ -1364,7 +1366,7 @@ P4A_API int p4a_field_generate(p4a_sim* s, int32_t kind, int32_t skip_bodies) {
   if (kind == P4A_FIELD_ANISOTROPIC && !s->smoothPos)
     return fail(s, P4A_ERR_ARG, "the anisotropic field needs a p4a_aniso pass first");
   HIP_TRY(s, hipSetDevice(s->device));
-  if (s->fieldRecCap < s->cap) {
+  if (!s->fieldRec || s->fieldRecCap < s->cap) {
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     free_list(s->fieldRecAllocs);
     const size_t cap = (size_t)std::max(1, s->cap);

@@ -229,3 +229,114 @@ def test_outside_nan_and_face_positions_are_binned_as_the_oracle_bins_them():
     nc = int(np.prod(P.gridDim[:]))
     assert key.max() < nc
     sim.close()
+
+
+@pytest.mark.parametrize("math", [1, 0])
+def test_tiny_bodies(math):
+    """Bodies of one particle (zero covariance: the reference's |det| < 1e-12
+    falls back to the identity), a 2x2x2 cube and a radius-1 lattice sphere:
+    one substep bit-exact against the float chain in exact math (within the
+    FP64 bars in fast math), then finite poses with orthonormal rotations."""
+    from particles4all_amd._lib import SHAPE_LATTICE_CUBE, SHAPE_LATTICE_SPHERE, lattice_scene
+    p, _, _ = workloads.dam_break_32(scale=0.5, math=math, bodies=False)
+    bl = [(SHAPE_LATTICE_CUBE, 1, 0.5, (8.3, 20.2, 8.1)), (SHAPE_LATTICE_CUBE, 2, 1.5, (16.2, 22.4, 8.3)),
+          (SHAPE_LATTICE_SPHERE, 1, 0.8, (24.1, 21.7, 16.6))]
+    sc = lattice_scene(p, (16, 16, 16), jitter=0.05, seed=5, bodies=bl)
+    sim = Sim(0)
+    sim.reset(p, sc)
+    counts = list(sc.body_count)
+    assert sc.n_bodies == 3 and counts[0] == 1
+    dt = 1 / 120
+    P = O.constants_from(p, sc, dt)
+    st = O.State(rd(sim, "POS"), rd(sim, "VEL"), sim.read("BODY")[:, 0], rd(sim, "REST"))
+    pid0 = sim.read("PID")
+    inv = (1.0 / np.maximum(1e-6, sc.body_density)).astype(np.float32).astype(F)
+    bnd = (rd(sim, "BPOS"), rd(sim, "BPOS")[:, 3].copy(), sim.read("BCELL_START"))
+    rot = np.tile(np.eye(3).reshape(1, 9), (sc.n_bodies, 1))
+    info = O.substep(O.F32 if math == 1 else O.F64, P, st, bnd, np.array(sc.body_centre, F),
+                     rot, inv, p.iterations, p.surface_tension_k > 0)
+    sim.substeps(dt, 1)
+    assert np.array_equal(sim.read("PID"), pid0[info["order"]])
+    if math == 1:
+        assert np.array_equal(rd(sim, "POS"), st.pos)
+        assert np.array_equal(rd(sim, "VEL")[:, :3], st.vel[:, :3])
+    else:
+        assert np.abs(rd(sim, "POS")[:, :3] - st.pos[:, :3]).max() <= 1e-4 * sc.h
+    sim.substeps(dt, 20)
+    R = sim.read("BODY_ROT").reshape(-1, 3, 4)[:, :, :3].astype(F)
+    c = sim.read("BODY_CENTRE")
+    assert np.all(np.isfinite(R)) and np.all(np.isfinite(c)) and np.all(np.isfinite(rd(sim, "POS")))
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-5)
+    assert np.allclose(R[0], np.eye(3))  # the one-particle body never turns
+    sim.close()
+
+
+@pytest.mark.parametrize("math", [1, 0])
+def test_batch_with_empty_and_single_particle_scenes(math):
+    """A batch whose scenes include an empty one and a one-particle one
+    between ordinary scenes: every scene bit-identical to its own Sim."""
+    from particles4all_amd import Batch
+    items = workloads.batch_scenes(n_scenes=3, seed=7, n_min=300, n_max=2000, math=math)
+    p0, base0 = items[0]
+    empty = (p0, CustomScene(base0, np.zeros((0, 3))))
+    p1, base1 = items[1]
+    lone = (p1, CustomScene(base1, np.array([[5.3, 9.1, 6.2]])))
+    items = [items[0], empty, items[1], lone, items[2]]
+    dt = 1 / 120
+    b = Batch(items)
+    b.substeps(dt, 4)
+    for s, (p, sc) in enumerate(items):
+        sim = Sim(0)
+        sim.reset(p, sc)
+        sim.substeps(dt, 4)
+        assert b.read(s, "POS").shape == sim.read("POS").shape
+        for name in ("POS", "VEL", "PID", "DENSITY", "LAMBDA"):
+            a, g = b.read(s, name), sim.read(name)
+            assert np.array_equal(a.view(np.uint32), g.view(np.uint32)), (s, name)
+        sim.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_density_never_below_the_self_term(precision):
+    """Every density includes the particle's own W(0) and only non-negative
+    terms, so rho_i >= m W(0) after every lambda, also for targets far from
+    where they were binned (the exploding cluster), in the FP32 and the
+    FP16-offset forms."""
+    p, base, _ = workloads.dam_break_32(scale=0.5, math=0, bodies=False, precision=precision)
+    sc = CustomScene(base, cluster())
+    sim = Sim(0)
+    sim.reset(p, sc)
+    dt = 1 / 120
+    P = O.constants_from(p, sc, dt)
+    self_rho = sc.mass * P.poly6 * P.h2 ** 3
+    sim.run_stage("PREDICT", dt)
+    sim.run_stage("GRID", dt)
+    for it in range(p.iterations):
+        sim.run_stage("LAMBDA", dt)
+        rho = rd(sim, "DENSITY")
+        assert np.all(np.isfinite(rho))
+        print(f"precision {precision} iteration {it}: min rho / (m W(0)) = {rho.min() / self_rho:.6f}")
+        assert rho.min() >= self_rho * (1 - 1e-5)
+        sim.run_stage("DELTA", dt)
+    sim.run_stage("FINISH", dt)
+    assert np.all(np.isfinite(rd(sim, "POS")))
+    sim.close()
+
+
+def test_empty_scene_fields_and_mesh():
+    p, base, _ = base_scene(math=0)
+    sc = CustomScene(base, np.zeros((0, 3)))
+    sim = Sim(0)
+    sim.reset(p, sc)
+    sim.aniso()
+    sim.field_configure([0, 0, 0], list(p.box), 24)
+    for aniso in (False, True):
+        sim.field_generate(anisotropic=aniso)
+        assert np.all(sim.read("FIELD") == 0)
+    sim.mesh_normal_field(2)
+    sim.mesh_march(0.4)
+    verts, total = sim.mesh()
+    assert total == 0 and len(verts) == 0
+    assert list(sim.read("MESH_INDIRECT")) == [0, 1, 0, 0]
+    sim.close()

@@ -318,12 +318,12 @@ class Sim:
             pass
 
     def reset(self, params, scene):
-        self.params = params
         self._check(lib().p4a_reset(self.h, C.byref(params), C.byref(scene.view)), "reset")
+        self.params = params
 
     def set_params(self, params):
-        self.params = params
         self._check(lib().p4a_set_params(self.h, C.byref(params)), "set_params")
+        self.params = params
 
     def step(self, frame_dt):
         sub = C.c_int32()

@@ -245,10 +245,15 @@ P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params*
     M.c.h = sc.h;
     M.c.mass = sc.mass;
     M.c.denomRest = sc.denom_rest;
+    if (!(sc.h > 0)) return fail(b, P4A_ERR_ARG, "scene " + std::to_string(s) + ": h must be > 0");
     long cells = 1;
     for (int k = 0; k < 3; k++) {
-      M.c.gridDim[k] = std::max(1, (int)std::floor(p.box[k] / sc.h));
+      // max(1, floor(box / h)), kept a double until it is known to fit
+      const double g = std::floor(p.box[k] / sc.h);
+      if (!(g < 2.0e9)) return fail(b, P4A_ERR_RANGE, "batch too large");
+      M.c.gridDim[k] = g >= 1 ? (int)g : 1;
       cells *= M.c.gridDim[k];
+      if (cells > (long)INT32_MAX) return fail(b, P4A_ERR_RANGE, "batch too large");
     }
     M.c.nCells = (int)cells;
     M.c.n = sc.n;
@@ -342,7 +347,7 @@ P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params*
     for (int i = 0; i < sc.n; i++) {
       const uint32_t bid = sc.phase[(size_t)i * 4];
       if (bid == 0) continue;
-      if ((int)bid > sc.n_bodies)
+      if (bid > (uint32_t)sc.n_bodies)
         return fail(b, P4A_ERR_ARG, "phase names a body that does not exist");
       members[bid - 1].push_back(M.pBase + (uint32_t)i);
       bodyOf[M.pBase + i] = M.bodyBase + bid;
@@ -468,7 +473,10 @@ P4A_API int p4a_batch_read(p4a_batch* b, int32_t scene, int32_t id, void* dst, s
     case P4A_BUF_PID: src = b->A.pid[b->parity] + M.pBase; elem = 4; break;
     default: return fail(b, P4A_ERR_ARG, "buffer not readable per scene");
   }
-  if (offset + bytes > n * elem) return fail(b, P4A_ERR_RANGE, "read past the scene's slice");
+  if (offset > n * elem || bytes > n * elem - offset)
+    return fail(b, P4A_ERR_RANGE, "read past the scene's slice");
+  if (id == P4A_BUF_PID && (offset % 4 || bytes % 4))
+    return fail(b, P4A_ERR_ARG, "ids are read in whole words");
   BTRY(b, hipSetDevice(b->device));
   BTRY(b, hipMemcpyAsync(dst, (const char*)src + offset, bytes, hipMemcpyDeviceToHost,
                          b->stream));

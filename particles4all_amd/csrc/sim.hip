@@ -509,11 +509,23 @@ void sort_boundary(const p4a_sim* s, const float* pts, const float* psi, int nb,
   sort_boundary_grid(s->h, s->gridDim, s->nCells, pts, psi, nb, sorted, sortedPsi, start);
 }
 
+// Cells per axis max(1, floor(box / h)); -1 when the grid would not fit an
+// int32 cell index (NaN boxes included).
+long grid_cells(const double box[3], double h, int dim[3]) {
+  long cells = 1;
+  for (int k = 0; k < 3; k++) {
+    const double g = std::floor(box[k] / h);
+    if (!(g < 2.0e9)) return -1;
+    dim[k] = g >= 1 ? (int)g : 1;
+    cells *= dim[k];
+    if (cells > (long)INT32_MAX - 4) return -1;
+  }
+  return cells;
+}
+
 int set_grid(p4a_sim* s) {
-  for (int k = 0; k < 3; k++)
-    s->gridDim[k] = std::max(1, (int)std::floor(s->params.box[k] / s->h));
-  const long cells = (long)s->gridDim[0] * s->gridDim[1] * s->gridDim[2];
-  if (cells > (long)INT32_MAX - 4) return fail(s, P4A_ERR_RANGE, "grid too large");
+  const long cells = grid_cells(s->params.box, s->h, s->gridDim);
+  if (cells < 0) return fail(s, P4A_ERR_RANGE, "grid too large");
   s->nCells = (int)cells;
   return P4A_OK;
 }
@@ -707,6 +719,16 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
     return fail(s, P4A_ERR_ARG, "body arrays missing");
   if (!(sc->h > 0) || !(p->spacing > 0)) return fail(s, P4A_ERR_ARG, "h and spacing must be > 0");
   if (const char* fe = form_error(p)) return fail(s, P4A_ERR_ARG, fe);
+  {
+    // refused before the old state is released: a failed reset leaves it usable
+    int dim[3];
+    if (grid_cells(p->box, sc->h, dim) < 0) return fail(s, P4A_ERR_RANGE, "grid too large");
+    const long pour = p->pour_budget < 0 ? (long)sc->n_fluid : (long)p->pour_budget;
+    if ((long)sc->n + pour > (long)INT32_MAX / 16) return fail(s, P4A_ERR_RANGE, "capacity too large");
+    for (int i = 0; i < sc->n; i++)
+      if (sc->phase[(size_t)i * 4] > (uint32_t)sc->n_bodies)
+        return fail(s, P4A_ERR_ARG, "phase names a body that does not exist");
+  }
   HIP_TRY(s, hipSetDevice(s->device));
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   drop_graphs(s);
@@ -786,7 +808,7 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   for (int i = 0; i < n; i++) {
     const uint32_t b = sc->phase[(size_t)i * 4];
     if (b == 0) continue;
-    if ((int)b > nB) return fail(s, P4A_ERR_ARG, "phase names a body that does not exist");
+    if (b > (uint32_t)nB) return fail(s, P4A_ERR_ARG, "phase names a body that does not exist");
     members[b - 1].push_back((uint32_t)i);
     s->bodyOfPidH[i] = b;
     for (int k = 0; k < 3; k++) s->restOfPidH[(size_t)i * 4 + k] = sc->rest[(size_t)i * 4 + k];
@@ -923,14 +945,20 @@ P4A_API int p4a_step(p4a_sim* s, double frame_dt, int32_t* substeps_run) {
   // time bank (sim.js:415-429)
   const int base = std::max(1, s->params.substeps);
   const double dtTarget = (1.0 / 60.0) / base;
+  // `(timeBank || 0) + frameDt`: a bank left at NaN by a NaN frame restarts
+  // from zero on the next frame
+  if (s->timeBank == 0 || std::isnan(s->timeBank)) s->timeBank = 0;
   s->timeBank += frame_dt;
-  int sub = (int)std::floor(s->timeBank / dtTarget + 1e-4);
+  // the count stays a double until it is known to fit an int
+  const double q = std::floor(s->timeBank / dtTarget + 1e-4);
   const int workCap = base * 8;
-  if (sub > workCap) {
+  int sub = 0;
+  if (q > workCap) {
     sub = workCap;
     s->timeBank = 0;
   } else {
-    s->timeBank -= sub * dtTarget;
+    s->timeBank -= q * dtTarget;
+    if (q >= 1) sub = (int)q;
   }
   s->lastSubsteps = sub;
   s->lastAdvanced = sub * dtTarget;
@@ -998,7 +1026,6 @@ P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
   int rc = check(s);
   if (rc) return rc;
   if (!box) return P4A_ERR_ARG;
-  s->ofsValid = false;
   const double oldBox[3] = {s->params.box[0], s->params.box[1], s->params.box[2]};
   bool moved = false;
   for (int a = 0; a < 3; a++)
@@ -1006,6 +1033,12 @@ P4A_API int p4a_resize_box(p4a_sim* s, const double box[3]) {
   if (!moved) return P4A_OK;
   for (int a = 0; a < 3; a++)
     if (!(box[a] > 0)) return fail(s, P4A_ERR_ARG, "box sizes must be > 0");
+  {
+    // refused before anything changes: the sim keeps its old box and grid
+    int dim[3];
+    if (grid_cells(box, s->h, dim) < 0) return fail(s, P4A_ERR_RANGE, "grid too large");
+  }
+  s->ofsValid = false;
   HIP_TRY(s, hipSetDevice(s->device));
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   drop_graphs(s);
@@ -1158,7 +1191,6 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   int rc = check(s);
   if (rc) return rc;
   if (!src && bytes) return P4A_ERR_ARG;
-  s->ofsValid = false;
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
       id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");
@@ -1167,6 +1199,7 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   BufView v = buf_view(s, id);
   if (!v.ptr) return fail(s, P4A_ERR_ARG, "unknown buffer id");
   if (offset > v.bytes || bytes > v.bytes - offset) return fail(s, P4A_ERR_RANGE, "write past end");
+  s->ofsValid = false;
   if (bytes) HIP_TRY(s, hipMemcpy((char*)v.ptr + offset, src, bytes, hipMemcpyHostToDevice));
   drop_graphs(s);
   return P4A_OK;
@@ -1312,7 +1345,7 @@ P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const doubl
   int vd[3];
   for (int k = 0; k < 3; k++) {
     const double v = std::ceil(size[k] / voxel) + 2.0 + 1.0;
-    if (v > 65536.0) return fail(s, P4A_ERR_RANGE, "field too large");
+    if (!(v <= 65536.0)) return fail(s, P4A_ERR_RANGE, "field too large");  // NaN too
     vd[k] = (int)v;
     nv *= (size_t)vd[k];
   }
@@ -1321,6 +1354,7 @@ P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const doubl
   if (nv > s->fieldVertCap) {
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     free_list(s->fieldAllocs);
+    s->fieldConfigured = false;  // until every buffer is back
     s->field = s->field2 = s->nfield = nullptr;
     s->fieldAcc = nullptr;
     s->meshBlockCount = s->meshBlockSum = s->meshBlockStart = s->meshCounter = nullptr;
@@ -1342,6 +1376,7 @@ P4A_API int p4a_field_configure(p4a_sim* s, const double box_min[3], const doubl
   if ((size_t)max_triangles != s->meshVertCap) {
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     free_list(s->meshVertAllocs);
+    s->fieldConfigured = false;
     s->meshVerts = nullptr;
     s->meshVertCap = 0;
     if ((rc = dalloc(s, s->meshVertAllocs, &s->meshVerts, (size_t)max_triangles * 3))) return rc;
@@ -1618,8 +1653,15 @@ P4A_API int p4a_snapshot_load(p4a_sim* s, const void* src, size_t bytes) {
       H.paramsBytes != sizeof(p4a_params))
     return fail(s, P4A_ERR_ARG, "snapshot of another version or build");
   if (H.n < 0 || H.cap < H.n || H.nBodies < 0 || H.nBoundary < 0 || H.nCells < 1 ||
-      H.nFluid < 0 || H.nFluid > H.n || H.heldBody > H.nBodies)
+      H.nFluid < 0 || H.nFluid > H.n || H.heldBody > H.nBodies || H.cap > INT32_MAX / 16 ||
+      !(H.h > 0) || !(H.d > 0))
     return fail(s, P4A_ERR_ARG, "snapshot header out of range");
+  {
+    // the grid that reset will build from the saved box must be the saved one
+    int dim[3];
+    if (grid_cells(H.params.box, H.h, dim) != (long)H.nCells)
+      return fail(s, P4A_ERR_ARG, "snapshot grid does not match its box");
+  }
   const size_t n = (size_t)H.n, nc = (size_t)H.nCells, nB = (size_t)H.nBodies,
                nBnd = (size_t)H.nBoundary;
   if (H.totalBytes != snap_bytes(n, nc, nB, nBnd) || bytes < H.totalBytes)
@@ -1657,7 +1699,12 @@ P4A_API int p4a_snapshot_load(p4a_sim* s, const void* src, size_t bytes) {
     }
     for (size_t k = 0; k < n; k++)
       if (body[k] > (uint32_t)nB) return fail(s, P4A_ERR_ARG, "snapshot names a body that does not exist");
-    if (cellStart[nc] != (uint32_t)n) return fail(s, P4A_ERR_ARG, "snapshot grid does not hold n particles");
+    // the neighbour walks index particles through these row bounds
+    if (cellStart[0] != 0 || cellStart[nc] != (uint32_t)n)
+      return fail(s, P4A_ERR_ARG, "snapshot grid does not hold n particles");
+    for (size_t c2 = 0; c2 < nc; c2++)
+      if (cellStart[c2] > cellStart[c2 + 1])
+        return fail(s, P4A_ERR_ARG, "snapshot grid rows are not in order");
   }
   // a scene in particle-id order, so reset derives the per-id tables
   std::vector<float> posP(n * 4), velP(n * 4);
@@ -1706,7 +1753,7 @@ P4A_API int p4a_snapshot_load(p4a_sim* s, const void* src, size_t bytes) {
   q.pour_budget = H.cap - H.n;
   int rc = p4a_reset(s, &q, &sc);
   if (rc) return rc;
-  if (s->nCells != H.nCells) return fail(s, P4A_ERR_ARG, "snapshot grid does not match its box");
+  if (s->nCells != H.nCells) return fail(s, P4A_ERR_HIP, "snapshot grid changed in reset");
   // the live state as it was saved (reset's grid build reordered the scene)
   s->params = H.params;
   s->pourBudget = H.pourBudget;

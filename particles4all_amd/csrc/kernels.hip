@@ -21,7 +21,7 @@
 //   scatterPhase's body buffer (wgsl.js:460-475) -> k_export_phase (on request)
 //   bodyPackWGSL (solid_wgsl.js:3-26)    -> k_body_pack
 // Forms of this build without a reference kernel: the FP16-offset solver
-// iterations (k_direct16<Lambda16/Delta16>), the anisotropic field splat
+// iterations (k_direct16<Lambda16/Delta16T>), the anisotropic field splat
 // (k_field_aniso_prep + k_field_splat) and the batched-scene wrappers
 // (*_batch).
 //
@@ -1333,15 +1333,16 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
       for (; j + 3 < e; j += 4) {
         HRec r[4];
         float l[4];
+        // one 64-bit base per group, the others by immediate offsets
 #pragma unroll
-        for (int u = 0; u < 4; u++) op.load16(ofs, j + u, r[u], l[u]);
+        for (int u = 0; u < 4; u++) op.load16(ofs, (size_t)j + u, r[u], l[u]);
 #pragma unroll
         for (int u = 0; u < 4; u++) op.fpair16(U, t, ay, az, r[u], l[u]);
       }
       for (; j < e; j++) {
         HRec r;
         float l;
-        op.load16(ofs, j, r, l);
+        op.load16(ofs, (size_t)j, r, l);
         op.fpair16(U, t, ay, az, r, l);
       }
       if (doB) {
@@ -1360,7 +1361,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 
 // lambda over FP16-offset records
 struct Lambda16 : LambdaFast {
-  __device__ void load16(const HRec* ofs, uint32_t j, HRec& r, float& l) const {
+  __device__ void load16(const HRec* ofs, size_t j, HRec& r, float& l) const {
     r = ofs[j];
     l = 0.0f;
   }
@@ -1389,10 +1390,14 @@ struct Lambda16 : LambdaFast {
 // the moved FP32 position and its re-encoded record
 using DeltaFast = DeltaFastT<true>;
 
-struct Delta16 : DeltaFast {
+template <bool SOR>
+struct Delta16T : DeltaFastT<SOR> {
+  using Target = typename DeltaFastT<SOR>::Target;
+  using DeltaFastT<SOR>::lambda;
+  using DeltaFastT<SOR>::out;
   HRec* ofsOut;
   const uint32_t* cellOfSlot;
-  __device__ void load16(const HRec* ofs, uint32_t j, HRec& r, float& l) const {
+  __device__ void load16(const HRec* ofs, size_t j, HRec& r, float& l) const {
     r = ofs[j];
     l = lambda[j];
   }
@@ -1412,10 +1417,10 @@ struct Delta16 : DeltaFast {
     t.dx = __fmaf_rn(f, rx, t.dx);
     t.dy = __fmaf_rn(f, ry, t.dy);
     t.dz = __fmaf_rn(f, rz, t.dz);
-    if (U.sorAverage == 1u) t.nc += ok ? 1.0f : 0.0f;  // uniform: a scalar branch
+    if constexpr (SOR) t.nc += ok ? 1.0f : 0.0f;
   }
   __device__ void finish16(const Uniform& U, uint32_t i, Target& t) const {
-    DeltaFast::finish(U, i, t);
+    DeltaFastT<SOR>::finish(U, i, t);
     const float4 np = out[i];
     ofsOut[i] = encode_ofs(U, cellOfSlot[i], np.x, np.y, np.z);
   }
@@ -2505,13 +2510,17 @@ hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred
 hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
                           const Drift& d, hipStream_t s) {
-  Delta16 op;
-  op.pred = pred;
-  op.lambda = A.lambda;
-  op.out = out;
-  op.ofsOut = ofsOut;
-  op.cellOfSlot = A.cellOfSlot;
-  return run_direct16(U, A, drifted(op, d), ofs, threads, s);
+  // the neighbour count only when sorAverage divides by it (as the FP32 form)
+  auto go = [&](auto op) {
+    op.pred = pred;
+    op.lambda = A.lambda;
+    op.out = out;
+    op.ofsOut = ofsOut;
+    op.cellOfSlot = A.cellOfSlot;
+    return run_direct16(U, A, drifted(op, d), ofs, threads, s);
+  };
+  if (U.sorAverage == 1u) return go(Delta16T<true>{});
+  return go(Delta16T<false>{});
 }
 
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,

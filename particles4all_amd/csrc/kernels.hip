@@ -631,7 +631,16 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           op.fpair(U, t, v);
         }
       } else {
-        for (uint32_t j = b; j < e; j++) {
+        // exact ops: candidates strictly in walk order, four loads in flight
+        uint32_t j = b;
+        for (; j + 3u < e; j += 4u) {
+          float4 v[4][Op::NREC];
+#pragma unroll
+          for (int u = 0; u < 4; u++) op.load(U, (size_t)j + u, v[u]);
+#pragma unroll
+          for (int u = 0; u < 4; u++) op.pair(U, t, j + u, v[u]);
+        }
+        for (; j < e; j++) {
           float4 v[Op::NREC];
           op.load(U, j, v);
           op.pair(U, t, j, v);
@@ -2638,7 +2647,6 @@ hipError_t launch_census(const Uniform& U, const Arrays& A, const float4* pred,
 hipError_t launch_aniso(const Uniform& U, const Arrays& A, const float4* pos,
                         const AnisoArgs& a, float4* smoothPos, float4* g, uint32_t* fallbacks,
                         bool exact, hipStream_t s) {
-  // (2*2+1)^2 rows of 16+4 cells at ~8 particles per cell: 3584 records
   if (exact) {
     AnisoOp<true> op{{}, pos, smoothPos, g, fallbacks, a};
     return run_direct<AnisoOp<true>, 256, 2>(U, A, op, s);

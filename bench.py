@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--settle", type=int, default=600,
                     help="substeps of settling before the field measurement")
     ap.add_argument("--resolution", type=int, default=256)
+    ap.add_argument("--tiles", default="", metavar="G1,G2",
+                    help="fix the traversal shapes (variant numbers of groups 1 and 2) "
+                         "instead of autotuning, e.g. for counter runs that must profile "
+                         "the shapes a timed run picked")
     ap.add_argument("--batch", type=int, default=0, metavar="SCENES",
                     help="instead of the headline line: SCENES independent scenes batched "
                          "against the same scenes stepped one by one (one JSON line)")
@@ -347,7 +351,7 @@ def main():
                                    "(the multi-GPU slab decomposition is design-only)"}))
         sys.exit(2)
     from particles4all_amd import Sim, workloads
-    from particles4all_amd._lib import copy_probe
+    from particles4all_amd._lib import copy_probe, lib
 
     if args.field:
         field_bench(args)
@@ -362,6 +366,9 @@ def main():
                           "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
 
+    if args.tiles:
+        # read when a Sim is created
+        os.environ["P4A_TILE1"], os.environ["P4A_TILE2"] = args.tiles.split(",")
     t0 = time.time()
     p, sc, dt = workloads.WORKLOADS[args.workload](scale=args.scale)
     log(f"scene {args.workload}: n={sc.n} bodies={sc.n_bodies} built in {time.time() - t0:.1f}s")
@@ -375,9 +382,14 @@ def main():
     for _ in range(args.warmup):
         sim.step(frame)
     sim.sync()
-    chosen, tune_ms = sim.tune()
-    log("autotuner: " + ", ".join(f"{k} {v:.3f}ms" for k, v in tune_ms.items()) +
-        f" -> {chosen}")
+    if args.tiles:
+        chosen = [lib().p4a_tile_variant_name(g, int(v)).decode()
+                  for g, v in zip((1, 2), args.tiles.split(","))]
+        log(f"fixed shapes: {chosen}")
+    else:
+        chosen, tune_ms = sim.tune()
+        log("autotuner: " + ", ".join(f"{k} {v:.3f}ms" for k, v in tune_ms.items()) +
+            f" -> {chosen}")
     sim.step(frame)  # captures the graphs with the chosen shapes
     sim.sync()
     log("warmup done")

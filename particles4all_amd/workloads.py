@@ -2,8 +2,10 @@
 
   dam_break_32     32^3 jittered block in a 64^3 box, two floating 8^3 cubes
                    of density 0.5; frame 1/60 s, 2 substeps, 4 iterations.
-  block_1m         128^3 block filling half of a 256x128x128 box; substep
-                   1/120 s, 4 iterations, XSPH 0.01, surface tension 0.5.
+  block_1m         1,048,576 particles: a 128x64x128 block (x, y, z) at the
+                   corner of a 256x128x128 box, half its width and half its
+                   height; substep 1/120 s, 4 iterations, XSPH 0.01, surface
+                   tension 0.5.
   block_4m_bodies  256x128x128 block in a 512x192x128 box plus 64 bodies
                    (32 lattice cubes, edges 6-12; 32 lattice spheres, radii
                    4-8; densities log-uniform in [0.3, 2.0]) in the space
@@ -54,7 +56,7 @@ def dam_break_2m(scale=1.0, seed=5, **kw):
 def block_1m(scale=1.0, seed=2, **kw):
     L = max(4, int(round(128 * scale)))
     p = _params([2 * L, L, L], substeps=2, iterations=4, xsph=0.01, tension=0.5, **kw)
-    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed)
+    sc = lattice_scene(p, (L, max(2, L // 2), L), jitter=0.05, seed=seed)
     return p, sc, 1.0 / 120.0
 
 

@@ -5,8 +5,8 @@ the sizes the throughput figures are quoted on, against the oracle on the
 same inputs (the C oracle is OpenMP-parallel: each comparison takes
 seconds):
 
-* block_1m (128^3 = 2,097,152 jittered particles in a 256x128x128 box,
-  tension on), exact math: one whole substep (graph replay) bit-identical
+* block_1m (1,048,576 jittered particles, a 128x64x128 block in a
+  256x128x128 box, tension on), exact math: one whole substep (graph replay) bit-identical
   to the oracle's float chain (stable permutation, positions, velocities,
   lambdas, densities); the neighbour census bit-exact on the binned and on
   the moved (stale-grid) state;
@@ -64,7 +64,7 @@ def substep_inputs(sim, p, sc, dt):
 
 def test_block_1m_substep_and_census_bit_exact():
     p, sc, dt = workloads.block_1m(math=1)
-    assert sc.n == 128 ** 3 and p.surface_tension_k > 0
+    assert sc.n == 1_048_576 and p.surface_tension_k > 0
     sim = Sim(0)
     sim.reset(p, sc)
     P, st, bnd, inv, rot, centre = substep_inputs(sim, p, sc, dt)

@@ -95,7 +95,9 @@ def test_lattice_workloads_closed_forms():
         assert sel.sum() == 512
         assert np.abs(sc.rest[sel, :3].mean(axis=0)).max() < 1e-6
     p, sc, dt = workloads.block_1m(scale=0.125)
-    assert sc.n == 16 ** 3 and len(sc.boundary_psi) == 33 * 17 * 17 - 31 * 15 * 15
+    assert sc.n == 16 * 8 * 16 and len(sc.boundary_psi) == 33 * 17 * 17 - 31 * 15 * 15
+    p, sc, dt = workloads.block_1m()
+    assert sc.n == 1_048_576 and list(p.box) == [256.0, 128.0, 128.0]
 
 
 def _node(script):

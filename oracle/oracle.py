@@ -294,6 +294,88 @@ def body_pack(centre, rot, stat):
     return out.reshape(nb * 6, 4)
 
 
+def _len3(x, y, z):
+    """WGSL length() as the kernels evaluate it: sqrt((x*x + y*y) + z*z) in
+    float32, one rounding per operation."""
+    return np.sqrt((x * x + y * y) + z * z)
+
+
+def impulse(pos, vel, origin, direction, imp, radius, speed_limit):
+    """impulseWGSL (wgsl.js:216-248) with Sim.applyRayImpulse's host side
+    (sim.js:658-669): nothing happens for radius <= 0 or a zero impulse; the
+    direction is divided by Math.hypot (|| 1) in double; every value then
+    lands in a float32 uniform and the kernel works in float32.  Returns the
+    new live velocities (float32, n x 4)."""
+    f32 = np.float32
+    v = np.array(vel, np.float32, copy=True).reshape(-1, 4)
+    if not (len(v) and not (radius <= 0) and not (math.hypot(*imp) <= 0)):
+        return v
+    dl = math.hypot(*direction) or 1.0
+    o = [f32(c) for c in origin]
+    d = [f32(c / dl) for c in direction]
+    J = [f32(c) for c in imp]
+    R, L = f32(radius), f32(speed_limit)
+    p = np.asarray(pos, np.float32).reshape(-1, 4)
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        tx, ty, tz = p[:, 0] - o[0], p[:, 1] - o[1], p[:, 2] - o[2]
+        along = (tx * d[0] + ty * d[1]) + tz * d[2]
+        ax, ay, az = tx - along * d[0], ty - along * d[1], tz - along * d[2]
+        d2 = (ax * ax + ay * ay) + az * az
+        hit = ~(along <= 0) & ~(d2 >= R * R)
+        x = f32(1.0) - np.sqrt(d2) / R
+        old = _len3(v[:, 0], v[:, 1], v[:, 2])
+        w = x * x
+        n = [v[:, k] + w * J[k] for k in range(3)]
+        ns = _len3(*n)
+        allowed = np.maximum(old, L)
+        over = ns > allowed
+        sc = allowed / ns
+        n = [np.where(over, c * sc, c) for c in n]
+    for k in range(3):
+        v[hit, k] = n[k][hit]
+    v[hit, 3] = 0.0
+    return v
+
+
+def body_drag(vel, slot_of_members, centre, goal, rate, limit, dt):
+    """bodyDragWGSL (wgsl.js:187-214) on the held body's particles (slots
+    `slot_of_members`) with Sim.step's uniform (sim.js:444-455): blend =
+    1 - exp(-max(dt, 0)/0.02) in double, stored as float32; the wanted
+    velocity (goal - centre) * rate, scaled to `limit` when longer."""
+    f32 = np.float32
+    v = np.array(vel, np.float32, copy=True).reshape(-1, 4)
+    blend = f32(1.0 - math.exp(-max(dt, 0.0) / 0.02))
+    c = np.asarray(centre, np.float32).reshape(4)
+    w = [(f32(goal[k]) - c[k]) * f32(rate) for k in range(3)]
+    sp = _len3(*[np.float32(x) for x in w])
+    if sp > f32(limit):
+        s = f32(limit) / sp
+        w = [x * s for x in w]
+    m = np.asarray(slot_of_members, np.int64)
+    for k in range(3):
+        v[m, k] = v[m, k] + (f32(w[k]) - v[m, k]) * blend
+    return v
+
+
+def resize(pos, old_box, new_box, spacing):
+    """resizeWGSL (wgsl.js:125-157) with Sim.resizeBox's uniform (sim.js:
+    742-756): on an axis that shrinks, p = newMin + clamp((p - oldMin) /
+    oldSize, 0, 1) * newSize; then every axis is clamped to [d/2, box -
+    d/2] (d/2 and box - d/2 formed in double, stored as float32).  Returns
+    the new live positions (float32, n x 4)."""
+    f32 = np.float32
+    p = np.array(pos, np.float32, copy=True).reshape(-1, 4)
+    half = 0.5 * spacing
+    for a in range(3):
+        os_, ns_ = f32(old_box[a]), f32(new_box[a])
+        x = p[:, a]
+        if ns_ < os_:
+            t = np.minimum(np.maximum((x - f32(0.0)) / os_, f32(0.0)), f32(1.0))
+            x = f32(0.0) + t * ns_
+        p[:, a] = np.minimum(np.maximum(x, f32(half)), f32(new_box[a] - half))
+    return p
+
+
 def ssfr_kernel_g(g):
     """The SSFR splat's reading of the packed G (ssfr_wgsl.js:21-28): the
     symmetric matrix of each particle's two float4."""

@@ -1121,9 +1121,10 @@ P4A_API int p4a_apply_ray_impulse(p4a_sim* s, const double origin[3], const doub
   if (rc) return rc;
   if (!origin || !dir || !impulse) return P4A_ERR_ARG;
   if (!s->n || radius <= 0) return P4A_OK;
-  const double len = std::sqrt(impulse[0] * impulse[0] + impulse[1] * impulse[1] + impulse[2] * impulse[2]);
+  // Math.hypot, as the reference (no overflow or underflow of the squares)
+  const double len = std::hypot(impulse[0], impulse[1], impulse[2]);
   if (len <= 0) return P4A_OK;
-  double dl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  double dl = std::hypot(dir[0], dir[1], dir[2]);
   if (!(dl != 0)) dl = 1;  // `|| 1`
   RayArgs R;
   for (int k = 0; k < 3; k++) {

@@ -448,3 +448,57 @@ def test_ssfr_kernel_g_layout():
     g[:, 0] = np.stack([G[:, 0, 0], G[:, 1, 0], G[:, 2, 0], G[:, 1, 1]], -1)
     g[:, 1] = np.stack([G[:, 2, 1], G[:, 2, 2], np.linalg.det(G), np.full(5, 30.0)], -1)
     assert np.allclose(O.ssfr_kernel_g(g), G)
+
+
+def test_impulse_closed_forms():
+    """impulseWGSL restated: a particle on the ray gets the full impulse (x =
+    1), one at radial distance r gets (1 - r/R)^2 of it; particles behind the
+    origin or at r >= R keep their velocity (w too); a capped velocity keeps
+    its direction with length max(|v|, limit); hit particles get w = 0."""
+    pos = np.float32([[5, 1, 1, 1], [5, 1, 2.5, 1], [-1, 1, 1, 1], [5, 3, 1, 1],
+                      [7, 1, 1, 1]])
+    vel = np.zeros((5, 4), np.float32)
+    vel[:, 3] = 7
+    vel[4] = [0, 2, 0, 7]
+    out = O.impulse(pos, vel, [0, 1, 1], [4, 0, 0], [0.5, 0, 0], 2.0, 10.0)
+    assert np.array_equal(out[0], np.float32([0.5, 0, 0, 0]))
+    assert np.array_equal(out[1], np.float32([0.5 * 0.25 ** 2, 0, 0, 0]))
+    assert np.array_equal(out[2], vel[2]) and np.array_equal(out[3], vel[3])
+    assert np.array_equal(out[4], np.float32([0.5, 2, 0, 0]))
+    # cap: |v + J| = |(30, 2, 0)| > max(|v| = 2, limit 3) -> length 3, same direction
+    out = O.impulse(pos, vel, [0, 1, 1], [1, 0, 0], [30, 0, 0], 2.0, 3.0)
+    n = np.linalg.norm(out[4, :3].astype(np.float64))
+    assert abs(n - 3.0) < 1e-6
+    assert abs(out[4, 0] / out[4, 1] - 15.0) < 1e-5
+    # no-ops: radius <= 0, zero impulse
+    assert np.array_equal(O.impulse(pos, vel, [0, 1, 1], [1, 0, 0], [1, 0, 0], 0.0, 1.0), vel)
+    assert np.array_equal(O.impulse(pos, vel, [0, 1, 1], [1, 0, 0], [0, 0, 0], 2.0, 1.0), vel)
+
+
+def test_body_drag_closed_forms():
+    """bodyDragWGSL restated: from rest a member moves to blend * w with
+    w = (goal - centre) * rate, or w scaled to `limit` when longer;
+    blend = 1 - exp(-dt/0.02); other particles are untouched."""
+    vel = np.zeros((4, 4), np.float32)
+    vel[:, 3] = 5
+    centre = np.float32([1, 2, 3, 0])
+    dt = 1 / 120
+    blend = np.float32(1 - math.exp(-dt / 0.02))
+    out = O.body_drag(vel, [1, 3], centre, [2, 2, 3], 4.0, 100.0, dt)
+    assert np.allclose(out[1, :3], [4 * blend, 0, 0], rtol=1e-6, atol=0)
+    assert np.array_equal(out[0], vel[0]) and np.array_equal(out[2], vel[2])
+    assert out[1, 3] == 5 and out[3, 3] == 5
+    out = O.body_drag(vel, [1], centre, [1, 2, 13], 4.0, 2.0, dt)  # |w| = 40 > 2
+    assert np.allclose(out[1, :3], [0, 0, 2 * blend], rtol=1e-6, atol=0)
+
+
+def test_resize_closed_forms():
+    """resizeWGSL restated: a shrinking axis maps [0, old] onto [0, new]
+    (outside the old box clamps to its face), a growing axis keeps the
+    coordinate; then every axis is clamped to [d/2, box - d/2]."""
+    pos = np.float32([[8, 9.8, 1, 1], [12, 0.1, 1, 1], [2, 5, 0.2, 1]])
+    out = O.resize(pos, [10, 10, 10], [5, 20, 10], 1.0)
+    assert np.array_equal(out[0, :3], np.float32([4, 9.8, 1]))
+    assert np.array_equal(out[1, :3], np.float32([4.5, 0.5, 1]))
+    assert np.array_equal(out[2, :3], np.float32([1, 5, 0.5]))
+    assert np.array_equal(out[:, 3], pos[:, 3])

@@ -182,3 +182,30 @@ def test_append_bit_exact_and_the_next_substep():
     same(sim.read("POS"), st.pos)
     same(sim.read("VEL")[:, :3], st.vel[:, :3])
     sim.close()
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_stats_and_body_pose(math):
+    """stats (wgsl.js:785-832) after a frame against the oracle's reduction of
+    the read-back densities and velocities: the maxima exactly, the sums to
+    FP64 rounding (the reduction trees differ); the body poses in the
+    reference's readback layout (sim.js:635-655: rot column-major from the
+    stored rows)."""
+    p, sc, frame = workloads.dam_break_32(scale=1.0, math=math)
+    sim = Sim(0)
+    sim.reset(p, sc)
+    for _ in range(3):
+        sim.step(frame)
+    got = sim.stats()
+    want = O.stats(rd(sim, "DENSITY"), rd(sim, "VEL"), 1.0 / p.rest_density, sc.mass)
+    print(f"stats: {got}")
+    assert got[1] == want[1] and got[2] == want[2]
+    np.testing.assert_allclose([got[0], got[3]], [want[0], want[3]], rtol=1e-12, atol=0)
+    assert 0.5 < got[0] < 1.5 and got[3] > 0
+    c, r = sim.body_pose()
+    centre, rows = sim.read("BODY_CENTRE"), sim.read("BODY_ROT").reshape(-1, 3, 4)
+    assert np.array_equal(c, centre[:, :3])
+    assert np.array_equal(r.reshape(-1, 3, 3), rows[:, :, :3].transpose(0, 2, 1))
+    R = r.reshape(-1, 3, 3).astype(F)
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-5)
+    sim.close()

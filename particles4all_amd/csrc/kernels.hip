@@ -1035,11 +1035,12 @@ struct LambdaFast : FastBase {
     const float r2 = r2_of(rx, ry, rz);
     const float tt = fmaxf(U.h2 - r2, 0.0f);
     t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
-    // r^2 clamped below at 1e-12: the self pair and coincident particles
-    // (r = 0) contribute s * 0 = 0 without a select
-    const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
+    // the gradient only for r^2 > 1e-12 (wgsl.js:512): the self pair,
+    // coincident and near-coincident pairs add nothing (a select, so the
+    // infinite rsq of r = 0 never reaches the sums)
+    const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-    const float s = hr * hr * ir;
+    const float s = r2 > 1.0e-12f ? hr * hr * ir : 0.0f;
     t.gx = __fmaf_rn(s, rx, t.gx);
     t.gy = __fmaf_rn(s, ry, t.gy);
     t.gz = __fmaf_rn(s, rz, t.gz);
@@ -1101,7 +1102,7 @@ struct DeltaFastT : FastBase {
   __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
     const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
     const float r2 = r2_of(rx, ry, rz);
-    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    const bool ok = r2 < U.h2 && r2 >= 1.0e-12f;  // [1e-12, h^2)
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
     const float tt = U.h2 - r2;
@@ -1175,10 +1176,10 @@ struct XsphFast : FastBase {
     t.ay = __fmaf_rn(f, v[1].y - t.vy, t.ay);
     t.az = __fmaf_rn(f, v[1].z - t.vz, t.az);
     if (NORMALS) {
-      // r^2 clamped below at 1e-12: coincident pairs add g * 0 = 0
-      const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
+      // the normal only for r^2 >= 1e-12 (wgsl.js:278), by a select
+      const float ir = __builtin_amdgcn_rsqf(r2);
       const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-      const float g = hr * hr * ir * v[1].w;
+      const float g = r2 >= 1.0e-12f ? hr * hr * ir * v[1].w : 0.0f;
       t.nx = __fmaf_rn(g, rx, t.nx);
       t.ny = __fmaf_rn(g, ry, t.ny);
       t.nz = __fmaf_rn(g, rz, t.nz);
@@ -1226,7 +1227,7 @@ struct TensionFast : FastBase {
   __device__ void fpair(const Uniform& U, Target& t, const float4* v) const {
     const float rx = t.px - v[0].x, ry = t.py - v[0].y, rz = t.pz - v[0].z;
     const float r2 = r2_of(rx, ry, rz);
-    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    const bool ok = r2 < U.h2 && r2 >= 1.0e-12f;  // [1e-12, h^2)
     // outside the support ir = 0 keeps every term below finite
     const float ir = ok ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float r = r2 * ir;
@@ -1380,11 +1381,12 @@ struct Lambda16 : LambdaFast {
     const float r2 = r2_of(rx, ry, rz);
     const float tt = fmaxf(U.h2 - r2, 0.0f);
     t.st3 = __fmaf_rn(tt * tt, tt, t.st3);
-    // r^2 clamped below at 1e-12: the self pair and coincident particles
-    // (r = 0) contribute s * 0 = 0 without a select
-    const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-12f));
+    // the gradient only for r^2 > 1e-12 (wgsl.js:512): the self pair,
+    // coincident and near-coincident pairs add nothing (a select, so the
+    // infinite rsq of r = 0 never reaches the sums)
+    const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = fmaxf(__fmaf_rn(-r2, ir, U.h), 0.0f);
-    const float s = hr * hr * ir;
+    const float s = r2 > 1.0e-12f ? hr * hr * ir : 0.0f;
     t.gx = __fmaf_rn(s, rx, t.gx);
     t.gy = __fmaf_rn(s, ry, t.gy);
     t.gz = __fmaf_rn(s, rz, t.gz);
@@ -1414,7 +1416,7 @@ struct Delta16T : DeltaFastT<SOR> {
                           float lj) const {
     const float rx = t.px - o.x, ry = dh(ay, o.y, U.minusOne), rz = dh(az, o.z, U.minusOne);
     const float r2 = r2_of(rx, ry, rz);
-    const bool ok = r2 < U.h2 && r2 > 1.0e-12f;
+    const bool ok = r2 < U.h2 && r2 >= 1.0e-12f;  // [1e-12, h^2)
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
     const float tt = U.h2 - r2;

@@ -15,6 +15,11 @@
   targets lie more than a cell from where they were binned and their own
   records fall outside their stencils (the fast lambda then adds its self
   term W(0) itself), for the single- and two-phase traversals;
+* pairs a few float ulps apart near the box corner (0 < r^2 < 1e-12, below
+  the reference's cutoffs of the gradient, normal, delta and tension terms)
+  give the fast lambda, delta and finish stages within the fast bars of
+  FP64 in both forms and for both traversals (this found the fast gradient
+  and normal terms taking such pairs in);
 * positions outside the box, on its faces, huge and NaN are predicted and
   binned as the oracle predicts and bins them (clamped cells): cell ids,
   cell starts and the stable permutation bit-exact.
@@ -139,6 +144,71 @@ def cluster(n=200, centre=(16.2, 3.1, 15.9), radius=0.04, seed=3):
     pts[2, 0] += 2e-6        # and one a float ulp apart (r^2 just above 1e-12)
     pts[2, 1:] = pts[0, 1:]
     return pts
+
+
+def near_coincident(n=48, centre=(1.3, 1.2, 1.4), seed=5):
+    """A cluster near the box corner (coordinates near 1: float ulp 1.2e-7)
+    with eight pairs one to three ulps apart, r^2 ~ 1e-14."""
+    rng = np.random.default_rng(seed)
+    pts = (np.array(centre) + rng.uniform(-0.5, 0.5, size=(n, 3))).astype(np.float32)
+    for k in range(0, 16, 2):
+        q = pts[k].copy()
+        for a in range(1 + (k // 2) % 3):
+            q[a] = np.nextafter(q[a], np.float32(10))
+        pts[k + 1] = q
+    return pts.astype(F)
+
+
+@pytest.mark.parametrize("precision,variant", [(0, 2), (0, 6), (1, 2)])
+def test_near_coincident_pairs(precision, variant, monkeypatch):
+    monkeypatch.setenv("P4A_TILE1", str(variant))
+    monkeypatch.setenv("P4A_TILE2", str(variant))
+    dt = 1 / 120
+    p, base, _ = workloads.block_1m(scale=0.125, math=0, precision=precision)
+    assert p.surface_tension_k > 0
+    sc = CustomScene(base, near_coincident())
+    sim = Sim(0)
+    sim.reset(p, sc)
+    sim.run_stage("PREDICT", dt)
+    sim.run_stage("GRID", dt)
+    P = O.constants_from(p, sc, dt)
+    bnd = (rd(sim, "BPOS"), rd(sim, "BPOS")[:, 3].copy(), sim.read("BCELL_START"))
+    cs = sim.read("CELL_START")
+    pred = rd(sim, "PRED")
+    if precision:
+        pred = O.fp16_quantize(P, sim.read("PRED"), sim.read("CELL_OF_SLOT"))
+    d = pred[:, None, :3] - pred[None, :, :3]
+    r2 = (d ** 2).sum(-1)
+    tiny = np.count_nonzero((r2 > 0) & (r2 < 1e-12)) // 2
+    print(f"precision {precision}: {tiny} pairs with 0 < r^2 < 1e-12")
+    assert tiny >= 4
+    h = sc.h
+    for it in range(p.iterations):
+        pred = rd(sim, "PRED")
+        at = O.fp16_quantize(P, sim.read("PRED"), sim.read("CELL_OF_SLOT")) if precision else pred
+        rho64, lam64 = O.lambda_(O.F64, P, at, cs, *bnd)
+        sim.run_stage("LAMBDA", dt)
+        rho_err = np.abs(rd(sim, "DENSITY") - rho64).max() / rho64.max()
+        lam_err = np.abs(rd(sim, "LAMBDA") - lam64).max() / max(np.abs(lam64).max(), 1e-30)
+        print(f"iteration {it}: density {rho_err:.2e}, lambda {lam_err:.2e}")
+        assert rho_err <= (1e-3 if precision else 1e-5) and lam_err <= (1e-2 if precision else 1e-4)
+        if precision:
+            out64 = O.delta_at(O.F64, P, at, pred, rd(sim, "LAMBDA"), cs, *bnd)
+        else:
+            out64 = O.delta(O.F64, P, pred, rd(sim, "LAMBDA"), cs, *bnd)
+        sim.run_stage("DELTA", dt)
+        err = np.abs(rd(sim, "PRED")[:, :3] - out64[:, :3]).max()
+        assert err <= (1e-3 if precision else 1e-4) * h, err
+    pred, pos, den = rd(sim, "PRED"), rd(sim, "POS"), rd(sim, "DENSITY")
+    _, corr, normal = O.xsph(O.F64, P, pred, pos, den, cs, True)
+    corr = O.tension(O.F64, P, pred, den, normal, cs, corr)
+    sim.run_stage("FINISH", dt)
+    gn = rd(sim, "NORMAL")[:, :3]
+    n_err = np.abs(gn - normal[:, :3]).max() / np.abs(normal[:, :3]).max()
+    c_err = np.abs(rd(sim, "CORR")[:, :3] - corr[:, :3]).max() / max(np.abs(corr[:, :3]).max(), 1e-30)
+    print(f"finish: normal {n_err:.2e}, corr {c_err:.2e}")
+    assert n_err <= 1e-4 and c_err <= 1e-3
+    sim.close()
 
 
 @pytest.mark.parametrize("variant", [0, 2, 6])

@@ -305,6 +305,11 @@ def batch_bench(args):
     b = Batch(items)
     b.substeps(dt, args.warmup)
     b.sync()
+    chosen, tune_ms = b.tune(dt)
+    log("batch autotuner: " + ", ".join(f"{k} {v:.3f}ms" for k, v in tune_ms.items()) +
+        f" -> {chosen}")
+    b.substeps(dt, 1)  # captures the graphs with the chosen shapes
+    b.sync()
     t0 = time.perf_counter()
     b.substeps(dt, K)
     b.sync()
@@ -338,6 +343,7 @@ def batch_bench(args):
         "config": {"workload": f"batch_scenes: {args.batch} scenes, particle counts "
                                "log-uniform in [2e3, 2e5], boxes U[32,128] d, 0-4 cubes, I = 4",
                    "particles": n_total, "scenes": args.batch},
+        "tile_shapes": chosen,
         "sequential": {"value": n_total * K / t_seq, "ms_per_step": t_seq / K * 1e3},
         "speedup": t_seq / t_batch,
     }

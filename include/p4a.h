@@ -498,6 +498,13 @@ P4A_API int p4a_batch_destroy(p4a_batch* b);
 P4A_API const char* p4a_batch_last_error(p4a_batch* b);
 /* count substeps of dt for every scene, then the per-scene stats */
 P4A_API int p4a_batch_substeps(p4a_batch* b, double dt, int32_t count);
+/* Autotuner of the batch, as p4a_tune: times the traversal shapes that have
+ * a batched form (variant ids 0, 1, 2, 5, 6, 7, 8) on the batch's current
+ * state with substep dt and keeps the fastest; chosen[0..1] receive them,
+ * ms (capacity entries) the mean ms of each, group 1 first.  Every shape
+ * gives the same bits. */
+P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* ms,
+                           int32_t capacity);
 P4A_API int p4a_batch_sync(p4a_batch* b);
 P4A_API int p4a_batch_scene_size(p4a_batch* b, int32_t scene, int32_t* n);
 /* One scene's slice (live order) of P4A_BUF_POS, VEL, PRED, DENSITY, LAMBDA

@@ -104,7 +104,7 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_mesh_normal_field", "p4a_mesh_march", "p4a_mesh_triangles", "p4a_body_pack",
            "p4a_snapshot_size", "p4a_snapshot_save", "p4a_snapshot_load",
            "p4a_batch_create", "p4a_batch_destroy",
-           "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_sync",
+           "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_tune", "p4a_batch_sync",
            "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
 
 
@@ -158,6 +158,8 @@ def lib():
     L.p4a_batch_last_error.argtypes = [vp]
     L.p4a_batch_last_error.restype = C.c_char_p
     L.p4a_batch_substeps.argtypes = [vp, C.c_double, C.c_int32]
+    L.p4a_batch_tune.argtypes = [vp, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                 C.c_int32]
     L.p4a_batch_sync.argtypes = [vp]
     L.p4a_batch_scene_size.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
     L.p4a_batch_read.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_size_t, C.c_size_t]
@@ -571,6 +573,19 @@ class Batch:
 
     def substeps(self, dt, count):
         self._check(lib().p4a_batch_substeps(self.h, dt, int(count)), "batch_substeps")
+
+    def tune(self, dt):
+        """p4a_batch_tune: returns (chosen variant names, {name: ms})."""
+        L = lib()
+        ch = (C.c_int32 * 2)()
+        ms = (C.c_double * 16)()
+        self._check(L.p4a_batch_tune(self.h, dt, ch, ms, 16), "batch_tune")
+        times, k = {}, 0
+        for g in (1, 2):
+            for v in (0, 1, 2, 5, 6, 7, 8):
+                times[f"g{g}:" + L.p4a_tile_variant_name(g, v).decode()] = ms[k]
+                k += 1
+        return [L.p4a_tile_variant_name(g, ch[g - 1]).decode() for g in (1, 2)], times
 
     def sync(self):
         self._check(lib().p4a_batch_sync(self.h), "batch_sync")

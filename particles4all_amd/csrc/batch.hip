@@ -54,10 +54,16 @@ struct p4a_batch {
   int parity = 0;
   float4* predCur = nullptr;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // traversal shapes of the one-record (lambda, delta) and two-record
+  // (xsph, tension) neighbour kernels; p4a_batch_tune picks them
+  TileCfg tile1{2}, tile2{2};
   std::vector<void*> allocs;
 };
 
 namespace {
+
+// tile variants with a batched form (the LDS row tiles have none)
+constexpr int kBatchVariants[] = {0, 1, 2, 5, 6, 7, 8};
 
 int fail(p4a_batch* b, int code, const std::string& msg) {
   b->err = msg;
@@ -148,7 +154,7 @@ int run_grid(p4a_batch* b, bool moveVel) {
 int run_substep_body(p4a_batch* b) {
   const Uniform T = totals(b);
   const BatchTab* B = &b->tab;
-  const TileCfg t{2};  // direct walk, 64-thread workgroups
+  const TileCfg t = b->tile1, t2 = b->tile2;
   BLAUNCH(b, launch_predict_count(T, b->A, b->parity, b->stream, B));
   int rc = run_grid(b, false);
   if (rc) return rc;
@@ -168,10 +174,10 @@ int run_substep_body(p4a_batch* b) {
   }
   if (!b->exact)
     BLAUNCH(b, launch_prep_finish(T, b->A, b->predCur, b->A.pos[par], b->stream, B));
-  BLAUNCH(b, launch_xsph(T, b->A, b->predCur, b->A.pos[par], b->exact, b->tension, t, Drift{},
+  BLAUNCH(b, launch_xsph(T, b->A, b->predCur, b->A.pos[par], b->exact, b->tension, t2, Drift{},
                          b->stream, B));
   if (b->tension)
-    BLAUNCH(b, launch_tension(T, b->A, b->predCur, b->A.pos[par], b->A.vel[par], b->exact, t,
+    BLAUNCH(b, launch_tension(T, b->A, b->predCur, b->A.pos[par], b->A.vel[par], b->exact, t2,
                               Drift{}, b->stream, B));
   else
     BLAUNCH(b, launch_finalize(T, b->A, b->predCur, b->A.pos[par], b->A.vel[par], b->stream,
@@ -439,6 +445,62 @@ P4A_API int p4a_batch_substeps(p4a_batch* b, double dt, int32_t count) {
   if (count > 0) {
     const Uniform T = totals(b);
     BLAUNCH(b, launch_stats(T, b->A, b->A.vel[b->parity], b->stream, &b->tab));
+  }
+  return P4A_OK;
+}
+
+P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* ms,
+                           int32_t capacity) {
+  if (!b || !(dt >= 0)) return P4A_ERR_ARG;
+  BTRY(b, hipSetDevice(b->device));
+  BTRY(b, hipStreamSynchronize(b->stream));
+  int rc = upload_uniforms(b, dt);
+  if (rc) return rc;
+  const Uniform T = totals(b);
+  const BatchTab* B = &b->tab;
+  hipEvent_t e0, e1;
+  BTRY(b, hipEventCreate(&e0));
+  BTRY(b, hipEventCreate(&e1));
+  int k = 0;
+  for (int group = 1; group <= 2; group++) {
+    float best = 1e30f;
+    int bestV = 2;
+    for (int v : kBatchVariants) {
+      const TileCfg c{v};
+      auto run = [&]() -> hipError_t {
+        if (group == 1)
+          return launch_lambda(T, b->A, b->predCur, b->exact, c, Drift{}, b->stream, B);
+        return launch_xsph(T, b->A, b->predCur, b->A.pos[b->parity], b->exact, b->tension, c,
+                           Drift{}, b->stream, B);
+      };
+      BTRY(b, run());
+      BTRY(b, hipEventRecord(e0, b->stream));
+      for (int r = 0; r < 3; r++) BTRY(b, run());
+      BTRY(b, hipEventRecord(e1, b->stream));
+      BTRY(b, hipEventSynchronize(e1));
+      float t = 0;
+      BTRY(b, hipEventElapsedTime(&t, e0, e1));
+      t /= 3;
+      if (ms && k < capacity) ms[k] = t;
+      k++;
+      if (t < best) {
+        best = t;
+        bestV = v;
+      }
+    }
+    (group == 1 ? b->tile1 : b->tile2).variant = bestV;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // the next substep captures its graphs with the chosen shapes
+  for (int p = 0; p < 2; p++)
+    if (b->graph[p]) {
+      hipGraphExecDestroy(b->graph[p]);
+      b->graph[p] = nullptr;
+    }
+  if (chosen) {
+    chosen[0] = b->tile1.variant;
+    chosen[1] = b->tile2.variant;
   }
   return P4A_OK;
 }

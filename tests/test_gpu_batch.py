@@ -2,7 +2,8 @@
 ABI: after several substeps every scene's positions, velocities, particle
 ids, densities and lambdas are bit-identical to a separate Sim of that
 scene (same kernels, same per-scene arithmetic; only the slices of the
-concatenated arrays differ), in both math modes; per-scene stats agree to
+concatenated arrays differ), in both math modes, also after the batch's
+autotuner has changed the traversal shapes between substeps; per-scene stats agree to
 FP64 rounding (the reduction tree differs)."""
 import numpy as np
 import pytest
@@ -19,6 +20,8 @@ def test_batch_equals_separate_sims(math):
     dt = 1 / 120
     b = Batch(items)
     b.substeps(dt, 3)
+    chosen, times = b.tune(dt)  # every shape gives the same bits
+    assert len(times) == 14 and all(t > 0 for t in times.values())
     b.substeps(dt, 2)
     for s, (p, sc) in enumerate(items):
         sim = Sim(0)

@@ -97,15 +97,6 @@ __device__ __forceinline__ float poly6_w(const Uniform& U, float r2) {
   return U.poly6 * t * t * t;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
-// on XCD b % 8).  Hand each XCD a contiguous range of tiles so neighbouring
-// tiles, which share halo rows, share one L2.
-__device__ __forceinline__ int xcd_remap(int bid, int nb) {
-  int xcd = bid & 7, k = bid >> 3;
-  int per = nb >> 3, rem = nb & 7;
-  return xcd * per + min(xcd, rem) + k;
-}
-
 // Records the wave's largest per-axis displacement (p -> o) into a drift
 // slot: one wave-wide max over the active lanes, and an atomic only when it
 // raises the stored bound.  Non-negative floats order like their bits; a NaN

@@ -233,6 +233,15 @@ int scan_blocks(uint32_t n);
 hipError_t launch_exclusive_scan(uint32_t* count, uint32_t n, uint32_t* blockSum, uint32_t* out,
                                  hipStream_t s);
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs
+// on XCD b % 8).  Hand each XCD a contiguous range of tiles so neighbouring
+// tiles, which share halo rows, share one L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  int xcd = bid & 7, k = bid >> 3;
+  int per = nb >> 3, rem = nb & 7;
+  return xcd * per + min(xcd, rem) + k;
+}
+
 // Unsigned 32-bit division by an invariant divisor d >= 1 with one high
 // multiply (Granlund and Montgomery, "Division by invariant integers using
 // multiplication", PLDI 1994, figure 4.1): q = (t + ((n - t) >> s1)) >> s2,

@@ -470,8 +470,9 @@ P4A_API int p4a_timing(p4a_sim* h, char* names, size_t names_bytes, double* ms,
 P4A_API int p4a_copy_probe(int device, size_t bytes, int32_t reps, double* ms);
 
 /* Autotuner: times the neighbour kernels' tile shapes on the live scene
- * (lambda for the one-record kernels, xsph+normals for the two-record
- * ones) and keeps the fastest for later steps.  chosen[0..1] receive the
+ * (lambda for the one-record kernels; xsph+normals followed by tension for
+ * the two-record ones, as a substep runs them) and keeps the fastest for
+ * later steps.  chosen[0..1] receive the
  * variant ids; ms (capacity entries) the mean ms of every variant tried,
  * group 1 first.  The particle state is not changed. */
 P4A_API int p4a_tune(p4a_sim* h, int32_t chosen[2], double* ms, int32_t capacity);

@@ -458,6 +458,16 @@ P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* m
   if (rc) return rc;
   const Uniform T = totals(b);
   const BatchTab* B = &b->tab;
+  // group 2 as the substep runs it: xsph+normals, then tension, whose
+  // position and velocity outputs go to scratch so the scenes' state stays
+  float4 *sPos = nullptr, *sVel = nullptr;
+  if (b->tension && b->N > 0) {
+    if (hipMalloc(&sPos, (size_t)b->N * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&sVel, (size_t)b->N * sizeof(float4)) != hipSuccess) {
+      hipFree(sPos);
+      return fail(b, P4A_ERR_NOMEM, "tuner scratch");
+    }
+  }
   hipEvent_t e0, e1;
   BTRY(b, hipEventCreate(&e0));
   BTRY(b, hipEventCreate(&e1));
@@ -470,8 +480,11 @@ P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* m
       auto run = [&]() -> hipError_t {
         if (group == 1)
           return launch_lambda(T, b->A, b->predCur, b->exact, c, Drift{}, b->stream, B);
-        return launch_xsph(T, b->A, b->predCur, b->A.pos[b->parity], b->exact, b->tension, c,
-                           Drift{}, b->stream, B);
+        hipError_t e = launch_xsph(T, b->A, b->predCur, b->A.pos[b->parity], b->exact,
+                                   b->tension, c, Drift{}, b->stream, B);
+        if (e != hipSuccess || !sPos) return e;
+        return launch_tension(T, b->A, b->predCur, sPos, sVel, b->exact, c, Drift{}, b->stream,
+                              B);
       };
       BTRY(b, run());
       BTRY(b, hipEventRecord(e0, b->stream));
@@ -492,6 +505,9 @@ P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* m
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  BTRY(b, hipStreamSynchronize(b->stream));
+  hipFree(sPos);
+  hipFree(sVel);
   // the next substep captures its graphs with the chosen shapes
   for (int p = 0; p < 2; p++)
     if (b->graph[p]) {

@@ -338,6 +338,13 @@ struct TileLds {
 // no row is skipped.
 struct OpBase {
   uint32_t* driftOut = nullptr;  // drift slot a position update records into
+  // two-phase walks (3x3 rows): the first chunk's hit mask of each row, kept
+  // for a later kernel over the same positions and grid, or taken from an
+  // earlier one instead of testing the chunk again (row r of target i at
+  // [r * maskStride + i]); null: test every chunk
+  uint32_t* maskOut = nullptr;
+  const uint32_t* maskIn = nullptr;
+  uint32_t maskStride = 0;
   template <class T>
   __device__ Cell cell(const Uniform& U, const T& t) const { return cell_of(U, t.px, t.py, t.pz); }
   template <class T>
@@ -537,12 +544,17 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
       seen = seen || (i >= b && i < e);
       if constexpr (Op::FAST && PH >= 1) {
+        const size_t slot = (size_t)((dz + CR) * (2 * CR + 1) + (dy + CR)) * op.maskStride + i;
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
           const uint32_t len = min(e - c0, 32u);
           const float4* P = op.pred + c0;
           // m shifts left once per candidate and takes the test as its low
           // bit (one add with carry): candidate u ends at bit len - 1 - u
           uint32_t m = 0u, k = 0u;
+          if (op.maskIn && c0 == b) {
+            m = op.maskIn[slot];  // the same test, made by the kernel before
+            k = len;
+          }
           for (; k + 3u < len; k += 4u) {
             float4 q[4];
 #pragma unroll
@@ -558,6 +570,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
             const float r2 = r2_of(t.px - q.x, t.py - q.y, t.pz - q.z);
             m = (m + m) + (r2 < U.h2 ? 1u : 0u);
           }
+          if (op.maskOut && c0 == b) op.maskOut[slot] = m;
           if constexpr (PH == 1) {
             // highest set bit first: candidates in their walk order; the next
             // hit's records are loaded before the current hit's term runs
@@ -2542,6 +2555,13 @@ hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HR
   return hipGetLastError();
 }
 
+// xsph+normals and tension walk the same rows of the same positions: with
+// a two-phase direct shape (single scenes), tension takes xsph's first-chunk
+// masks instead of testing those candidates again.
+bool reuse_masks(const Arrays& A, const TileCfg& t, const BatchTab* B) {
+  return !B && A.hitMask && t.variant >= 5 && t.variant <= 8;
+}
+
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
                        hipStream_t s, const BatchTab* B) {
@@ -2549,7 +2569,14 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
   if (normals) {
     if (exact)
       return run_tile2(U, A, drifted(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
-    return run_tile2(U, A, drifted(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s, B);
+  {
+    XsphFast<true> op{{}, pred, A.velq, A.density, A.corr, A.normal};
+    if (reuse_masks(A, t, B)) {
+      op.maskOut = A.hitMask;  // for the tension kernel that follows
+      op.maskStride = A.hitStride;
+    }
+    return run_tile2(U, A, drifted(op, d), t, s, B);
+  }
   }
   if (exact)
     return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
@@ -2561,7 +2588,12 @@ hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred,
                           hipStream_t s, const BatchTab* B) {
   if (exact)
     return run_tile2(U, A, drifted(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, d), t, s, B);
-  return run_tile2(U, A, drifted(TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel}, d), t, s, B);
+  TensionFast op{{}, pred, pos, A.velq, A.normal, A.corr, vel};
+  if (reuse_masks(A, t, B)) {
+    op.maskIn = A.hitMask;  // written by xsph+normals over the same positions
+    op.maskStride = A.hitStride;
+  }
+  return run_tile2(U, A, drifted(op, d), t, s, B);
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,

@@ -83,6 +83,12 @@ struct Arrays {
   float4* corr;
   float4* normal;
   float4* velq;         // fast path: (pred - pos)/dt and m/max(rho, rho0/2)
+  // fast path with tension, two-phase walks: the r^2 < h^2 mask of the first
+  // 32-candidate chunk of each of a target's 9 stencil rows, written by xsph
+  // and read by tension (same positions, same grid); row r of particle i at
+  // hitMask[r * hitStride + i]
+  uint32_t* hitMask;
+  uint32_t hitStride;
   HRec* ofs[2];           // FP16 form: offsets of pred[parity] / predX
   uint32_t* drift;      // DRIFT_SLOTS displacement bounds (float bits)
   uint32_t* cellOfSlot; // cell each sorted slot was binned into

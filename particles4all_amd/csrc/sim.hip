@@ -792,6 +792,8 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   ALLOC(A.corr, cap);
   ALLOC(A.normal, cap);
   ALLOC(A.velq, cap);
+  ALLOC(A.hitMask, cap * 9);
+  A.hitStride = (uint32_t)cap;
   ALLOC(A.bodyOfPid, cap);
   ALLOC(A.restOfPid, cap);
   for (int k = 0; k < 3; k++) ALLOC(A.census[k], cap);
@@ -1822,6 +1824,18 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
   const double dt = (1.0 / 60.0) / std::max(1, s->params.substeps);
   const Uniform U = make_uniform(s, dt);
   const bool ex = exact_math(s);
+  const bool tension = s->params.surface_tension_k > 0;
+  // group 2 is timed as the substep runs it: xsph+normals, then tension
+  // (which in the two-phase shapes reuses xsph's hit masks); tension's
+  // position and velocity outputs go to scratch, so the live state stays
+  std::vector<void*> scratch;
+  float4 *sPos = nullptr, *sVel = nullptr;
+  if (tension && s->n > 0) {
+    if ((rc = dalloc(s, scratch, &sPos, (size_t)s->n)) || (rc = dalloc(s, scratch, &sVel, (size_t)s->n))) {
+      free_list(scratch);
+      return rc;
+    }
+  }
   hipEvent_t e0, e1;
   HIP_TRY(s, hipEventCreate(&e0));
   HIP_TRY(s, hipEventCreate(&e1));
@@ -1833,8 +1847,10 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
       TileCfg c{v};
       auto run = [&]() -> hipError_t {
         if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, Drift{}, s->stream);
-        return launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, true, c, Drift{},
-                           s->stream);
+        hipError_t e = launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, tension, c,
+                                   Drift{}, s->stream);
+        if (e != hipSuccess || !sPos) return e;
+        return launch_tension(U, s->A, s->predCur, sPos, sVel, ex, c, Drift{}, s->stream);
       };
       HIP_TRY(s, run());
       HIP_TRY(s, hipEventRecord(e0, s->stream));
@@ -1856,6 +1872,8 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  HIP_TRY(s, hipStreamSynchronize(s->stream));
+  free_list(scratch);
   drop_graphs(s);
   if (chosen) {
     chosen[0] = s->tile1.variant;

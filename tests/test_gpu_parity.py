@@ -358,6 +358,7 @@ def _fast_run(variant, monkeypatch):
     sim.run_stage("FINISH", dt)
     out["vel"] = rd(sim, "VEL")
     out["normal"] = rd(sim, "NORMAL")
+    out["corr"] = rd(sim, "CORR")
     sim.close()
     return out
 
@@ -367,7 +368,9 @@ def test_two_phase_walk_matches_single_phase_in_fast_math(variant, monkeypatch):
     """The two-phase walk (test r^2 < h^2, then the pair term for the hits)
     sums the same non-zero terms in the same order as the single-phase
     branchless walk; the terms it skips are the branchless form's zeros and
-    its ~1e-13 gradient residue at r just above h (hardware rsq)."""
+    its ~1e-13 gradient residue at r just above h (hardware rsq).  In these
+    shapes the tension kernel takes xsph's first-chunk masks instead of its
+    own test (corrections and velocities below)."""
     a = _fast_run(2, monkeypatch)
     b = _fast_run(variant, monkeypatch)
     for k in a:
@@ -405,3 +408,27 @@ def test_drift_slots_hold_the_largest_displacement():
     sim.run_stage("PREDICT", dt)
     assert not sim.read("DRIFT").any()
     sim.close()
+
+
+def test_tune_leaves_the_state_and_the_bits():
+    """p4a_tune times the shapes on the live scene (lambda; xsph+normals then
+    tension, whose outputs go to scratch): positions, velocities and ids are
+    unchanged, and in exact math the substeps after it equal those of a twin
+    that did not tune, bit for bit."""
+    p, sc, _ = workloads.block_1m(scale=0.125, math=1)
+    assert p.surface_tension_k > 0
+    a, b = Sim(0), Sim(0)
+    for s in (a, b):
+        s.reset(p, sc)
+        s.substeps(1 / 120, 2)
+    before = {k: b.read(k).copy() for k in ("POS", "VEL", "PID")}
+    chosen, times = b.tune()
+    print(f"tuned: {chosen}")
+    for k, v in before.items():
+        assert np.array_equal(b.read(k).view(np.uint32), v.view(np.uint32)), k
+    for s in (a, b):
+        s.substeps(1 / 120, 2)
+    for k in ("POS", "VEL", "PID", "DENSITY", "LAMBDA"):
+        assert np.array_equal(a.read(k).view(np.uint32), b.read(k).view(np.uint32)), k
+    a.close()
+    b.close()

@@ -71,11 +71,14 @@ __device__ __forceinline__ uint32_t cell_index(const Uniform& U, Cell c) {
 // position binned into that cell: x kept as float, y and z as half offsets
 // from the cell's centre boxMin + (index + 0.5) h, rounded to nearest even.
 __device__ __forceinline__ void cell_xyz(const Uniform& U, uint32_t c, int& x, int& y, int& z) {
+  // two divisions by invariant integers as multiplications (no integer
+  // divide instruction on the GPU)
   const uint32_t gx = (uint32_t)U.gridDim[0], gy = (uint32_t)U.gridDim[1];
-  x = (int)(c % gx);
-  const uint32_t yz = c / gx;
-  y = (int)(yz % gy);
-  z = (int)(yz / gy);
+  const uint32_t yz = U.divGx.div(c);
+  const uint32_t zz = U.divGy.div(yz);
+  x = (int)(c - yz * gx);
+  y = (int)(yz - zz * gy);
+  z = (int)zz;
 }
 __device__ __forceinline__ float cell_centre(const Uniform& U, int a, int k) {
   return U.boxMin[a] + ((float)k + 0.5f) * U.h;

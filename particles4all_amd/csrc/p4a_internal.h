@@ -21,12 +21,33 @@ struct __align__(8) HRec {
 // The reference's `Params` uniform (wgsl.js:4-40) with the values
 // Sim.uploadParams derives (sim.js:371-413).  Passed to every kernel by
 // value, so a captured graph carries its own copy.
+// Unsigned 32-bit division by an invariant divisor d >= 1 with one high
+// multiply (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, figure 4.1): q = (t + ((n - t) >> s1)) >> s2,
+// t = umulhi(m, n), exact for every 32-bit n.
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+  FastDiv() : d(1), m(1), s1(0), s2(0) {}
+  explicit FastDiv(uint32_t div) : d(div) {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < div) l++;
+    m = (uint32_t)((((1ull << 32) * ((1ull << l) - div)) / div) + 1);
+    s1 = l < 1 ? l : 1;
+    s2 = l > 1 ? l - 1 : 0;
+  }
+  __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    const uint32_t t = __umulhi(m, n);
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
 struct Uniform {
   float boxMin[3];
   float dt;
   float boxMax[3];
   float h;
   int gridDim[3];
+  FastDiv divGx, divGy;  // cells per row, rows per plane (flat cell id -> x, y, z)
   float h2;
   float clampMin[3];
   float poly6;
@@ -247,26 +268,6 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   int per = nb >> 3, rem = nb & 7;
   return xcd * per + min(xcd, rem) + k;
 }
-
-// Unsigned 32-bit division by an invariant divisor d >= 1 with one high
-// multiply (Granlund and Montgomery, "Division by invariant integers using
-// multiplication", PLDI 1994, figure 4.1): q = (t + ((n - t) >> s1)) >> s2,
-// t = umulhi(m, n), exact for every 32-bit n.
-struct FastDiv {
-  uint32_t d, m, s1, s2;
-  FastDiv() : d(1), m(1), s1(0), s2(0) {}
-  explicit FastDiv(uint32_t div) : d(div) {
-    uint32_t l = 0;
-    while (l < 32 && (1ull << l) < div) l++;
-    m = (uint32_t)((((1ull << 32) * ((1ull << l) - div)) / div) + 1);
-    s1 = l < 1 ? l : 1;
-    s2 = l > 1 ? l - 1 : 0;
-  }
-  __device__ __forceinline__ uint32_t div(uint32_t n) const {
-    const uint32_t t = __umulhi(m, n);
-    return (t + ((n - t) >> s1)) >> s2;
-  }
-};
 
 // Surface extraction on the field lattice (SurfaceMesh.smoothField,
 // buildNormalField, generate's march; mesh_wgsl.js:70-244).

@@ -53,6 +53,8 @@ Uniform make_uniform_from(const SceneConsts& s, double dt) {
     U.clampMin[k] = (float)(0.5 * d);
     U.clampMax[k] = (float)(p.box[k] - 0.5 * d);
   }
+  U.divGx = FastDiv((uint32_t)s.gridDim[0]);
+  U.divGy = FastDiv((uint32_t)s.gridDim[1]);
   U.dt = (float)dt;
   U.h = (float)h;
   U.h2 = (float)(h * h);

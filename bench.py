@@ -453,7 +453,12 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.workload, dom),
                      "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
-                     "mean_launch_ms": dom_ms, "copy_probe_GBps": copy_gbs},
+                     "mean_launch_ms": dom_ms, "copy_probe_GBps": copy_gbs,
+                     # the same model for every neighbour kernel (tension's figure
+                     # counts the row tests it takes from xsph's masks as gathered)
+                     "frac_by_kernel": {k: round(ALGO_BYTES[k] * n / (kt[k][0] * 1e-3) / 1e9
+                                                 / HBM_PEAK_GBS, 4)
+                                        for k in ALGO_BYTES if k in kt}},
         "kernels_ms": {k: round(v[0], 4) for k, v in kt.items()},
         "tile_shapes": chosen,
         "fp16_offset": fp16,

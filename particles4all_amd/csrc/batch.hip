@@ -132,6 +132,7 @@ Uniform totals(const p4a_batch* b) {
   U.nBodies = b->nBodies;
   U.nBodyParts = b->nBodyParts;
   U.minusOne = -1.0f;
+  U.slotIdCount = 0xffffffffu;  // every id's slot (bodies of all scenes, not one range)
   // the delta walk forms neighbour counts when any scene averages by them
   for (const auto& m : b->m)
     if (m.c.params.sor_average) U.sorAverage = 1u;

@@ -312,7 +312,9 @@ k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __rest
   predOut[d] = pred[i];
   const uint32_t q = pid[i];
   pidOut[d] = q;
-  slotOf[q] = d;
+  // a scattered 4-byte store per particle otherwise: kept for the ids the
+  // body kernels look up
+  if (q - U.slotIdLo < U.slotIdCount) slotOf[q] = d;
   if (vel) velOut[d] = vel[i];
   if (ofsOut) {
     const float4 p = pred[i];

@@ -48,6 +48,9 @@ struct Uniform {
   float h;
   int gridDim[3];
   FastDiv divGx, divGy;  // cells per row, rows per plane (flat cell id -> x, y, z)
+  // the sort records the slot of particle ids in [slotIdLo, slotIdLo +
+  // slotIdCount) only: the body particles, which the body kernels reach by id
+  uint32_t slotIdLo, slotIdCount;
   float h2;
   float clampMin[3];
   float poly6;

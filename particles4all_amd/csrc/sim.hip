@@ -139,6 +139,10 @@ struct p4a_sim {
   int gridDim[3] = {1, 1, 1};
   int nCells = 1;
   int n = 0, cap = 0, nFluid = 0, nBodyParts = 0, nBodies = 0, nBoundary = 0, pourBudget = 0;
+  // particle ids whose slot the sort keeps in slotOf: the body particles'
+  // range when their ids are contiguous (fluid first, then the bodies, as
+  // scenes are built), every id otherwise
+  uint32_t slotIdLo = 0, slotIdCount = 0xffffffffu;
   double bodyScale = 0, covScale = 0;
   // state
   int parity = 0, gen = 0, lastSubsteps = 0;
@@ -274,7 +278,10 @@ Uniform make_uniform(const p4a_sim* s, double dt) {
   c.nBoundary = s->nBoundary;
   c.nBodyParts = s->nBodyParts;
   c.nBodies = s->nBodies;
-  return make_uniform_from(c, dt);
+  Uniform U = make_uniform_from(c, dt);
+  U.slotIdLo = s->slotIdLo;
+  U.slotIdCount = s->slotIdCount;
+  return U;
 }
 
 hipEvent_t take_event(p4a_sim* s) {
@@ -819,6 +826,21 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
     nBodyParts++;
   }
   s->nBodyParts = nBodyParts;
+  {
+    uint32_t lo = 0xffffffffu, hi = 0;
+    for (int b = 0; b < nB; b++)
+      for (uint32_t q : members[b]) {
+        lo = std::min(lo, q);
+        hi = std::max(hi, q + 1);
+      }
+    s->slotIdLo = 0;
+    s->slotIdCount = 0xffffffffu;
+    if (nBodyParts == 0) s->slotIdCount = 0;
+    else if (hi - lo == (uint32_t)nBodyParts) {
+      s->slotIdLo = lo;
+      s->slotIdCount = hi - lo;
+    }
+  }
   std::vector<uint32_t> pids, first(nB + 1, 0);
   s->bodyCountH.assign(nB, 0);
   for (int b = 0; b < nB; b++) {

@@ -637,6 +637,21 @@ napi_value MeshTriangles(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// tune(h) -> {lambda: shape name, xsph: shape name} (p4a_tune)
+napi_value Tune(napi_env env, napi_callback_info info) {
+  auto a = args(env, info, 1);
+  p4a_sim* h = handle(env, a[0]);
+  int32_t chosen[2] = {0, 0};
+  P4A_CALL(h, p4a_tune(h, chosen, nullptr, 0), "p4a_tune");
+  napi_value o, v;
+  napi_create_object(env, &o);
+  napi_create_string_utf8(env, p4a_tile_variant_name(1, chosen[0]), NAPI_AUTO_LENGTH, &v);
+  napi_set_named_property(env, o, "lambda", v);
+  napi_create_string_utf8(env, p4a_tile_variant_name(2, chosen[1]), NAPI_AUTO_LENGTH, &v);
+  napi_set_named_property(env, o, "xsph", v);
+  return o;
+}
+
 // fieldGenerate(h, anisotropic, skipBodies)
 napi_value FieldGenerate(napi_env env, napi_callback_info info) {
   auto a = args(env, info, 3);
@@ -701,7 +716,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"fieldSmooth", FieldSmooth}, {"meshNormalField", MeshNormalField},
              {"meshMarch", MeshMarch},   {"meshTriangles", MeshTriangles},
              {"bodyPack", BodyPack},     {"snapshotSave", SnapshotSave},
-             {"snapshotLoad", SnapshotLoad}};
+             {"snapshotLoad", SnapshotLoad}, {"tune", Tune}};
   for (auto& f : fns) {
     napi_value v;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);

@@ -115,6 +115,13 @@ class Sim {
 
   // step(frameDt) (sim.js:415-656).  stats and bodyPose are read back at
   // the end of the step (the reference fills them a frame or more later).
+  // Autotuner (no counterpart in the reference): times the neighbour
+  // kernels' traversal shapes on the live scene and keeps the fastest for
+  // later steps; returns the shapes chosen.  The particle state is unchanged.
+  tune() {
+    return native.tune(this.handle);
+  }
+
   step(frameDt) {
     native.setParams(this.handle, this.params);
     const sub = native.step(this.handle, frameDt);

@@ -18,6 +18,14 @@ out.cap = sim.cap;
 out.nBodies = sim.nBodies;
 out.bodyShapes = sim.bodies.map(b => b.shape);
 out.sub = sim.step(1 / 60);
+// the autotuner leaves the particles where they are and stepping goes on
+{
+  const before = Array.from(sim.read('pos'));
+  out.tuned = sim.tune();
+  const after = Array.from(sim.read('pos'));
+  out.tuneKeepsState = before.length === after.length && before.every((v, i) => Object.is(v, after[i]));
+  out.subAfterTune = sim.step(1 / 60);
+}
 // the density buffer of a finished step belongs to the last solver
 // iteration's input; run predict, grid and one lambda so that pred and
 // density refer to the same positions

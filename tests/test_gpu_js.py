@@ -19,6 +19,8 @@ def test_js_sim_verify_harness_and_interactions():
     print(out)
     assert out["nBodies"] == 3 and out["bodyShapes"] == ["sphere", "torus", "box"]
     assert out["cap"] > out["n0"] and out["sub"] == 2
+    assert out["tuneKeepsState"] and out["subAfterTune"] == 2
+    assert set(out["tuned"]) == {"lambda", "xsph"} and all(out["tuned"].values())
     assert out["scanMonotonic"] and out["scanTotal"] == out["n0"]
     assert out["worstDensityRelError"] < 1e-4
     assert out["nonFinite"] == 0 and out["lambdaMax"] <= 0

@@ -311,6 +311,8 @@ P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params*
   BALLOC(A.corr, N);
   BALLOC(A.normal, N);
   BALLOC(A.velq, N);
+  BALLOC(A.hitMask, (size_t)N * 9);  // xsph's first-chunk hit masks for tension
+  A.hitStride = N;
   BALLOC(A.bodyOfPid, N);
   BALLOC(A.restOfPid, N);
   BALLOC(A.cellCount, (size_t)C + 1);

@@ -2561,10 +2561,11 @@ hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HR
 }
 
 // xsph+normals and tension walk the same rows of the same positions: with
-// a two-phase direct shape (single scenes), tension takes xsph's first-chunk
-// masks instead of testing those candidates again.
-bool reuse_masks(const Arrays& A, const TileCfg& t, const BatchTab* B) {
-  return !B && A.hitMask && t.variant >= 5 && t.variant <= 8;
+// a two-phase direct shape, tension takes xsph's first-chunk masks instead
+// of testing those candidates again (a batch indexes them by its global
+// particle index).
+bool reuse_masks(const Arrays& A, const TileCfg& t, const BatchTab*) {
+  return A.hitMask && t.variant >= 5 && t.variant <= 8;
 }
 
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,

@@ -84,12 +84,13 @@ def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
     return p, sc, 1.0 / 120.0
 
 
-def batch_scenes(n_scenes=256, seed=4, n_min=2000, n_max=200000, math=0, **kw):
+def batch_scenes(n_scenes=256, seed=4, n_min=2000, n_max=200000, math=0, tension=0.0, **kw):
     """Many independent scenes of uneven size: particle counts log-uniform
     in [n_min, n_max], box extents U[32, 128] d per axis (each axis at least
     the block plus 8 d of headroom), 0-4 lattice cubes of 4-8 particle
     edges above the block, densities log-uniform in [0.3, 2]; I = 4, FP32,
-    XSPH 0.05, no tension, substep 1/120 s.  Returns [(params, scene)]."""
+    XSPH 0.05, surface tension `tension` (none by default), substep 1/120 s.
+    Returns [(params, scene)]."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(n_scenes):
@@ -101,7 +102,7 @@ def batch_scenes(n_scenes=256, seed=4, n_min=2000, n_max=200000, math=0, **kw):
         box = [float(max(rng.integers(32, 129), b + 8)) for b in (bx, by, bz)]
         box = [min(128.0, v) for v in box]
         bx, by, bz = (int(min(b, v - 8)) for b, v in zip((bx, by, bz), box))
-        p = _params(box, substeps=2, iterations=4, xsph=0.05, tension=0.0, math=math, **kw)
+        p = _params(box, substeps=2, iterations=4, xsph=0.05, tension=tension, math=math, **kw)
         bl = []
         for _ in range(int(rng.integers(0, 5))):
             e = int(rng.integers(4, 9))

@@ -22,9 +22,10 @@ def variant_ids(names):
     return ids
 
 
-@pytest.mark.parametrize("math", [1, 0])
-def test_batch_equals_separate_sims(math, monkeypatch):
-    items = workloads.batch_scenes(n_scenes=7, seed=11, n_min=300, n_max=6000, math=math)
+@pytest.mark.parametrize("math,tension", [(1, 0.0), (0, 0.0), (0, 0.5)])
+def test_batch_equals_separate_sims(math, tension, monkeypatch):
+    items = workloads.batch_scenes(n_scenes=7, seed=11, n_min=300, n_max=6000, math=math,
+                                   tension=tension)
     assert sum(sc.n_bodies for _, sc in items) > 0
     dt = 1 / 120
     b = Batch(items)

@@ -207,3 +207,30 @@ def test_nan_and_infinite_frames():
     same(state(b), state(a))
     a.close()
     b.close()
+
+
+def test_batch_refuses_bad_arguments():
+    """p4a_batch_* refuse unknown scenes, unreadable buffers, reads past a
+    scene's slice (also at offsets near 2^64) and ids read in part of a
+    word; the batch still steps afterwards."""
+    from particles4all_amd import Batch
+    items = workloads.batch_scenes(n_scenes=3, seed=5, n_min=300, n_max=900)
+    b = Batch(items)
+    L = lib()
+    n0 = b.scene_size(0)
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    assert L.p4a_batch_read(b.h, 3, BUF["POS"], ptr, 0, 16) == 1        # no scene 3
+    assert L.p4a_batch_read(b.h, -1, BUF["POS"], ptr, 0, 16) == 1
+    assert L.p4a_batch_read(b.h, 0, BUF["CORR"], ptr, 0, 16) == 1      # not per scene
+    assert L.p4a_batch_read(b.h, 0, BUF["POS"], ptr, n0 * 16, 16) == 5
+    assert L.p4a_batch_read(b.h, 0, BUF["POS"], ptr, (1 << 64) - 8, 16) == 5
+    assert L.p4a_batch_read(b.h, 0, BUF["PID"], ptr, 2, 4) == 1        # part of a word
+    assert L.p4a_batch_substeps(b.h, -1.0, 1) == 1
+    assert L.p4a_batch_substeps(b.h, float("nan"), 1) == 1
+    assert L.p4a_batch_substeps(b.h, 1 / 120, -1) == 1
+    b.substeps(1 / 120, 2)
+    for s in range(3):
+        pid = b.read(s, "PID")
+        assert np.array_equal(np.sort(pid), np.arange(len(pid), dtype=np.uint32))
+    b.close()

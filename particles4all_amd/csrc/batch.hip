@@ -463,17 +463,15 @@ P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* m
   const BatchTab* B = &b->tab;
   // group 2 as the substep runs it: xsph+normals, then tension, whose
   // position and velocity outputs go to scratch so the scenes' state stays
-  float4 *sPos = nullptr, *sVel = nullptr;
-  if (b->tension && b->N > 0) {
-    if (hipMalloc(&sPos, (size_t)b->N * sizeof(float4)) != hipSuccess ||
-        hipMalloc(&sVel, (size_t)b->N * sizeof(float4)) != hipSuccess) {
-      hipFree(sPos);
-      return fail(b, P4A_ERR_NOMEM, "tuner scratch");
-    }
-  }
-  hipEvent_t e0, e1;
-  BTRY(b, hipEventCreate(&e0));
-  BTRY(b, hipEventCreate(&e1));
+  TunerScratch S;
+  if (b->tension && b->N > 0)
+    for (void*& q : S.buf)
+      if (hipMalloc(&q, (size_t)b->N * sizeof(float4)) != hipSuccess)
+        return fail(b, P4A_ERR_NOMEM, "tuner scratch");
+  float4 *sPos = (float4*)S.buf[0], *sVel = (float4*)S.buf[1];
+  BTRY(b, hipEventCreate(&S.ev[0]));
+  BTRY(b, hipEventCreate(&S.ev[1]));
+  hipEvent_t e0 = S.ev[0], e1 = S.ev[1];
   int k = 0;
   for (int group = 1; group <= 2; group++) {
     float best = 1e30f;
@@ -506,11 +504,7 @@ P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* m
     }
     (group == 1 ? b->tile1 : b->tile2).variant = bestV;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   BTRY(b, hipStreamSynchronize(b->stream));
-  hipFree(sPos);
-  hipFree(sVel);
   // the next substep captures its graphs with the chosen shapes
   for (int p = 0; p < 2; p++)
     if (b->graph[p]) {

@@ -292,6 +292,22 @@ hipError_t launch_mesh_march(const MeshArgs& M, const float* field, const float*
 
 int stats_blocks(uint32_t n);
 
+// The autotuners' device scratch and timing events, released on every
+// return path, an error part way through included.
+struct TunerScratch {
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  TunerScratch() = default;
+  TunerScratch(const TunerScratch&) = delete;
+  TunerScratch& operator=(const TunerScratch&) = delete;
+  ~TunerScratch() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+    for (void* p : buf)
+      if (p) hipFree(p);
+  }
+};
+
 // Host-side restatements shared by the single-scene and batched handles.
 struct SceneConsts {
   p4a_params params;

@@ -1852,17 +1852,15 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
   // group 2 is timed as the substep runs it: xsph+normals, then tension
   // (which in the two-phase shapes reuses xsph's hit masks); tension's
   // position and velocity outputs go to scratch, so the live state stays
-  std::vector<void*> scratch;
-  float4 *sPos = nullptr, *sVel = nullptr;
-  if (tension && s->n > 0) {
-    if ((rc = dalloc(s, scratch, &sPos, (size_t)s->n)) || (rc = dalloc(s, scratch, &sVel, (size_t)s->n))) {
-      free_list(scratch);
-      return rc;
-    }
-  }
-  hipEvent_t e0, e1;
-  HIP_TRY(s, hipEventCreate(&e0));
-  HIP_TRY(s, hipEventCreate(&e1));
+  TunerScratch S;
+  if (tension && s->n > 0)
+    for (void*& q : S.buf)
+      if (hipMalloc(&q, (size_t)s->n * sizeof(float4)) != hipSuccess)
+        return fail(s, P4A_ERR_NOMEM, "tuner scratch");
+  float4 *sPos = (float4*)S.buf[0], *sVel = (float4*)S.buf[1];
+  HIP_TRY(s, hipEventCreate(&S.ev[0]));
+  HIP_TRY(s, hipEventCreate(&S.ev[1]));
+  hipEvent_t e0 = S.ev[0], e1 = S.ev[1];
   int k = 0;
   for (int group = 1; group <= 2; group++) {
     float best = 1e30f;
@@ -1894,10 +1892,7 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
     if (group == 1) s->tile1.variant = bestV;
     else s->tile2.variant = bestV;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   HIP_TRY(s, hipStreamSynchronize(s->stream));
-  free_list(scratch);
   drop_graphs(s);
   if (chosen) {
     chosen[0] = s->tile1.variant;

@@ -491,6 +491,8 @@ P4A_API const char* p4a_version(void);
  * keep their own box, h, mass, gravity, XSPH, SOR and solver constants;
  * they must share the iteration count, tension on/off and the math mode,
  * and run the FP32 form.  Scene arrays are copied; no pour capacity.
+ * When p4a_batch_create fails after its argument checks, *out still holds a
+ * handle whose p4a_batch_last_error names the cause; destroy it as usual.
  * ------------------------------------------------------------------- */
 typedef struct p4a_batch p4a_batch;
 P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params* params,

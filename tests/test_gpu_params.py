@@ -87,3 +87,43 @@ def test_set_params_between_substeps_bit_exact():
     sim.set_params(p)
     checked_substep(sim, p, sc)
     sim.close()
+
+
+def _err(a, b):
+    d = np.abs(a - b)
+    return float(d.max()), float(np.sqrt((d ** 2).mean()))
+
+
+@pytest.mark.parametrize("iterations", [1, 3])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fast_forms_odd_iteration_counts(iterations, precision):
+    """Fast math with odd iteration counts, FP32 and FP16-offset forms,
+    against the FP64 oracle (the FP16 form against the oracle's FP64
+    restatement of that form, evaluated at the same quantized positions).
+    Bars as for four iterations: positions within 1e-4 h at most and 1e-6 h
+    RMS (FP32), 1e-3 h and 1e-5 h (FP16); velocities within the position
+    bar over dt."""
+    p, sc, _ = workloads.dam_break_32(scale=0.5, math=0, bodies=False, precision=precision)
+    p.iterations = iterations
+    p.surface_tension_k = 0.5
+    sim = Sim(0)
+    sim.reset(p, sc)
+    sim.substeps(DT, 1)
+    h = sc.h
+    P = O.constants_from(p, sc, DT)
+    st = O.State(rd(sim, "POS"), rd(sim, "VEL"), sim.read("BODY")[:, 0], rd(sim, "REST"))
+    pid0 = sim.read("PID")
+    bpos = rd(sim, "BPOS")
+    bnd = (bpos, bpos[:, 3].copy(), sim.read("BCELL_START"))
+    info = O.substep(O.F64, P, st, bnd, np.zeros((0, 3)), np.zeros((0, 9)), np.zeros(0),
+                     iterations, True, fp16=bool(precision))
+    sim.substeps(DT, 1)
+    assert np.array_equal(sim.read("PID"), pid0[info["order"]])
+    ep = _err(rd(sim, "POS")[:, :3], st.pos[:, :3])
+    ev = _err(rd(sim, "VEL")[:, :3], st.vel[:, :3])
+    print(f"iterations={iterations} precision={precision}: pos max {ep[0] / h:.3e} h "
+          f"rms {ep[1] / h:.3e} h; vel max {ev[0]:.3e}", flush=True)
+    bar, rms = (1e-3, 1e-5) if precision else (1e-4, 1e-6)
+    assert ep[0] <= bar * h and ep[1] <= rms * h
+    assert ev[0] <= bar * h / DT
+    sim.close()

@@ -208,17 +208,20 @@ P4A_API int p4a_substeps(p4a_sim* h, double dt, int32_t count);
 P4A_API int p4a_resize_box(p4a_sim* h, const double box[3]);
 
 /* Sim.appendFluid(pos, vel) (sim.js:774-809): pos3/vel3 are count*3
- * floats; *appended receives the number actually added (0 when full). */
+ * floats; *appended receives the number actually added (0 when full).
+ * Non-finite positions or velocities are refused (P4A_ERR_ARG, nothing
+ * appended). */
 P4A_API int p4a_append_fluid(p4a_sim* h, const float* pos3, const float* vel3,
                              int32_t count, int32_t* appended);
 
 /* Sim.applyRayImpulse(origin, dir, impulse, radius, speedLimit)
- * (sim.js:658-676). */
+ * (sim.js:658-676).  Non-finite arguments are refused (P4A_ERR_ARG). */
 P4A_API int p4a_apply_ray_impulse(p4a_sim* h, const double origin[3],
                                   const double dir[3], const double impulse[3],
                                   double radius, double speed_limit);
 
-/* Sim.holdBody / Sim.releaseBody (sim.js:678-685). */
+/* Sim.holdBody / Sim.releaseBody (sim.js:678-685).  A non-finite target,
+ * rate or limit is refused (P4A_ERR_ARG, the hold unchanged). */
 P4A_API int p4a_hold_body(p4a_sim* h, int32_t body_1based, const double target[3],
                           double rate, double limit);
 P4A_API int p4a_release_body(p4a_sim* h);
@@ -274,7 +277,10 @@ enum p4a_buf {
 /* Copy `bytes` bytes at byte `offset` of buffer `id` to host `dst`
  * (synchronous; the reference's __readBuf, main.js:810-820). */
 P4A_API int p4a_read(p4a_sim* h, int32_t id, void* dst, size_t offset, size_t bytes);
-/* Overwrite part of a state buffer from the host (test and restore use). */
+/* Overwrite part of a state buffer from the host (test and restore use).
+ * The index buffers (PID, ORDER, KEY, SLOT, CELL_START, BCELL_START,
+ * CELL_OF_SLOT) and the synthesised BODY/REST views are not writable;
+ * POS, VEL and PRED take whole finite floats only (P4A_ERR_ARG). */
 P4A_API int p4a_write(p4a_sim* h, int32_t id, const void* src, size_t offset, size_t bytes);
 /* Device address and size of a buffer for zero-copy interop. */
 P4A_API int p4a_device_ptr(p4a_sim* h, int32_t id, void** ptr, size_t* bytes);

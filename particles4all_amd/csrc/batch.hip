@@ -303,6 +303,7 @@ P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params*
   BALLOC(A.cellOfSlot, N);
   BALLOC(A.key, N);
   BALLOC(A.tmp, N);
+  BALLOC(A.fatList, fat_list_len(N));
   BALLOC(A.order, N);
   BALLOC(A.slot, N);
   BALLOC(A.slotOf, N);

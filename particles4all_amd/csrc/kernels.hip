@@ -264,11 +264,14 @@ k_scan_add(uint32_t* __restrict__ count, uint32_t nCells, const uint32_t* __rest
 // Unstable placement by atomics (the reference's scatterSlot order), one
 // atomic per run of equal cells in a wave; k_sort_move then orders every
 // cell's segment by pre-sort index, which makes the result the stable
-// counting sort whatever order the atomics resolved in.
+// counting sort whatever order the atomics resolved in.  Thread 0 also
+// empties the fat-cell list k_sort_move fills.
 __global__ void __launch_bounds__(WG)
 k_scatter_tmp(uint32_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ cellStart,
-              uint32_t* __restrict__ cursor, uint32_t* __restrict__ tmp) {
+              uint32_t* __restrict__ cursor, uint32_t* __restrict__ tmp,
+              uint32_t* __restrict__ fatList) {
   const uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i == 0) fatList[0] = 0u;
   const bool ok = i < n;
   const uint32_t c = ok ? key[i] : 0xffffffffu;
   Run r = wave_runs(c, ok);
@@ -282,46 +285,181 @@ k_scatter_tmp(uint32_t n, const uint32_t* __restrict__ key, const uint32_t* __re
   }
 }
 
-// Stable rank inside each cell segment, then the moves of scatterMove and
-// scatterPhase (wgsl.js:440-475) as a gather into sorted slots.  Writes
-// slot (the reference's scatterSlot output) and the inverse map order.
+// The moves of scatterMove and scatterPhase (wgsl.js:440-475) for one
+// particle: pre-sort index i of cell c goes to sorted slot d.  Writes slot
+// (the reference's scatterSlot output) and the inverse map order.
 // Velocities move only when `vel` is given: inside a substep they are dead
 // until velFromPos rewrites them, so the hot path skips that traffic.
+struct MoveArgs {
+  const float4* __restrict__ pos;
+  const float4* __restrict__ pred;
+  const uint32_t* __restrict__ pid;
+  float4* __restrict__ posOut;
+  float4* __restrict__ predOut;
+  uint32_t* __restrict__ pidOut;
+  uint32_t* __restrict__ order;
+  uint32_t* __restrict__ slot;
+  uint32_t* __restrict__ slotOf;
+  const float4* __restrict__ vel;
+  float4* __restrict__ velOut;
+  HRec* __restrict__ ofsOut;
+  uint32_t* __restrict__ cellOfSlot;
+};
+__device__ __forceinline__ void move_one(const Uniform& U, const MoveArgs& M, uint32_t i,
+                                         uint32_t d, uint32_t c) {
+  M.order[d] = i;
+  M.slot[i] = d;
+  M.posOut[d] = M.pos[i];
+  M.predOut[d] = M.pred[i];
+  const uint32_t q = M.pid[i];
+  M.pidOut[d] = q;
+  // a scattered 4-byte store per particle otherwise: kept for the ids the
+  // body kernels look up
+  if (q - U.slotIdLo < U.slotIdCount) M.slotOf[q] = d;
+  if (M.vel) M.velOut[d] = M.vel[i];
+  if (M.ofsOut) {
+    const float4 p = M.pred[i];
+    M.ofsOut[d] = encode_ofs(U, c, p.x, p.y, p.z);
+    M.cellOfSlot[d] = c;
+  }
+}
+
+// Stable rank inside each cell segment of at most FAT_CELL particles (a
+// scan of the segment), then the moves as a gather into sorted slots.  The
+// first thread of a longer segment lists its cell for k_sort_fat instead,
+// so no thread scans more than FAT_CELL entries whatever the occupancy.
 __global__ void __launch_bounds__(WG)
 k_sort_move(uint32_t n, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ key,
             const uint32_t* __restrict__ cellStart, uint32_t* __restrict__ count,
-            const float4* __restrict__ pos, const float4* __restrict__ pred,
-            const uint32_t* __restrict__ pid, float4* __restrict__ posOut,
-            float4* __restrict__ predOut, uint32_t* __restrict__ pidOut,
-            uint32_t* __restrict__ order, uint32_t* __restrict__ slot,
-            uint32_t* __restrict__ slotOf, const float4* __restrict__ vel,
-            float4* __restrict__ velOut, HRec* __restrict__ ofsOut,
-            uint32_t* __restrict__ cellOfSlot, Uniform U) {
+            uint32_t* __restrict__ fatList, uint32_t fatCap, MoveArgs M, Uniform U) {
   const uint32_t s = blockIdx.x * WG + threadIdx.x;
   if (s >= n) return;
   const uint32_t i = tmp[s];
   const uint32_t c = key[i];
-  const uint32_t b = cellStart[c], e = min(cellStart[c + 1], n);
+  // clamped so a segment stays inside [0, n) even where counts and keys
+  // disagree
+  const uint32_t b = min(cellStart[c], n), e = max(b, min(cellStart[c + 1], n));
+  if (s == b) count[c] = 0u;
+  if (e - b > FAT_CELL) {
+    if (s == b) {
+      const uint32_t f = atomicAdd(&fatList[0], 1u);
+      if (f < fatCap) fatList[1 + f] = c;
+    }
+    return;
+  }
   uint32_t rank = 0;
   for (uint32_t t = b; t < e; t++) rank += (tmp[t] < i) ? 1u : 0u;
   const uint32_t d = b + rank;
   if (d >= n) return;  // holds unless counts and keys disagree
-  order[d] = i;
-  slot[i] = d;
-  posOut[d] = pos[i];
-  predOut[d] = pred[i];
-  const uint32_t q = pid[i];
-  pidOut[d] = q;
-  // a scattered 4-byte store per particle otherwise: kept for the ids the
-  // body kernels look up
-  if (q - U.slotIdLo < U.slotIdCount) slotOf[q] = d;
-  if (vel) velOut[d] = vel[i];
-  if (ofsOut) {
-    const float4 p = pred[i];
-    ofsOut[d] = encode_ofs(U, c, p.x, p.y, p.z);
-    cellOfSlot[d] = c;
+  move_one(U, M, i, d, c);
+}
+
+// Segments longer than FAT_CELL: one workgroup per listed cell sorts the
+// segment's pre-sort indices (distinct values) by an LSD radix sort of
+// 8-bit digits over value - min, ping-ponging between the segment's slices
+// of tmp and order; each pass is a histogram, an exclusive scan and a
+// scatter of 256-entry chunks in order, ranked inside a chunk by digit
+// matching over wave ballots, so the pass is stable.  Work is linear in the
+// segment length (times the number of digits of its index range).
+__global__ void __launch_bounds__(WG)
+k_sort_fat(uint32_t n, uint32_t* __restrict__ tmp, const uint32_t* __restrict__ cellStart,
+           const uint32_t* __restrict__ fatList, uint32_t fatCap, MoveArgs M, Uniform U) {
+  static_assert(WG == 256, "one thread per digit");
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[WG / 64][256];
+  __shared__ uint32_t red[2][WG / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t nFat = min(fatList[0], fatCap);
+  for (uint32_t f = blockIdx.x; f < nFat; f += gridDim.x) {
+    const uint32_t c = fatList[1 + f];
+    const uint32_t b = min(cellStart[c], n), e = max(b, min(cellStart[c + 1], n));
+    const uint32_t k = e - b;
+    // index range of the segment
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (uint32_t t = tid; t < k; t += WG) {
+      const uint32_t v = tmp[b + t];
+      lo = min(lo, v);
+      hi = max(hi, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+      hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+    }
+    if (lane == 0) {
+      red[0][wave] = lo;
+      red[1][wave] = hi;
+    }
+    __syncthreads();
+    lo = red[0][0];
+    hi = red[1][0];
+    for (int w = 1; w < WG / 64; w++) {
+      lo = min(lo, red[0][w]);
+      hi = max(hi, red[1][w]);
+    }
+    const uint32_t span = hi - lo;
+    const int bits = span ? 32 - __builtin_clz(span) : 0;
+    const int passes = (bits + 7) / 8;
+    uint32_t* src = tmp + b;
+    uint32_t* dst = M.order + b;
+    for (int ps = 0; ps < passes; ps++) {
+      const int sh = 8 * ps;
+      base[tid] = 0u;
+      for (int w = 0; w < WG / 64; w++) wcnt[w][tid] = 0u;
+      __syncthreads();
+      for (uint32_t t = tid; t < k; t += WG) atomicAdd(&base[((src[t] - lo) >> sh) & 255u], 1u);
+      __syncthreads();
+      // exclusive scan of the 256 digit counts (one per thread)
+      const uint32_t cnt = base[tid];
+      uint32_t inc = cnt;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += u;
+      }
+      if (lane == 63) red[0][wave] = inc;
+      __syncthreads();
+      uint32_t pre = 0;
+      for (int w = 0; w < wave; w++) pre += red[0][w];
+      base[tid] = pre + inc - cnt;
+      __syncthreads();
+      for (uint32_t c0 = 0; c0 < k; c0 += WG) {
+        const uint32_t t = c0 + (uint32_t)tid;
+        const bool valid = t < k;
+        const uint32_t v = valid ? src[t] : 0u;
+        const uint32_t dg = ((v - lo) >> sh) & 255u;
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const bool on = (dg >> bit) & 1u;
+          const unsigned long long bal = __ballot(valid && on);
+          peers &= on ? bal : ~bal;
+        }
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        if (valid && rank == 0u) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+          uint32_t d = base[dg] + rank;
+          for (int w = 0; w < wave; w++) d += wcnt[w][dg];
+          dst[d] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < WG / 64; w++) {
+          add += wcnt[w][tid];
+          wcnt[w][tid] = 0u;
+        }
+        base[tid] += add;
+        __syncthreads();
+      }
+      uint32_t* sw = src;
+      src = dst;
+      dst = sw;
+    }
+    // src holds the segment in ascending pre-sort index order
+    for (uint32_t t = tid; t < k; t += WG) move_one(U, M, src[t], b + t, c);
+    __syncthreads();
   }
-  if (s == b) count[c] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2468,13 +2606,23 @@ hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveV
                           bool fp16, hipStream_t s) {
   if (U.n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scatter_tmp, dim3(blocks(U.n)), dim3(WG), 0, s, U.n, A.key, A.cellStart,
-                     A.cellCount, A.tmp);
+                     A.cellCount, A.tmp, A.fatList);
+  MoveArgs M{A.pos[par], A.pred[par], A.pid[par], A.pos[par ^ 1], A.pred[par ^ 1],
+             A.pid[par ^ 1], A.order, A.slot, A.slotOf,
+             moveVel ? A.vel[par] : nullptr, moveVel ? A.vel[par ^ 1] : nullptr,
+             fp16 ? A.ofs[0] : nullptr, A.cellOfSlot};
+  // at most n / (FAT_CELL + 1) segments are longer than FAT_CELL
+  const uint32_t fatCap = U.n / (FAT_CELL + 1);
   hipLaunchKernelGGL(k_sort_move, dim3(blocks(U.n)), dim3(WG), 0, s, U.n, A.tmp, A.key,
-                     A.cellStart, A.cellCount, A.pos[par], A.pred[par], A.pid[par],
-                     A.pos[par ^ 1], A.pred[par ^ 1], A.pid[par ^ 1], A.order, A.slot,
-                     A.slotOf, moveVel ? A.vel[par] : nullptr,
-                     moveVel ? A.vel[par ^ 1] : nullptr, fp16 ? A.ofs[0] : nullptr,
-                     A.cellOfSlot, U);
+                     A.cellStart, A.cellCount, A.fatList, fatCap, M, U);
+  // Such segments exist only when the state is far from a fluid at rest
+  // density (a collapsed or blown-up scene); the workgroups leave at once
+  // when none is listed.
+  if (fatCap > 0) {
+    const unsigned g = fatCap < 64u ? fatCap : 64u;
+    hipLaunchKernelGGL(k_sort_fat, dim3(g), dim3(WG), 0, s, U.n, A.tmp, A.cellStart,
+                       A.fatList, fatCap, M, U);
+  }
   return hipGetLastError();
 }
 

@@ -85,6 +85,12 @@ struct BatchTab {
 // far a particle can be from the cell it was binned into.  Slots are
 // cleared by predict.
 constexpr int DRIFT_SLOTS = 256;
+
+// The stable sort ranks the particles of a cell by scanning the cell's
+// segment when it holds at most FAT_CELL of them; longer segments are
+// listed and sorted by a workgroup-wide radix sort (linear in the segment).
+constexpr uint32_t FAT_CELL = 64;
+inline size_t fat_list_len(size_t cap) { return cap / (FAT_CELL + 1) + 2; }
 struct Drift {
   uint32_t* out = nullptr;  // slot the launch records into (null: none)
 };
@@ -99,6 +105,9 @@ struct Arrays {
   uint32_t* pid[2];     // particle id per slot
   uint32_t* key;        // cell id per pre-sort index
   uint32_t* tmp;        // unstable scatter target (pre-sort index per slot)
+  // cells holding more than FAT_CELL particles, listed by the sort for its
+  // segment sort: [0] = count, [1..] = cell ids (fat_list_len(cap) words)
+  uint32_t* fatList;
   uint32_t* order;      // pre-sort index per sorted slot
   uint32_t* slot;       // sorted slot per pre-sort index
   uint32_t* slotOf;     // live slot per particle id

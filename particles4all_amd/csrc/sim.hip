@@ -392,6 +392,13 @@ int launch_body_solve_any(p4a_sim* s, const Uniform& U) {
   return P4A_OK;
 }
 
+template <class T>
+bool all_finite(const T* v, size_t count) {
+  for (size_t k = 0; k < count; k++)
+    if (!std::isfinite(v[k])) return false;
+  return true;
+}
+
 float4* other_pred(p4a_sim* s, int par) {
   return s->predCur == s->A.pred[par] ? s->A.predX : s->A.pred[par];
 }
@@ -793,6 +800,7 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   ALLOC(A.cellOfSlot, cap);
   ALLOC(A.key, cap);
   ALLOC(A.tmp, cap);
+  ALLOC(A.fatList, fat_list_len(cap));
   ALLOC(A.order, cap);
   ALLOC(A.slot, cap);
   ALLOC(A.slotOf, cap);
@@ -1103,11 +1111,12 @@ P4A_API int p4a_append_fluid(p4a_sim* s, const float* pos3, const float* vel3, i
   int rc = check(s);
   if (rc) return rc;
   if (appended) *appended = 0;
-  s->ofsValid = false;
-  if (count <= 0) return P4A_OK;
-  if (!pos3 || !vel3) return P4A_ERR_ARG;
   const int room = std::max(0, s->cap - s->n);
   const int k = std::min(count, room);
+  if (count > 0 && (!pos3 || !vel3)) return P4A_ERR_ARG;
+  if (k > 0 && (!all_finite(pos3, (size_t)k * 3) || !all_finite(vel3, (size_t)k * 3)))
+    return fail(s, P4A_ERR_ARG, "appended positions and velocities must be finite");
+  s->ofsValid = false;
   if (k <= 0) return P4A_OK;
   HIP_TRY(s, hipSetDevice(s->device));
   HIP_TRY(s, hipStreamSynchronize(s->stream));
@@ -1146,6 +1155,9 @@ P4A_API int p4a_apply_ray_impulse(p4a_sim* s, const double origin[3], const doub
   int rc = check(s);
   if (rc) return rc;
   if (!origin || !dir || !impulse) return P4A_ERR_ARG;
+  if (!all_finite(origin, 3) || !all_finite(dir, 3) || !all_finite(impulse, 3) ||
+      !std::isfinite(radius) || !std::isfinite(speed_limit))
+    return fail(s, P4A_ERR_ARG, "ray impulse arguments must be finite");
   if (!s->n || radius <= 0) return P4A_OK;
   // Math.hypot, as the reference (no overflow or underflow of the squares)
   const double len = std::hypot(impulse[0], impulse[1], impulse[2]);
@@ -1171,6 +1183,8 @@ P4A_API int p4a_hold_body(p4a_sim* s, int32_t body, const double target[3], doub
   int rc = check(s);
   if (rc) return rc;
   if (!target) return P4A_ERR_ARG;
+  if (!all_finite(target, 3) || !std::isfinite(rate) || !std::isfinite(limit))
+    return fail(s, P4A_ERR_ARG, "hold target, rate and limit must be finite");
   s->heldBody = (body >= 1 && body <= s->nBodies) ? body : -1;
   for (int k = 0; k < 3; k++) s->heldTarget[k] = target[k];
   s->heldRate = rate;
@@ -1218,9 +1232,20 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   int rc = check(s);
   if (rc) return rc;
   if (!src && bytes) return P4A_ERR_ARG;
+  // index buffers: the walks and the sort index particle arrays through
+  // them without bounds checks, so only the library writes them
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
-      id == P4A_BUF_KEY || id == P4A_BUF_SLOT)
+      id == P4A_BUF_KEY || id == P4A_BUF_SLOT || id == P4A_BUF_CELL_START ||
+      id == P4A_BUF_BCELL_START || id == P4A_BUF_CELL_OF_SLOT)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");
+  // particle positions and velocities stay finite (a non-finite position
+  // would be binned into a clamped corner cell)
+  if (id == P4A_BUF_POS || id == P4A_BUF_VEL || id == P4A_BUF_PRED) {
+    if (offset % 4 != 0 || bytes % 4 != 0)
+      return fail(s, P4A_ERR_ARG, "float buffers are written in whole floats");
+    if (!all_finite((const float*)src, bytes / 4))
+      return fail(s, P4A_ERR_ARG, "positions and velocities must be finite");
+  }
   HIP_TRY(s, hipSetDevice(s->device));
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   BufView v = buf_view(s, id);

@@ -83,6 +83,34 @@ def battery(sim, p, sc, snap):
     raw(sim, L.p4a_write(sim.h, BUF["PID"], ptr, 0, 4), "P4A_ERR_ARG", "write ids")
     raw(sim, L.p4a_write(sim.h, BUF["SLOT"], ptr, 0, 4), "P4A_ERR_ARG", "write slots")
     raw(sim, L.p4a_write(sim.h, 999, ptr, 0, 4), "P4A_ERR_ARG", "write unknown id")
+    # index buffers the walks and the sort index through: not writable
+    for name in ("CELL_START", "BCELL_START", "CELL_OF_SLOT", "ORDER", "KEY"):
+        raw(sim, L.p4a_write(sim.h, BUF[name], ptr, 0, 4), "P4A_ERR_ARG", f"write {name}")
+    # positions and velocities: whole finite floats only
+    bad = np.zeros(8, np.float32)
+    for k, v in ((1, np.nan), (6, np.inf), (3, -np.inf)):
+        bad[:] = 1.0
+        bad[k] = v
+        for name in ("POS", "VEL", "PRED"):
+            raw(sim, L.p4a_write(sim.h, BUF[name], bad.ctypes.data, 0, 32), "P4A_ERR_ARG",
+                f"write non-finite {name}")
+    raw(sim, L.p4a_write(sim.h, BUF["POS"], ptr, 2, 16), "P4A_ERR_ARG", "write part of a float")
+    # non-finite interaction arguments
+    nan = float("nan")
+    refused("P4A_ERR_ARG", lambda: sim.apply_ray_impulse([0, 0, 0], [1, 0, 0], [nan, 1, 1], 4.0, 10.0))
+    refused("P4A_ERR_ARG", lambda: sim.apply_ray_impulse([0, nan, 0], [1, 0, 0], [1, 1, 1], 4.0, 10.0))
+    refused("P4A_ERR_ARG", lambda: sim.apply_ray_impulse([0, 0, 0], [1, 0, 0], [1, 1, 1], nan, 10.0))
+    refused("P4A_ERR_ARG", lambda: sim.apply_ray_impulse([0, 0, 0], [1, 0, 0], [1, 1, 1], 4.0, float("inf")))
+    refused("P4A_ERR_ARG", lambda: sim.hold_body(1, [nan, 2.0, 3.0], 10.0, 4.0))
+    refused("P4A_ERR_ARG", lambda: sim.hold_body(1, [1.0, 2.0, 3.0], float("inf"), 4.0))
+    refused("P4A_ERR_ARG", lambda: sim.hold_body(1, [1.0, 2.0, 3.0], 10.0, nan))
+    pts = np.full((4, 3), 5.0, np.float32)
+    pts[2, 1] = np.nan
+    refused("P4A_ERR_ARG", lambda: sim.append_fluid(pts, np.zeros_like(pts)))
+    pts[2, 1] = 5.0
+    vel = np.zeros_like(pts)
+    vel[3, 0] = np.inf
+    refused("P4A_ERR_ARG", lambda: sim.append_fluid(pts, vel))
     d, nbytes = C.c_void_p(), C.c_size_t()
     raw(sim, L.p4a_device_ptr(sim.h, 999, C.byref(d), C.byref(nbytes)), "P4A_ERR_ARG", "ptr")
     raw(sim, L.p4a_run_stage(sim.h, 99, 1 / 120), "P4A_ERR_ARG", "unknown stage")

@@ -402,18 +402,23 @@ void jacobi_ref(R a[3][3], R v[3][3]) {  // aniso_wgsl.js:30-55, row-major
       }
 }
 
+// Targets sel[0..nsel) (all n when sel is null); outputs are indexed like
+// the targets, neighbours come from all n particles.
 template <class R>
 void aniso_k(const OrcParams& P, int n, const double* pos, const uint32_t* cs, const double* ap,
-             double* smooth, double* g) {
+             double* smooth, double* g, const int* sel = nullptr, int nsel = 0) {
   K<R> k(P);
   const R lambda = (R)ap[0], support = (R)ap[1], radius = (R)ap[2];
   const int cells = (int)ap[3];
   const R kr = (R)ap[4], kn = (R)ap[5], ks = (R)ap[6], stretch = (R)ap[7], minExt = (R)ap[8];
   const int neps = (int)ap[9];
   const R invSupport = (R)(1.0 / ap[1]);
+  const int nt = sel ? nsel : n;
 #pragma omp parallel for schedule(dynamic, 1024)
-  for (int i = 0; i < n; i++) {
-    const R xi[3] = {(R)pos[i * 4], (R)pos[i * 4 + 1], (R)pos[i * 4 + 2]};
+  for (int o = 0; o < nt; o++) {
+    const int src = sel ? sel[o] : o;
+    const int i = o;  // output slot
+    const R xi[3] = {(R)pos[src * 4], (R)pos[src * 4 + 1], (R)pos[src * 4 + 2]};
     int c[3];
     for (int a = 0; a < 3; a++) {
       R f = std::floor((xi[a] - k.boxMin[a]) * invSupport);
@@ -505,6 +510,13 @@ void orc_aniso(int mode, const OrcParams* P, int n, const double* pos, const uin
                const double* ap, double* smooth, double* g) {
   if (mode == 0) aniso_k<float>(*P, n, pos, cs, ap, smooth, g);
   else aniso_k<double>(*P, n, pos, cs, ap, smooth, g);
+}
+
+// The same for the targets sel[0..nsel) only (outputs nsel records).
+void orc_aniso_sel(int mode, const OrcParams* P, int n, const double* pos, const uint32_t* cs,
+                   const double* ap, int nsel, const int* sel, double* smooth, double* g) {
+  if (mode == 0) aniso_k<float>(*P, n, pos, cs, ap, smooth, g, sel, nsel);
+  else aniso_k<double>(*P, n, pos, cs, ap, smooth, g, sel, nsel);
 }
 
 // The Jacobi eigensolve of the anisotropy pass on one symmetric matrix

@@ -481,9 +481,10 @@ def stats(density, vel, inv_rho0, mass):
 
 
 def aniso(mode, P, pos, cs, radius_scale=2.0, lam=0.9, kr=4.0, kn=1.0, ks=1.0, stretch=2.0,
-          min_extent=0.0, neps=25):
+          min_extent=0.0, neps=25, sel=None):
     """Anisotropy pass (aniso_wgsl.js:3-159) with the parameters of
-    SurfaceMesh.buildAnisotropy (mesh.js:221-242)."""
+    SurfaceMesh.buildAnisotropy (mesh.js:221-242); with `sel`, for those
+    targets only (outputs in the order of sel)."""
     pos = np.ascontiguousarray(pos, np.float64)
     cs = np.ascontiguousarray(cs, np.uint32)
     h = P.h
@@ -491,6 +492,15 @@ def aniso(mode, P, pos, cs, radius_scale=2.0, lam=0.9, kr=4.0, kn=1.0, ks=1.0, s
     ap = np.array([lam, h, radius, math.ceil(radius_scale), kr, kn, ks / (0.15 * radius * radius),
                    stretch, min_extent, neps], np.float64)
     n = len(pos)
+    if sel is not None:
+        sel = np.ascontiguousarray(sel, np.int32)
+        assert sel.size == 0 or (sel.min() >= 0 and sel.max() < n)
+        k = len(sel)
+        smooth = np.zeros((k, 4))
+        g = np.zeros((k, 8))
+        lib().orc_aniso_sel(mode, C.byref(P), n, _p(pos), _p(cs), _p(ap), k, _p(sel),
+                            _p(smooth), _p(g))
+        return smooth, g.reshape(k * 2, 4)
     smooth = np.zeros((n, 4))
     g = np.zeros((n, 8))
     lib().orc_aniso(mode, C.byref(P), n, _p(pos), _p(cs), _p(ap), _p(smooth), _p(g))
@@ -508,6 +518,7 @@ def jacobi(mode, a):
 def _setup(L):
     vp = C.c_void_p
     L.orc_aniso.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
+    L.orc_aniso_sel.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp, vp]
     L.orc_jacobi.argtypes = [C.c_int, vp, vp, vp]
     L.orc_cell_ids.argtypes = [C.c_int, C.c_void_p, C.c_int, vp, vp]
     L.orc_grid.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]

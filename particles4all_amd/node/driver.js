@@ -1,113 +1,159 @@
 'use strict';
-// The callers of Sim that live in the reference's app shell, restated for a
-// headless host: the pour hose (src/main.js:89-160), the box-wall driver
-// (main.js:974-985) and the per-frame stepping loop (main.js:986-1000).
-// `ui` carries the same fields the reference keeps on its ui object.
+// Headless counterparts of the Sim callers in the reference's app shell
+// (behaviour of src/main.js:89-160 and 974-1000): a pour hose feeding
+// Sim.appendFluid, a box wall moving toward its target, and the per-frame
+// loop that steps the simulation around them.  State lives on a `ui`
+// object with the reference's field names (pouring, pourLeft, pourSpeed,
+// pourWidth, pourHeight, pourTilt, pourExtruded, pourNextLayer, pourSeed,
+// timeScale, paused, boxBaseX, boxScaleX).
 
-const POUR_SLACK = 1.25;
+// The jet is emitted as square layers of particles one spacing apart
+// across, LAYER_PITCH spacings apart along the jet.
+const LAYER_PITCH = 1.25;
+const MAX_LAYERS_PER_CALL = 32;
+const MAX_SLICES_PER_FRAME = 64;
 
+// Particles across the nozzle: its width in spacings, rounded, at least 1.
 function pourLatticeAcross(width, spacing) {
   return Math.max(1, Math.round(width / Math.max(spacing, 1e-4)));
 }
 
+// Particles per second the hose delivers: a layer of k x k every
+// LAYER_PITCH spacings of jet.
 function pourRatePerSecond(ui, spacing) {
   const k = pourLatticeAcross(ui.pourWidth, spacing);
-  return Math.floor(k * k * ui.pourSpeed / Math.max(spacing * POUR_SLACK, 1e-4));
+  return Math.floor(k * k * ui.pourSpeed / Math.max(spacing * LAYER_PITCH, 1e-4));
 }
 
-// The hose's LCG (main.js:100-104); state lives on `ui.pourSeed`.
-function pourJitter(ui, step) {
+// Placement jitter of the hose, +-0.001 spacing: a 31-bit linear
+// congruential generator (multiplier 1103515245, increment 12345, taken in
+// double precision as JavaScript evaluates it), bits 8-23 as the fraction.
+function pourJitter(ui, spacing) {
   ui.pourSeed = (ui.pourSeed * 1103515245 + 12345) & 0x7fffffff;
-  return ((ui.pourSeed >>> 8 & 0xffff) / 65535 - 0.5) * 0.002 * step;
+  const frac = ((ui.pourSeed >>> 8) & 0xffff) / 65535;
+  return (frac - 0.5) * 0.002 * spacing;
 }
 
 function makeUi(sim, opts) {
-  opts = opts || {};
-  return Object.assign({
-    pouring: false, pourLeft: sim.pourBudget || 0, pourBudget: sim.pourBudget || 0,
+  const budget = sim.pourBudget || 0;
+  const ui = {
+    pouring: false, pourLeft: budget, pourBudget: budget,
     pourSpeed: 3.0, pourWidth: 0.24, pourHeight: 0.85, pourTilt: 0, pourExtruded: 0,
     pourNextLayer: 0, pourSeed: 22222, timeScale: 1, paused: false,
     boxBaseX: sim.params.box[0], boxScaleX: 1,
-  }, opts);
+  };
+  return Object.assign(ui, opts || {});
 }
 
-// pourStep(sim, ui, simDt) (main.js:106-160)
+// Layers whose place on the jet the extruded length has reached since the
+// last emission (layer m sits m * pitch behind the jet's front).
+function dueLayers(ui, pitch) {
+  let due = 0;
+  while ((ui.pourNextLayer + due) * pitch <= ui.pourExtruded) due++;
+  return due;
+}
+
+// Centre and vertical velocity of layer m: a ballistic jet leaving the
+// nozzle (one spacing inside the wall at x = h + d, at pourHeight of the
+// box, mid-depth) at pourSpeed, tilted down by pourTilt degrees.
+function layerState(sim, ui, m, pitch) {
+  const p = sim.params;
+  const tilt = ui.pourTilt * Math.PI / 180;
+  const vx = ui.pourSpeed * Math.cos(tilt), vy0 = -ui.pourSpeed * Math.sin(tilt);
+  const t = (ui.pourExtruded - m * pitch) / Math.max(ui.pourSpeed, 1e-6);
+  return {
+    x: sim.h + p.spacing + vx * t,
+    y: ui.pourHeight * p.box[1] + vy0 * t - 0.5 * p.gravity * t * t,
+    z: 0.5 * p.box[2],
+    vx,
+    vy: vy0 - p.gravity * t,
+  };
+}
+
+// Advances the hose by simDt of simulated time and appends the layers that
+// came due (at most the remaining budget).  Returns the particles added.
 function pourStep(sim, ui, simDt) {
   if (!ui.pouring || ui.pourLeft <= 0 || simDt <= 0) return 0;
-  const p = sim.params;
-  const d = p.spacing;
-  const box = p.box;
-  const step = d;
-  const layerStep = d * POUR_SLACK;
+  const d = sim.params.spacing;
+  const box = sim.params.box;
+  const pitch = d * LAYER_PITCH;
   ui.pourExtruded += ui.pourSpeed * simDt;
-  let layers = 0;
-  for (let m = ui.pourNextLayer; m * layerStep <= ui.pourExtruded; ++m) ++layers;
-  if (layers <= 0) return 0;
-  layers = Math.min(layers, 32);
+  const due = Math.min(dueLayers(ui, pitch), MAX_LAYERS_PER_CALL);
+  if (due === 0) return 0;
   const k = pourLatticeAcross(ui.pourWidth, d);
-  const span = (k - 1) * step;
-  const nozzle = [sim.h + d, ui.pourHeight * box[1], 0.5 * box[2]];
-  const tilt = ui.pourTilt * Math.PI / 180;
-  const jet = [ui.pourSpeed * Math.cos(tilt), -ui.pourSpeed * Math.sin(tilt), 0];
+  const half = 0.5 * (k - 1) * d;
   const pos = [], vel = [];
-  const g = p.gravity;
-  for (let l = 0; l < layers && pos.length / 3 < ui.pourLeft; ++l) {
-    const m = ui.pourNextLayer + l;
-    const behind = ui.pourExtruded - m * layerStep;
-    const t = behind / Math.max(ui.pourSpeed, 1e-6);
-    const cx = nozzle[0] + jet[0] * t;
-    const cy = nozzle[1] + jet[1] * t - 0.5 * g * t * t;
-    const vy = jet[1] - g * t;
-    if (cx >= box[0] - d || cy <= d) break;
-    for (let iy = 0; iy < k && pos.length / 3 < ui.pourLeft; ++iy)
-      for (let iz = 0; iz < k && pos.length / 3 < ui.pourLeft; ++iz) {
-        const y = cy - 0.5 * span + iy * step + pourJitter(ui, step);
+  let count = 0;
+  emit:
+  for (let l = 0; l < due; l++) {
+    if (count >= ui.pourLeft) break;
+    const c = layerState(sim, ui, ui.pourNextLayer + l, pitch);
+    // the jet has left the box: this layer and the later ones are dropped
+    if (c.x >= box[0] - d || c.y <= d) break;
+    for (let a = 0; a < k; a++) {
+      for (let b = 0; b < k; b++) {
+        if (count >= ui.pourLeft) break emit;
+        const y = c.y - half + a * d + pourJitter(ui, d);
         if (y <= d || y >= box[1] - d) continue;
-        pos.push(cx + pourJitter(ui, step), y,
-                 nozzle[2] - 0.5 * span + iz * step + pourJitter(ui, step));
-        vel.push(jet[0], vy, 0);
+        const x = c.x + pourJitter(ui, d);
+        const z = c.z - half + b * d + pourJitter(ui, d);
+        pos.push(x, y, z);
+        vel.push(c.vx, c.vy, 0);
+        count++;
       }
+    }
   }
-  ui.pourNextLayer += layers;
-  if (!pos.length) return 0;
+  ui.pourNextLayer += due;
+  if (count === 0) return 0;
   const added = sim.appendFluid(pos, vel);
   ui.pourLeft = Math.max(0, ui.pourLeft - added);
   if (ui.pourLeft === 0) ui.pouring = false;
   return added;
 }
 
-// Box-wall driver (main.js:974-985): move wall x toward the target at
-// most 2*dt per frame.
+// Moves the box's x wall toward boxBaseX * boxScaleX by at most 2 dt.
+// Returns whether it moved.
 function boxWallStep(sim, ui, dt) {
-  const want = ui.boxBaseX * ui.boxScaleX;
-  const gap = want - sim.params.box[0];
-  const step = Math.sign(gap) * Math.min(Math.abs(gap), 2 * dt);
-  if (Math.abs(step) > 1e-6) {
-    sim.resizeBox([sim.params.box[0] + step, sim.params.box[1], sim.params.box[2]]);
-    return true;
-  }
-  return false;
+  const box = sim.params.box;
+  const gap = ui.boxBaseX * ui.boxScaleX - box[0];
+  const move = Math.sign(gap) * Math.min(Math.abs(gap), 2 * dt);
+  if (Math.abs(move) <= 1e-6) return false;
+  sim.resizeBox([box[0] + move, box[1], box[2]]);
+  return true;
 }
 
-// One rendered frame's simulation work (main.js:974-1000).
+// Lengths of the slices a frame of `total` seconds is stepped in while the
+// hose runs: at most one substep of a 1/60 s frame each, at most 64.
+function frameSlices(total, substeps) {
+  const most = 1 / 60 / Math.max(1, substeps);
+  const out = [];
+  let left = total;
+  while (left > 1e-6 && out.length < MAX_SLICES_PER_FRAME) {
+    const s = Math.min(left, most);
+    out.push(s);
+    left -= s;
+  }
+  return out;
+}
+
+// One rendered frame of simulation work: the wall, then the frame stepped
+// whole, or in slices with the hose fed after each while it pours.
+// Returns the particles poured.
 function advanceFrame(sim, ui, dt) {
   if (ui.paused) return 0;
   boxWallStep(sim, ui, dt);
+  const total = dt * ui.timeScale;
+  if (!(ui.pouring && ui.pourLeft > 0)) {
+    sim.step(total);
+    return 0;
+  }
   let poured = 0;
-  if (ui.pouring && ui.pourLeft > 0) {
-    const sub = 1 / 60 / Math.max(1, sim.params.substeps);
-    let want = dt * ui.timeScale;
-    for (let guard = 0; want > 1e-6 && guard < 64; ++guard) {
-      const slice = Math.min(want, sub);
-      sim.step(slice);
-      poured += pourStep(sim, ui, sim.lastAdvanced || 0);
-      want -= slice;
-    }
-  } else {
-    sim.step(dt * ui.timeScale);
+  for (const s of frameSlices(total, sim.params.substeps)) {
+    sim.step(s);
+    poured += pourStep(sim, ui, sim.lastAdvanced || 0);
   }
   return poured;
 }
 
-module.exports = { POUR_SLACK, pourLatticeAcross, pourRatePerSecond, pourJitter, pourStep,
-                   boxWallStep, advanceFrame, makeUi };
+module.exports = { LAYER_PITCH, pourLatticeAcross, pourRatePerSecond, pourJitter, pourStep,
+                   boxWallStep, advanceFrame, frameSlices, makeUi };

@@ -46,95 +46,113 @@ const UNSUPPORTED = [
   'ssfrhalfthick', 'weburl', 'webhighperf', 'webpresets',
 ];
 
-// parseIni (settingsini.js:45-55)
+// Reader of the reference's preset format (behaviour of settingsini.js:45-55):
+// one `key = value` entry per line, keys case-insensitive, `#` and `;`
+// comment lines, a line without a key before its first `=` ignored, the
+// last entry of a key winning.
+const ENTRY = /^([^#;=][^=]*)=(.*)$/;
+
 function parseIni(text) {
-  const kv = {};
-  for (const raw of String(text).split(/\r?\n/)) {
-    const line = raw.trim();
-    if (!line || line.startsWith('#') || line.startsWith(';')) continue;
-    const eq = line.indexOf('=');
-    if (eq <= 0) continue;
-    kv[line.slice(0, eq).trim().toLowerCase()] = line.slice(eq + 1).trim();
-  }
-  return kv;
+  const out = {};
+  String(text).split(/\r?\n/).forEach((raw) => {
+    const m = ENTRY.exec(raw.trim());
+    if (m) out[m[1].trim().toLowerCase()] = m[2].trim();
+  });
+  return out;
 }
 
 const VIEW = { 0: 'particles', 1: 'mesh', 2: 'mesh', 3: 'ray', 4: 'ssfr' };
+const own = (o, k) => Object.prototype.hasOwnProperty.call(o, k);
 
-// iniToQuery (settingsini.js:59-93)
-function iniToQuery(text, opts) {
-  opts = opts || {};
-  const kv = parseIni(text);
-  const sp = new URLSearchParams();
-  const applied = [], skipped = [];
-  for (const k of Object.keys(kv)) {
-    const v = kv[k];
-    const q = KEYS[k];
-    if (q) { sp.set(q, v); applied.push(k); continue; }
-    if (k === 'display') {
-      const view = VIEW[String(parseInt(v, 10))];
-      if (view) { sp.set('view', view); applied.push(k); } else skipped.push(k);
-      continue;
-    }
-    if (k === 'cubemap') continue;
-    if (k === 'body') {
-      if (v.trim()) sp.set('body', v.trim()); else sp.set('bodies', '0');
-      applied.push(k);
-      continue;
-    }
-    skipped.push(k);
+// The URL entries one ini key contributes, or null for a key or value the
+// app does not understand.
+function urlEntries(key, value) {
+  if (own(KEYS, key)) return [[KEYS[key], value]];
+  if (key === 'display') {
+    const view = VIEW[parseInt(value, 10)];
+    return view ? [['view', view]] : null;
   }
-  let missingPanorama = null;
-  if (kv.cubemap) {
-    const name = kv.cubemap.replace(/^"|"$/g, '').split(/[\\/]/).pop();
-    if (name) {
-      sp.set('cubemap', 'env/' + name);
-      applied.push('cubemap');
-      missingPanorama = name;
-    }
+  if (key === 'body') {
+    const list = value.trim();
+    return list ? [['body', list]] : [['bodies', '0']];
   }
-  const keep = opts.keep || {};
-  for (const k of Object.keys(keep)) {
-    const v = keep[k];
-    if (v !== null && v !== undefined && !sp.has(k)) sp.set(k, v);
-  }
-  return { query: sp.toString(), applied, skipped, missingPanorama };
+  return null;
 }
 
-// The simulation part of main.js:361-390: URL query -> params.
+// File name of a panorama path, without the quotes a path may carry.
+function panoramaName(value) {
+  let v = value;
+  if (v.startsWith('"')) v = v.slice(1);
+  if (v.endsWith('"')) v = v.slice(0, -1);
+  return v.slice(Math.max(v.lastIndexOf('/'), v.lastIndexOf('\\')) + 1);
+}
+
+// A preset file as the URL query the app starts from (behaviour of
+// settingsini.js:59-93): the keys it took (`applied`), the ones it did not
+// (`skipped`), and the panorama a server has to provide.  `opts.keep` adds
+// entries the file does not set.
+function iniToQuery(text, opts) {
+  const kv = parseIni(text);
+  const entries = [];
+  const applied = [];
+  const skipped = [];
+  Object.keys(kv).filter(k => k !== 'cubemap').forEach((k) => {
+    const got = urlEntries(k, kv[k]);
+    (got ? applied : skipped).push(k);
+    if (got) got.forEach(e => entries.push(e));
+  });
+  const panorama = kv.cubemap ? panoramaName(kv.cubemap) : '';
+  if (panorama) {
+    entries.push(['cubemap', 'env/' + panorama]);
+    applied.push('cubemap');
+  }
+  const sp = new URLSearchParams();
+  entries.forEach(e => sp.set(e[0], e[1]));
+  const keep = (opts && opts.keep) || {};
+  Object.keys(keep).forEach((k) => {
+    if (keep[k] !== null && keep[k] !== undefined && !sp.has(k)) sp.set(k, keep[k]);
+  });
+  return { query: sp.toString(), applied, skipped, missingPanorama: panorama || null };
+}
+
+// URL key -> Sim parameter, for the numeric parameters the app reads from
+// the query (main.js:361-390).
+const NUMERIC = [
+  ['particles', 'targetParticleCount'], ['bodysize', 'bodySize'], ['substeps', 'substeps'],
+  ['iters', 'iterations'], ['omega', 'omega'], ['xsph', 'xsphC'], ['scorr', 'sCorrK'],
+  ['spacing', 'spacing'], ['gravity', 'gravity'], ['tension', 'surfaceTensionK'],
+  ['cfm', 'cfmEpsilonRel'], ['scorrdq', 'sCorrDq'],
+];
+
+// Three or more finite numbers separated by spaces or commas, else null.
+function numberList(text, atLeast) {
+  if (!text) return null;
+  const nums = text.split(/[ ,]+/).map(Number);
+  if (nums.length < atLeast || nums.some(x => !Number.isFinite(x))) return null;
+  return nums;
+}
+
+// The simulation parameters a URL query selects on top of a preset's
+// defaults: numbers (a present key is taken as Number(value)), the body
+// list (`bodies=0` empties it, `body=a,b` replaces it), `boundary=0`, and a
+// box of three numbers.
 function paramsFromQuery(query, preset) {
   const q = new URLSearchParams(query);
-  const qNum = (k, d) => (q.has(k) ? Number(q.get(k)) : d);
-  const qVec = (k, n) => {
-    const v = q.get(k);
-    if (!v) return null;
-    const a = v.split(/[ ,]+/).map(Number);
-    return a.length >= n && a.every(x => Number.isFinite(x)) ? a : null;
-  };
   const params = defaultParams(preset);
-  const bodiesOn = q.get('bodies') !== '0';
-  if (!bodiesOn) params.bodies = [];
-  params.targetParticleCount = qNum('particles', params.targetParticleCount);
-  if (bodiesOn && q.get('body')) params.bodies = q.get('body').split(',');
-  params.bodySize = qNum('bodysize', params.bodySize);
-  params.substeps = qNum('substeps', params.substeps);
-  params.iterations = qNum('iters', params.iterations);
-  params.omega = qNum('omega', params.omega);
-  params.xsphC = qNum('xsph', params.xsphC);
-  params.sCorrK = qNum('scorr', params.sCorrK);
-  params.spacing = qNum('spacing', params.spacing);
-  params.gravity = qNum('gravity', params.gravity);
-  params.surfaceTensionK = qNum('tension', params.surfaceTensionK);
-  params.cfmEpsilonRel = qNum('cfm', params.cfmEpsilonRel);
-  params.sCorrDq = qNum('scorrdq', params.sCorrDq);
+  NUMERIC.forEach(([key, name]) => {
+    if (q.has(key)) params[name] = Number(q.get(key));
+  });
+  if (q.get('bodies') === '0') params.bodies = [];
+  else if (q.get('body')) params.bodies = q.get('body').split(',');
   params.noBoundary = q.get('boundary') === '0';
-  const b = qVec('box', 3);
-  if (b) params.box = b.slice(0, 3);
+  const box = numberList(q.get('box'), 3);
+  if (box) params.box = box.slice(0, 3);
   return params;
 }
 
-// Construction from a preset: presets/<name>.ini through iniToQuery, URL
-// overrides first (main.js:255-279), then paramsFromQuery.
+// Construction from a preset: presets/<name>.ini read into a query, the
+// caller's overrides winning over the file (main.js:255-279), then the
+// parameters of that query.
 function loadPreset(name, overrides, dir) {
   const preset = PRESETS[name] ? name : 'small';
   const q = new URLSearchParams(overrides || '');
@@ -142,12 +160,11 @@ function loadPreset(name, overrides, dir) {
   if (PRESETS[preset].ini) {
     const file = path.join(dir || path.join(__dirname, '..'), PRESETS[preset].ini);
     const r = iniToQuery(fs.readFileSync(file, 'utf8'));
-    for (const kv of new URLSearchParams(r.query)) if (!q.has(kv[0])) q.set(kv[0], kv[1]);
+    new URLSearchParams(r.query).forEach((v, k) => { if (!q.has(k)) q.set(k, v); });
     const unknown = r.skipped.filter(k => UNSUPPORTED.indexOf(k) < 0);
     if (unknown.length) note = `${file}: ${unknown.length} unknown keys (${unknown.join(', ')})`;
   }
-  const params = paramsFromQuery(q.toString(), preset);
-  return { params, query: q.toString(), note };
+  return { params: paramsFromQuery(q.toString(), preset), query: q.toString(), note };
 }
 
 module.exports = { KEYS, UNSUPPORTED, parseIni, iniToQuery, paramsFromQuery, loadPreset };

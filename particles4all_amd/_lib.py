@@ -459,6 +459,19 @@ class Sim:
         buf = np.frombuffer(bytes(data), np.uint8)
         self._check(lib().p4a_snapshot_load(self.h, _ptr(buf), buf.nbytes), "snapshot_load")
 
+    def body_state_fp64(self, snap=None):
+        """The FP64 body state (centres (nB, 3), row-major rotations (nB, 9))
+        as the snapshot layout holds it: after the per-particle arrays and
+        the cell starts of the last grid build."""
+        snap = self.snapshot() if snap is None else snap
+        i = self.info()
+        n, nc, nB, nb = i.n, i.n_cells, i.n_bodies, i.n_boundary
+        hdr = len(snap) - (n * 60 + (nc + 2) * 4 + nB * 172 + nb * 16)
+        off = hdr + n * 60 + (nc + 2) * 4
+        centre = np.frombuffer(snap, np.float64, 4 * nB, off).reshape(nB, 4)[:, :3].copy()
+        rot = np.frombuffer(snap, np.float64, 9 * nB, off + 32 * nB).reshape(nB, 9).copy()
+        return centre, rot
+
     def save_snapshot(self, path):
         with open(path, "wb") as f:
             f.write(self.snapshot())

@@ -6,7 +6,7 @@
                    corner of a 256x128x128 box, half its width and half its
                    height; substep 1/120 s, 4 iterations, XSPH 0.01, surface
                    tension 0.5.
-  block_4m_bodies  256x128x128 block in a 512x192x128 box plus 64 bodies
+  block_4m_bodies  256x128x128 block in a 512x192x130 box plus 64 bodies
                    (32 lattice cubes, edges 6-12; 32 lattice spheres, radii
                    4-8; densities log-uniform in [0.3, 2.0]) in the space
                    above the fluid; settings as block_1m.
@@ -14,6 +14,13 @@
 Jitter is uniform +-0.05d per coordinate from the reference's LCG idiom
 (scene.js:52-56); mass and denomRest are calibrated on the unjittered
 lattice.  `scale` shrinks the lattice for tests (box shrinks with it).
+
+The lattice sites start one spacing inside the box (sites margin + (i +
+0.5) d), as the reference's fluidBlock places them (margin = d,
+scene.js:45, 61-63), so the fluid's outer layer sits 1.5 d from the wall
+samples and the block starts near rest density; block_4m_bodies' box is
+two spacings deeper than its lattice for that.  block_1m keeps its block
+against the walls (its 128-deep block spans the 128-deep box).
 """
 import numpy as np
 
@@ -27,7 +34,7 @@ def _params(box, substeps, iterations, xsph, tension, gravity=9.8, pour_budget=0
     return p
 
 
-def dam_break_32(scale=1.0, seed=1, bodies=True, **kw):
+def dam_break_32(scale=1.0, seed=1, bodies=True, margin=1.0, **kw):
     L = max(4, int(round(32 * scale)))
     B = 2 * L
     p = _params([B, B, B], substeps=2, iterations=4, xsph=0.05, tension=0.0, **kw)
@@ -37,11 +44,11 @@ def dam_break_32(scale=1.0, seed=1, bodies=True, **kw):
         y = 1.5 * L
         bl = [(SHAPE_LATTICE_CUBE, e, 0.5, (0.5 * L, y, 0.5 * L)),
               (SHAPE_LATTICE_CUBE, e, 0.5, (1.5 * L, y, 0.5 * L))]
-    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed, bodies=bl)
+    sc = lattice_scene(p, (L, L, L), origin=(margin,) * 3, jitter=0.05, seed=seed, bodies=bl)
     return p, sc, 1.0 / 60.0
 
 
-def dam_break_2m(scale=1.0, seed=5, **kw):
+def dam_break_2m(scale=1.0, seed=5, margin=1.0, **kw):
     """The dam break recipe scaled up for the anisotropy and field pass:
     a jittered 128^3 lattice (2,097,152 particles) at the corner of a 256^3
     box, substeps 2, I = 4, XSPH 0.05, no tension, no bodies (the scaled
@@ -49,7 +56,7 @@ def dam_break_2m(scale=1.0, seed=5, **kw):
     L = max(4, int(round(128 * scale)))
     B = 2 * L
     p = _params([B, B, B], substeps=2, iterations=4, xsph=0.05, tension=0.0, **kw)
-    sc = lattice_scene(p, (L, L, L), jitter=0.05, seed=seed)
+    sc = lattice_scene(p, (L, L, L), origin=(margin,) * 3, jitter=0.05, seed=seed)
     return p, sc, 1.0 / 60.0
 
 
@@ -60,9 +67,9 @@ def block_1m(scale=1.0, seed=2, **kw):
     return p, sc, 1.0 / 120.0
 
 
-def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
+def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, margin=1.0, **kw):
     nx, ny, nz = (max(4, int(round(v * scale))) for v in (256, 128, 128))
-    box = [2 * nx, ny * 1.5, nz]
+    box = [2 * nx, ny * 1.5, nz + 2 * margin]
     p = _params(box, substeps=2, iterations=4, xsph=0.01, tension=0.5, **kw)
     rng = np.random.default_rng(seed)
     bl = []
@@ -80,7 +87,8 @@ def block_4m_bodies(scale=1.0, seed=3, n_bodies=64, **kw):
         else:
             rad = int(rng.integers(4, 9))
             bl.append((SHAPE_LATTICE_SPHERE, max(1, round(rad * s)), dens, (cx, y, cz)))
-    sc = lattice_scene(p, (nx, ny, nz), jitter=0.05, seed=seed, bodies=bl)
+    sc = lattice_scene(p, (nx, ny, nz), origin=(margin,) * 3, jitter=0.05, seed=seed,
+                       bodies=bl)
     return p, sc, 1.0 / 120.0
 
 

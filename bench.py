@@ -87,6 +87,9 @@ def parse():
                          "the shapes a timed run picked")
     ap.add_argument("--batch", type=int, default=0, metavar="SCENES",
                     help="only the batch measurement over SCENES scenes, as its own JSON line")
+    ap.add_argument("--neighbour-skin", type=float, default=None,
+                    help="skin of the per-substep neighbour masks in units of h (0: full "
+                         "27-cell walks); default: the library's")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
     return ap.parse_args()
@@ -472,6 +475,8 @@ def headline(args):
         os.environ["P4A_TILE1"], os.environ["P4A_TILE2"] = args.tiles.split(",")
     t0 = time.time()
     p, sc, dt = workloads.WORKLOADS[args.workload](scale=args.scale)
+    if args.neighbour_skin is not None:
+        p.neighbour_skin = args.neighbour_skin
     log(f"scene {args.workload}: n={sc.n} bodies={sc.n_bodies} built in {time.time() - t0:.1f}s")
     sim = Sim(0)
     sim.reset(p, sc)

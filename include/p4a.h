@@ -87,6 +87,11 @@ typedef struct p4a_params {
   int32_t precision;         /* enum p4a_precision */
   int32_t math;              /* enum p4a_math */
   int32_t use_graph;         /* 1: replay each substep as a captured hipGraph */
+  double neighbour_skin;     /* FP32 forms: skin S of the per-substep neighbour
+                                masks, in units of h (0 <= S <= 1; 0: every
+                                solver kernel walks the full 27-cell stencil).
+                                The results are the full walk's bit for bit
+                                either way.  No counterpart in the reference. */
 } p4a_params;
 
 /* Fill `p` with defaultParams('small') (src/scene.js:14-31). */
@@ -271,6 +276,11 @@ enum p4a_buf {
   P4A_BUF_MESH_INDIRECT = 27, /* uint 4: draw-indirect arguments of the last
                               march (vertex count, 1, 0, 0) */
   P4A_BUF_BODY_PACKED = 28, /* float4 6*n_bodies: the last p4a_body_pack */
+  P4A_BUF_NEIGHBOUR_STATE = 29, /* uint 513: largest travel since the last
+                              neighbour-mask build (float bits), then the
+                              targets the last two list launches left to the
+                              full walk, in 256 counts per launch parity.
+                              Read only; no counterpart in the reference */
   P4A_BUF_COUNT_
 };
 

@@ -26,7 +26,7 @@ BUF = dict(POS=0, VEL=1, PRED=2, BODY=3, REST=4, LAMBDA=5, DENSITY=6, SLOT=7, CO
            NORMAL=9, CELL_START=10, BPOS=11, BCELL_START=12, BODY_CENTRE=13, BODY_ROT=14,
            PID=15, KEY=16, ORDER=17, BPSI=18, SMOOTH_POS=19, ANISO_G=20,
            OFS=21, CELL_OF_SLOT=22, DRIFT=23, FIELD=24, FIELD_NORMAL=25, MESH_VERTS=26,
-           MESH_INDIRECT=27, BODY_PACKED=28)
+           MESH_INDIRECT=27, BODY_PACKED=28, NEIGHBOUR_STATE=29)
 STAGE = dict(PREDICT=0, GRID=1, BODY_PRE=2, LAMBDA=3, DELTA=4, BODY_SOLVE=5, FINISH=6, STATS=7)
 
 
@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("sor_average", C.c_int32), ("surface_tension_k", C.c_double),
                 ("box", C.c_double * 3), ("no_boundary", C.c_int32),
                 ("pour_budget", C.c_int32), ("precision", C.c_int32), ("math", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("neighbour_skin", C.c_double)]
 
 
 class BodySpec(C.Structure):
@@ -363,7 +363,8 @@ class Sim:
                 "FIELD_NORMAL": (np.float32, self.field_info().n_verts, 1),
                 "MESH_VERTS": (np.uint32, 3 * self.field_info().max_triangles, 2),
                 "MESH_INDIRECT": (np.uint32, 4, 1),
-                "BODY_PACKED": (np.float32, 6 * i.n_bodies, 4)}[name]
+                "BODY_PACKED": (np.float32, 6 * i.n_bodies, 4),
+                "NEIGHBOUR_STATE": (np.uint32, 513, 1)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")

@@ -100,19 +100,30 @@ __device__ __forceinline__ float poly6_w(const Uniform& U, float r2) {
   return U.poly6 * t * t * t;
 }
 
-// Records the wave's largest per-axis displacement (p -> o) into a drift
-// slot: one wave-wide max over the active lanes, and an atomic only when it
-// raises the stored bound.  Non-negative floats order like their bits; a NaN
-// or infinite displacement stores a bound that switches culling off.
+// Records the largest per-axis displacement (p -> o) into a drift slot: an
+// atomic only from a lane that raises the stored bound.  These records run
+// in the divergent tails of the walks (lanes that left early, queued
+// targets), so they use no wave-wide reduction, which would read the
+// registers of inactive lanes.  Non-negative floats order like their bits;
+// a NaN or infinite displacement stores a bound that switches culling off.
+__device__ __forceinline__ void raise_bound(uint32_t* out, float d) {
+  const uint32_t m = __float_as_uint(d);
+  if (m > *(volatile const uint32_t*)out) atomicMax(out, m);
+}
 __device__ __forceinline__ void record_drift(uint32_t* out, float px, float py, float pz,
                                              float ox, float oy, float oz) {
   if (!out) return;
-  const float d = fmaxf(fabsf(ox - px), fmaxf(fabsf(oy - py), fabsf(oz - pz)));
-  const uint32_t m = __ockl_wfred_max_u32(__float_as_uint(d));
-  const uint64_t ex = __builtin_amdgcn_read_exec();
-  if ((uint32_t)__lane_id() == (uint32_t)__builtin_ctzll(ex) &&
-      m > *(volatile const uint32_t*)out)
-    atomicMax(out, m);
+  raise_bound(out, fmaxf(fabsf(ox - px), fmaxf(fabsf(oy - py), fabsf(oz - pz))));
+}
+
+// Records the distance of a new position from the position at the
+// neighbour-mask build into the travel bound (float bits; a NaN distance
+// stores a bound that fails every later check).
+__device__ __forceinline__ void record_travel(uint32_t* out, const float4* __restrict__ ref,
+                                              uint32_t i, float x, float y, float z) {
+  if (!out) return;
+  const float4 r = ref[i];
+  raise_bound(out, sqrtf(r2_of(x - r.x, y - r.y, z - r.z)));
 }
 
 // ---------------------------------------------------------------------------
@@ -481,6 +492,10 @@ struct TileLds {
 // no row is skipped.
 struct OpBase {
   uint32_t* driftOut = nullptr;  // drift slot a position update records into
+  // a position update after a neighbour-mask build records the largest
+  // distance of any particle from its build position (nbTravel, float bits)
+  const float4* nbRef = nullptr;
+  uint32_t* nbTravel = nullptr;
   // two-phase walks (3x3 rows): the first chunk's hit mask of each row, kept
   // for a later kernel over the same positions and grid, or taken from an
   // earlier one instead of testing the chunk again (row r of target i at
@@ -661,10 +676,20 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // lie inside h, and the full term costs ~25 issue slots against ~10 for
 // the test, so the walk issues far fewer instructions; the terms dropped
 // are the ones the branchless pair term reduces to zero.
-template <class Op, int NT, int CR, int PH = 0>
+// With LB (lambda, once per substep) the walk also builds the target's
+// neighbour masks: bit k of row r's mask marks candidate b_r + k within
+// reach (r^2 < nb->reach2), and the build position and cell go to nbRef.
+// Bit for candidate k of a row (k < 64) when r^2 is within reach.
+__device__ __forceinline__ uint64_t reach_bit(float r2, float reach2, uint32_t k) {
+  return (r2 < reach2 && k < 64u) ? (1ull << (k & 63u)) : 0ull;
+}
+
+template <class Op, int NT, int CR, int PH = 0, bool LB = false>
 __device__ __forceinline__ void
 direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restrict__ cellStart,
-            const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+            const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+            const NbArgs* nb = nullptr) {
+  static_assert(!LB || (PH == 0 && CR == 1), "masks are built by the one-phase 27-cell walk");
   cellStart += U.cBase;
   bcellStart += U.cBase;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
@@ -676,6 +701,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
                         c.z <= CR || c.z >= gz - 1 - CR;
   const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
   bool seen = false;  // the target's own record lay in a visited row
+  bool unlisted = false;
   for (int dz = -CR; dz <= CR; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
@@ -686,6 +712,8 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
       seen = seen || (i >= b && i < e);
+      uint64_t rowMask = 0;
+      if constexpr (LB) unlisted = unlisted || (e - b > 64u);
       if constexpr (Op::FAST && PH >= 1) {
         const size_t slot = (size_t)((dz + CR) * (2 * CR + 1) + (dy + CR)) * op.maskStride + i;
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
@@ -769,12 +797,21 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           float4 v[4][Op::NREC];
 #pragma unroll
           for (int u = 0; u < 4; u++) op.load(U, j + u, v[u]);
+          if constexpr (LB) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              rowMask |= reach_bit(r2_of(t.px - v[u][0].x, t.py - v[u][0].y, t.pz - v[u][0].z),
+                                   nb->reach2, (uint32_t)(j - b) + u);
+          }
 #pragma unroll
           for (int u = 0; u < 4; u++) op.fpair(U, t, v[u]);
         }
         for (; j < eEnd; j++) {
           float4 v[Op::NREC];
           op.load(U, j, v);
+          if constexpr (LB)
+            rowMask |= reach_bit(r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z), nb->reach2,
+                                 (uint32_t)(j - b));
           op.fpair(U, t, v);
         }
       } else {
@@ -784,15 +821,26 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           float4 v[4][Op::NREC];
 #pragma unroll
           for (int u = 0; u < 4; u++) op.load(U, (size_t)j + u, v[u]);
+          if constexpr (LB) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              rowMask |= reach_bit(r2_of(t.px - v[u][0].x, t.py - v[u][0].y, t.pz - v[u][0].z),
+                                   nb->reach2, j - b + u);
+          }
 #pragma unroll
           for (int u = 0; u < 4; u++) op.pair(U, t, j + u, v[u]);
         }
         for (; j < e; j++) {
           float4 v[Op::NREC];
           op.load(U, j, v);
+          if constexpr (LB)
+            rowMask |= reach_bit(r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z), nb->reach2,
+                                 j - b);
           op.pair(U, t, j, v);
         }
       }
+      if constexpr (LB)
+        nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] = rowMask;
       if (doB) {
         const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
@@ -802,6 +850,13 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
   if constexpr (Op::FAST) {
     if (!seen) op.self_outside(U, t);
   }
+  // a target whose own record is in none of its rows (it has moved more
+  // than a cell since the grid build) stays on the full walk, which adds
+  // its self term
+  if constexpr (LB)
+    nb->ref[i] = make_float4(
+        t.px, t.py, t.pz,
+        __uint_as_float(cell_index(U, c) | ((unlisted || !seen) ? NB_UNLISTED : 0u)));
   op.finish(U, i, t);
 }
 
@@ -812,6 +867,190 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (i >= U.n) return;
   direct_walk<Op, NT, CR, PH>(U, op, i, cellStart, bcellStart, bpos);
+}
+
+// The first lambda after a grid build: the full walk, building the masks.
+// It also restarts the travel bound and the queues of the list launches.
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_direct_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  if (blockIdx.x == 0)
+    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += NT) nb.ctl[k] = 0u;
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  direct_walk<Op, NT, 1, 0, true>(U, op, i, cellStart, bcellStart, bpos, &nb);
+}
+
+// The marked hits of one row half (32 candidates from `base`) in stencil
+// order, four per trip with their record reads in flight together; a
+// trip's missing hits reread its first hit and take the far sentinel (fast
+// ops, whose terms it reduces to zero) or are skipped (exact ops).  `rec`
+// reads the records of candidate j.
+template <class Op, class Rec>
+__device__ __forceinline__ void hit_loop(const Uniform& U, const Op& op,
+                                         typename Op::Target& t, uint32_t m, uint32_t base,
+                                         const Rec& rec) {
+  while (m) {
+    uint32_t jj[4];
+    bool hit[4];
+    jj[0] = base + (uint32_t)__builtin_ctz(m);
+    hit[0] = true;
+    m &= m - 1u;
+#pragma unroll
+    for (int u = 1; u < 4; u++) {
+      hit[u] = m != 0u;
+      jj[u] = hit[u] ? base + (uint32_t)__builtin_ctz(m) : jj[0];
+      m &= m - 1u;
+    }
+    float4 v[4][Op::NREC];
+#pragma unroll
+    for (int u = 0; u < 4; u++) rec(jj[u], v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if constexpr (Op::FAST) {
+#pragma unroll
+        for (int r = 0; r < Op::NREC; r++)
+          if (!hit[u]) v[u][r] = Op::sentinel(r);
+        op.fpair(U, t, v[u]);
+      } else {
+        if (hit[u]) op.pair(U, t, jj[u], v[u]);
+      }
+    }
+  }
+}
+
+// The same, one hit per trip (records read from LDS, whose latency the
+// four-wide trips would only pad).
+template <class Op, class Rec>
+__device__ __forceinline__ void hit_loop1(const Uniform& U, const Op& op,
+                                          typename Op::Target& t, uint32_t m, uint32_t base,
+                                          const Rec& rec) {
+  while (m) {
+    const uint32_t j = base + (uint32_t)__builtin_ctz(m);
+    m &= m - 1u;
+    float4 v[Op::NREC];
+    rec(j, v);
+    if constexpr (Op::FAST) op.fpair(U, t, v);
+    else op.pair(U, t, j, v);
+  }
+}
+
+// Records of one stencil row staged per wave: the union of the wave's row
+// ranges, when it holds at most NB_STAGE candidates (a wave's targets lie in
+// a few neighbouring cells of one grid row, so its rows overlap).
+constexpr uint32_t NB_STAGE = 256;
+
+// A list launch (one wave per workgroup).  Each target whose masks still
+// hold -- it is in the cell it was in at the build, and its travel since
+// the build plus the largest travel of any particle stays below the skin --
+// takes only its marked candidates of each of its 9 stencil rows, in
+// stencil order, each row followed by its wall samples as in the full
+// walk.  The wave stages each row's candidates in LDS with coalesced loads,
+// so the hits, which differ from lane to lane, are read from LDS rather
+// than gathered from memory.  Other targets are queued (one atomic per
+// wave) for k_fallback.  Block 0 empties the other queue, which the
+// previous list launch's fallback has consumed.
+template <class Op>
+__global__ void __launch_bounds__(64)
+k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+         const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  __shared__ float4 S[Op::NREC][NB_STAGE];
+  if (blockIdx.x == 0)
+    for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
+      nb.ctl[1 + (uint32_t)(nb.slot ^ 1) * NB_QSEG + k] = 0u;
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  typename Op::Target t;
+  Cell c{0, 0, 0};
+  bool valid = false;
+  if (i < U.n) {
+    op.target(U, i, t);
+    c = op.cell(U, t);
+    const float4 ref = nb.ref[i];
+    const float travel = sqrtf(r2_of(t.px - ref.x, t.py - ref.y, t.pz - ref.z));
+    const float most = __uint_as_float(*(volatile const uint32_t*)nb.ctl);
+    valid = __float_as_uint(ref.w) == cell_index(U, c) && travel + most <= nb.lim;
+    const bool queued = !valid;
+    const uint64_t qm = __ballot(queued);
+    if (qm) {
+      const int lane = __lane_id();
+      const int leader = __builtin_ctzll(qm);
+      uint32_t base = 0;
+      const uint32_t seg = blockIdx.x % NB_QSEG;
+      if (lane == leader)
+        base = atomicAdd(&nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], (uint32_t)__popcll(qm));
+      base = (uint32_t)__shfl((int)base, leader, 64);
+      const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
+      if (queued && at < nb.segCap) nb.queue[nb.slot][seg * nb.segCap + at] = i;
+    }
+  }
+  if (!__ballot(valid)) return;  // the whole wave is queued (or past n)
+  const int xlo = max(c.x - 1, 0);
+  const int xhi = min(c.x + 1, gx - 1);
+  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
+                        c.z >= gz - 2;
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && valid;
+  const int lane = __lane_id();
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const int z = c.z + dz, y = c.y + dy;
+      const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+      uint32_t b = 0, e = 0;
+      uint64_t m64 = 0;
+      if (rowOk) {
+        b = cellStart[rb + xlo];
+        e = cellStart[rb + xhi + 1];
+        m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
+      }
+      // union of the wave's ranges that hold a hit (every lane is active
+      // here, the others contribute the neutral values)
+      const bool has = m64 != 0ull;
+      const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
+      const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
+      if (ub < ue && ue - ub <= NB_STAGE) {
+        const uint32_t span = ue - ub;
+        for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
+          float4 v[Op::NREC];
+          op.load(U, (size_t)ub + q, v);
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+        }
+        __syncthreads();
+        const auto rec = [&](uint32_t j, float4* v) {
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
+        };
+        hit_loop1(U, op, t, (uint32_t)m64, b, rec);
+        // rows of more than 32 candidates are rare: the upper half only
+        // when some lane has hits there
+        if (__ballot((m64 >> 32) != 0ull)) hit_loop1(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+        __syncthreads();
+      } else if (has) {
+        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
+        hit_loop(U, op, t, (uint32_t)m64, b, rec);
+        hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+      }
+      if (doB && rowOk) {
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+      }
+    }
+  }
+  if (valid) op.finish(U, i, t);
+}
+
+// The targets a list launch queued, walked in full.
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  const uint32_t seg = blockIdx.x % NB_QSEG, part = blockIdx.x / NB_QSEG;
+  const uint32_t cnt = min(nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], nb.segCap);
+  const uint32_t* __restrict__ q = nb.queue[nb.slot] + (size_t)seg * nb.segCap;
+  for (uint32_t k = part * NT + threadIdx.x; k < cnt; k += NB_FALLBACK_PARTS * NT)
+    direct_walk<Op, NT, 1, 0>(U, op, q[k], cellStart, bcellStart, bpos);
 }
 
 // Batched scenes: each workgroup lies in one scene's 256-aligned slice of
@@ -995,6 +1234,7 @@ struct DeltaOp : OpBase {
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
     record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
+    record_travel(nbTravel, nbRef, i, ox, oy, oz);
   }
 };
 
@@ -1285,6 +1525,7 @@ struct DeltaFastT : FastBase {
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
     record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
+    record_travel(nbTravel, nbRef, i, ox, oy, oz);
   }
 };
 
@@ -1878,7 +2119,8 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
              const float4* __restrict__ restOfPid, const float* __restrict__ invRatio,
              double* __restrict__ centreD, double* __restrict__ rotD,
              float4* __restrict__ centreF, float4* __restrict__ rotF, HRec* __restrict__ ofs,
-             const uint32_t* __restrict__ cellOfSlot, Uniform U, uint32_t* __restrict__ driftOut) {
+             const uint32_t* __restrict__ cellOfSlot, Uniform U, uint32_t* __restrict__ driftOut,
+             const float4* __restrict__ nbRef, uint32_t* __restrict__ nbTravel) {
   __shared__ double sh[WG / 64][15];
   __shared__ float sc[12];
   const int b = blockIdx.x;
@@ -1959,7 +2201,7 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
   __syncthreads();
   // bodyProject (wgsl.js:763-783)
   const float ox = sc[0], oy = sc[1], oz = sc[2];
-  float dmax = 0.0f;
+  float dmax = 0.0f, tmax = 0.0f;
   for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
     const uint32_t q = bodyPids[k];
     const float4 r = restOfPid[q];
@@ -1970,6 +2212,12 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
     const float4 np = make_float4(ox + px, oy + py, oz + pz, 1.0f);
     const float4 op = pred[sl];
     dmax = fmaxf(dmax, fmaxf(fabsf(np.x - op.x), fmaxf(fabsf(np.y - op.y), fabsf(np.z - op.z))));
+    if (nbTravel) {
+      // a NaN distance must survive the maximum (fmaxf would drop it)
+      const float4 r = nbRef[sl];
+      const float d = sqrtf(r2_of(np.x - r.x, np.y - r.y, np.z - r.z));
+      tmax = (d > tmax || d != d) ? d : tmax;
+    }
     pred[sl] = np;
     if (ofs) ofs[sl] = encode_ofs(U, cellOfSlot[sl], np.x, np.y, np.z);
   }
@@ -1983,6 +2231,18 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
       uint32_t v = wmax[0];
       for (int k = 1; k < WG / 64; k++) v = max(v, wmax[k]);
       atomicMax(driftOut, v);
+    }
+  }
+  if (nbTravel) {
+    // largest distance of the body's particles from their build positions
+    __shared__ uint32_t tw[WG / 64];
+    const uint32_t m = __ockl_wfred_max_u32(__float_as_uint(tmax));
+    if (__lane_id() == 0) tw[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t v = tw[0];
+      for (int k = 1; k < WG / 64; k++) v = max(v, tw[k]);
+      atomicMax(nbTravel, v);
     }
   }
 }
@@ -2632,21 +2892,61 @@ Op drifted(Op op, const Drift& d) {
   return op;
 }
 
+// Workgroup size of the mask build and the list launches: the direct walk's
+// of the chosen shape, 64 for the LDS-tile shapes.
+int nb_threads(const TileCfg& t) { return t.variant == 0 ? 256 : t.variant == 1 ? 128 : 64; }
+
+template <class Op, int NT>
+hipError_t run_nb(const Uniform& U, const Arrays& A, const Op& op, const NbLaunch& nb,
+                  hipStream_t s) {
+  const unsigned g = (U.n + NT - 1) / NT;
+  if (nb.mode == Nb::BUILD) {
+    hipLaunchKernelGGL((k_direct_build<Op, NT>), dim3(g), dim3(NT), 0, s, U, op, nb.args,
+                       A.cellStart, A.bcellStart, A.bpos);
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL((k_listed<Op>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op, nb.args,
+                     A.cellStart, A.bcellStart, A.bpos);
+  hipLaunchKernelGGL((k_fallback<Op, NT>), dim3(NB_QSEG * NB_FALLBACK_PARTS), dim3(NT), 0, s, U,
+                     op, nb.args, A.cellStart, A.bcellStart, A.bpos);
+  return hipGetLastError();
+}
+
+// The launch of a solver op under a neighbour-mask mode (OFF: the tile
+// shape's walk).
+template <class Op>
+hipError_t run_solver(const Uniform& U, const Arrays& A, Op op, const TileCfg& t,
+                      const NbLaunch& nb, hipStream_t s, const BatchTab* B) {
+  if (B || nb.mode == Nb::OFF || U.n == 0) return run_tile1(U, A, op, t, s, B);
+  if (nb.mode == Nb::USE) {
+    op.nbRef = nb.args.ref;
+    op.nbTravel = nb.args.ctl;
+  }
+  switch (nb_threads(t)) {
+    case 256: return run_nb<Op, 256>(U, A, op, nb, s);
+    case 128: return run_nb<Op, 128>(U, A, op, nb, s);
+    default: return run_nb<Op, 64>(U, A, op, nb, s);
+  }
+}
+
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred, bool exact,
-                         const TileCfg& t, const Drift& d, hipStream_t s, const BatchTab* B) {
+                         const TileCfg& t, const Drift& d, hipStream_t s, const BatchTab* B,
+                         const NbLaunch& nb) {
   if (exact)
-    return run_tile1(U, A, drifted(LambdaOp<true>{{}, pred, A.density, A.lambda}, d), t, s, B);
-  return run_tile1(U, A, drifted(LambdaFast{{}, pred, A.density, A.lambda, (float*)pred}, d), t, s,
-                   B);
+    return run_solver(U, A, drifted(LambdaOp<true>{{}, pred, A.density, A.lambda}, d), t, nb,
+                      s, B);
+  return run_solver(U, A, drifted(LambdaFast{{}, pred, A.density, A.lambda, (float*)pred}, d), t,
+                    nb, s, B);
 }
 
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
-                        const BatchTab* B) {
-  if (exact) return run_tile1(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, s, B);
+                        const BatchTab* B, const NbLaunch& nb) {
+  if (exact)
+    return run_solver(U, A, drifted(DeltaOp<true>{{}, pred, A.lambda, out}, d), t, nb, s, B);
   if (U.sorAverage == 1u)
-    return run_tile1(U, A, drifted(DeltaFastT<true>{{}, pred, A.lambda, out}, d), t, s, B);
-  return run_tile1(U, A, drifted(DeltaFastT<false>{{}, pred, A.lambda, out}, d), t, s, B);
+    return run_solver(U, A, drifted(DeltaFastT<true>{{}, pred, A.lambda, out}, d), t, nb, s, B);
+  return run_solver(U, A, drifted(DeltaFastT<false>{{}, pred, A.lambda, out}, d), t, nb, s, B);
 }
 
 template <class Op>
@@ -2700,11 +3000,12 @@ hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred
 }
 
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             uint32_t* driftOut, hipStream_t s) {
+                             uint32_t* driftOut, hipStream_t s, bool nbTravel) {
   if (U.nBodies == 0) return hipSuccess;
   hipLaunchKernelGGL(k_body_solve, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
                      A.bodyFirst, A.restOfPid, A.bodyInvRatio, A.bodyCentreD, A.bodyRotD,
-                     A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U, driftOut);
+                     A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U, driftOut,
+                     nbTravel ? A.nbRef : nullptr, nbTravel ? A.nbCtl : nullptr);
   return hipGetLastError();
 }
 

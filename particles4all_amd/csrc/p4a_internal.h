@@ -90,6 +90,37 @@ constexpr int DRIFT_SLOTS = 256;
 // segment when it holds at most FAT_CELL of them; longer segments are
 // listed and sorted by a workgroup-wide radix sort (linear in the segment).
 constexpr uint32_t FAT_CELL = 64;
+
+// Neighbour masks of one substep.  The first lambda after a grid build walks
+// the full 27-cell stencil of every target and marks, per stencil row (its
+// 3 cells, at most 64 candidates), the candidates within reach = h + S of
+// the target (S, the skin, in units of h from the parameters).  A later
+// kernel of the substep takes only the marked candidates, in stencil order,
+// when the target is in the cell it was in at the build and its travel
+// since the build plus the largest travel of any particle is below the
+// skin: an unmarked candidate then stays at least h away, where every pair
+// term is zero, so the sums are the full walk's bit for bit.  Other targets
+// are queued and walked in full by a second launch.
+struct NbArgs {
+  uint64_t* mask;
+  float4* ref;
+  uint32_t* ctl;
+  uint32_t* queue[2];
+  uint32_t stride;
+  uint32_t segCap;  // entries per queue segment
+  float reach2;  // (h + S)^2 (build)
+  float lim;     // S less a rounding margin (use)
+  int slot;      // queue of this launch
+};
+// Each list launch appends the targets it leaves to the full walk to one
+// of NB_QSEG queue segments (by workgroup), so the appends spread over
+// NB_QSEG counters instead of contending for one.
+constexpr uint32_t NB_QSEG = 256;
+inline uint32_t nb_seg_cap(uint32_t n) { return ((n + 63u) / 64u + NB_QSEG - 1u) / NB_QSEG * 64u; }
+constexpr uint32_t NB_CTL_WORDS = 1 + 2 * NB_QSEG;
+// nbRef.w bit of a target with a stencil row of more than 64 candidates
+constexpr uint32_t NB_UNLISTED = 0x80000000u;
+constexpr int NB_FALLBACK_PARTS = 4;  // workgroups per queue segment
 inline size_t fat_list_len(size_t cap) { return cap / (FAT_CELL + 1) + 2; }
 struct Drift {
   uint32_t* out = nullptr;  // slot the launch records into (null: none)
@@ -122,6 +153,14 @@ struct Arrays {
   // hitMask[r * hitStride + i]
   uint32_t* hitMask;
   uint32_t hitStride;
+  // per-substep neighbour masks (FP32 forms): built by the first lambda
+  // after a grid build, read by the later solver kernels of the substep
+  uint64_t* nbMask;     // 9 x nbStride: row r's candidates within reach of target i
+  float4* nbRef;        // position at the build, w = cell at the build (bits)
+  uint32_t* nbCtl;      // [0] largest travel since the build, then the
+                        // queue counts: NB_QSEG segments per launch parity
+  uint32_t* nbQueue[2]; // targets a list launch left to the full walk
+  uint32_t nbStride;
   HRec* ofs[2];           // FP16 form: offsets of pred[parity] / predX
   uint32_t* drift;      // DRIFT_SLOTS displacement bounds (float bits)
   uint32_t* cellOfSlot; // cell each sorted slot was binned into
@@ -222,16 +261,24 @@ hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred
 hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
                           const Drift& d, hipStream_t s);
+// How a solver launch uses the neighbour masks: not at all, building them
+// (lambda after a grid build), or taking them (with the queued targets'
+// full walks in a second launch; delta also records travel).
+enum class Nb { OFF, BUILD, USE };
+struct NbLaunch {
+  Nb mode = Nb::OFF;
+  NbArgs args{};
+};
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
                          bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
-                         const BatchTab* B = nullptr);
+                         const BatchTab* B = nullptr, const NbLaunch& nb = NbLaunch{});
 hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, float4* out,
                         bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
-                        const BatchTab* B = nullptr);
+                        const BatchTab* B = nullptr, const NbLaunch& nb = NbLaunch{});
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s);
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             uint32_t* driftOut, hipStream_t s);
+                             uint32_t* driftOut, hipStream_t s, bool nbTravel = false);
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
                        hipStream_t s, const BatchTab* B = nullptr);

@@ -327,6 +327,7 @@ P4A_API int p4a_params_default(p4a_params* p) {
   p->precision = P4A_PREC_FP32;
   p->math = P4A_MATH_FAST;
   p->use_graph = 1;
+  p->neighbour_skin = 0.1;
   return P4A_OK;
 }
 

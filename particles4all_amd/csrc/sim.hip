@@ -132,6 +132,12 @@ struct p4a_sim {
   std::string err;
   bool ready = false;
   bool ofsValid = false;  // FP16 records match the current pred buffers
+  // neighbour masks: built since the last grid build or position change,
+  // list launches since the build (their queue alternates)
+  bool nbBuilt = false;
+  int nbLaunch = 0;
+  bool nbBuiltAfter[2] = {false, false};  // the host state a captured substep leaves
+  int nbLaunchAfter[2] = {0, 0};
   int driftCount = 0;     // drift slots written since the last grid build
   p4a_params params{};
   // scene
@@ -257,6 +263,9 @@ void free_list(std::vector<void*>& v) {
 }
 
 void drop_graphs(p4a_sim* s) {
+  // every caller changes positions, the grid or the parameters behind the
+  // masks' back
+  s->nbBuilt = false;
   for (int k = 0; k < 2; k++) {
     if (s->graph[k]) hipGraphExecDestroy(s->graph[k]);
     s->graph[k] = nullptr;
@@ -329,6 +338,8 @@ const char* form_error(const p4a_params* p) {
     return "unknown precision";
   if (p->precision == P4A_PREC_FP16_OFFSET && p->math == P4A_MATH_EXACT)
     return "the FP16-offset form runs with P4A_MATH_FAST only";
+  if (!(p->neighbour_skin >= 0.0 && p->neighbour_skin <= 1.0))
+    return "neighbour_skin must lie in [0, 1]";
   return nullptr;
 }
 bool fp16_form(const p4a_sim* s) { return s->params.precision == P4A_PREC_FP16_OFFSET; }
@@ -354,6 +365,45 @@ uint32_t* drift_slot(const p4a_sim* s) {
   return s->driftCount < DRIFT_SLOTS ? s->A.drift + s->driftCount : nullptr;
 }
 
+bool nb_on(const p4a_sim* s) { return !fp16_form(s) && s->params.neighbour_skin > 0.0; }
+
+// Mask arguments of the next solver launch: BUILD for the first lambda
+// after the masks were dropped, USE once built, OFF otherwise.
+NbLaunch nb_launch(p4a_sim* s, bool lambda) {
+  NbLaunch L;
+  if (!nb_on(s) || (!s->nbBuilt && !lambda)) return L;
+  const double S = s->params.neighbour_skin * s->h;
+  NbArgs& a = L.args;
+  a.mask = s->A.nbMask;
+  a.ref = s->A.nbRef;
+  a.ctl = s->A.nbCtl;
+  a.queue[0] = s->A.nbQueue[0];
+  a.queue[1] = s->A.nbQueue[1];
+  a.stride = s->A.nbStride;
+  a.segCap = nb_seg_cap((uint32_t)s->n);
+  a.reach2 = (float)((s->h + S) * (s->h + S));
+  // a margin of 1e-4 h covers the float rounding of the distances at the
+  // build and at the use and of the travels
+  a.lim = (float)(S - 1e-4 * s->h);
+  if (!s->nbBuilt) {
+    L.mode = Nb::BUILD;
+    a.slot = 0;
+  } else {
+    L.mode = Nb::USE;
+    a.slot = s->nbLaunch & 1;
+  }
+  return L;
+}
+
+void nb_advance(p4a_sim* s, const NbLaunch& L) {
+  if (L.mode == Nb::BUILD) {
+    s->nbBuilt = true;
+    s->nbLaunch = 0;
+  } else if (L.mode == Nb::USE) {
+    s->nbLaunch++;
+  }
+}
+
 int launch_lambda_any(p4a_sim* s, const Uniform& U) {
   const Drift d;
   if (fp16_form(s)) {
@@ -361,8 +411,10 @@ int launch_lambda_any(p4a_sim* s, const Uniform& U) {
     LAUNCH(s, "lambda", launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur),
                                         direct_threads(s->tile1), d, s->stream));
   } else {
+    const NbLaunch nb = nb_launch(s, true);
     LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, d,
-                                      s->stream));
+                                      s->stream, nullptr, nb));
+    nb_advance(s, nb);
   }
   return P4A_OK;
 }
@@ -375,8 +427,10 @@ int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
     LAUNCH(s, "delta", launch_delta16(U, s->A, s->predCur, ofs_of(s, s->predCur), out,
                                       ofs_of(s, out), direct_threads(s->tile1), d, s->stream));
   } else {
+    const NbLaunch nb = nb_launch(s, false);
     LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1, d,
-                                    s->stream));
+                                    s->stream, nullptr, nb));
+    nb_advance(s, nb);
     s->ofsValid = false;
   }
   s->driftCount++;
@@ -387,7 +441,8 @@ int launch_body_solve_any(p4a_sim* s, const Uniform& U) {
   if (!fp16_form(s)) s->ofsValid = false;
   LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur,
                                             fp16_form(s) ? ofs_of(s, s->predCur) : nullptr,
-                                            drift_slot(s), s->stream));
+                                            drift_slot(s), s->stream,
+                                            nb_on(s) && s->nbBuilt));
   s->driftCount++;
   return P4A_OK;
 }
@@ -410,6 +465,7 @@ int run_grid(p4a_sim* s, const Uniform& U, bool moveVel) {
   LAUNCH(s, "scan", launch_scan(U, s->A, s->stream));
   LAUNCH(s, "scatter", launch_scatter(U, s->A, par, moveVel, fp16_form(s), s->stream));
   s->ofsValid = fp16_form(s);
+  s->nbBuilt = false;
   s->driftCount = 0;
   s->parity ^= 1;
   s->predCur = s->A.pred[s->parity];
@@ -457,7 +513,7 @@ std::string graph_key(const p4a_sim* s, const Uniform& U) {
   k += std::to_string(s->params.iterations) + "/" + std::to_string(s->params.math) + "/" +
        std::to_string(s->params.precision) + "/" +
        std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.variant) +
-       "/" + std::to_string(s->tile2.variant);
+       "/" + std::to_string(s->tile2.variant) + "/" + std::to_string(s->params.neighbour_skin);
   return k;
 }
 
@@ -499,10 +555,14 @@ int run_substep(p4a_sim* s, double dt) {
       // capture only recorded the work; the host-side state advanced as if
       // it ran, so rewind it and replay the graph below.
       s->parity = par0;
+      s->nbBuiltAfter[par0] = s->nbBuilt;
+      s->nbLaunchAfter[par0] = s->nbLaunch;
     }
     HIP_TRY(s, hipGraphLaunch(s->graph[par0], s->stream));
     // host-side bookkeeping of what the graph does
     s->parity = par0 ^ 1;
+    s->nbBuilt = s->nbBuiltAfter[par0];
+    s->nbLaunch = s->nbLaunchAfter[par0];
     const int iters = std::max(1, s->params.iterations);
     s->predCur = (iters % 2 == 0) ? s->A.pred[s->parity] : s->A.predX;
   }
@@ -605,6 +665,7 @@ BufView buf_view(p4a_sim* s, int id) {
     case P4A_BUF_ANISO_G: return {s->anisoG, s->anisoG ? n * 32 : 0};
     case P4A_BUF_OFS: return {ofs_of(s, s->predCur), n * 8};
     case P4A_BUF_CELL_OF_SLOT: return {s->A.cellOfSlot, n * 4};
+    case P4A_BUF_NEIGHBOUR_STATE: return {s->A.nbCtl, (size_t)NB_CTL_WORDS * 4};
     case P4A_BUF_DRIFT: return {s->A.drift, (size_t)DRIFT_SLOTS * 4};
     case P4A_BUF_FIELD: return {s->field, s->field ? s->fieldVerts * 4 : 0};
     case P4A_BUF_FIELD_NORMAL: return {s->nfield, s->nfield ? s->fieldVerts * 4 : 0};
@@ -801,6 +862,12 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   ALLOC(A.key, cap);
   ALLOC(A.tmp, cap);
   ALLOC(A.fatList, fat_list_len(cap));
+  ALLOC(A.nbMask, cap * 9);
+  A.nbStride = (uint32_t)cap;
+  ALLOC(A.nbRef, cap);
+  ALLOC(A.nbCtl, NB_CTL_WORDS);
+  ALLOC(A.nbQueue[0], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
+  ALLOC(A.nbQueue[1], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
   ALLOC(A.order, cap);
   ALLOC(A.slot, cap);
   ALLOC(A.slotOf, cap);
@@ -955,6 +1022,8 @@ P4A_API int p4a_set_params(p4a_sim* s, const p4a_params* p) {
   q.math = p->math;
   q.precision = p->precision;
   q.use_graph = p->use_graph;
+  q.neighbour_skin = p->neighbour_skin;
+  s->nbBuilt = false;
   return P4A_OK;
 }
 
@@ -1022,6 +1091,7 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
         LAUNCH(s, "body_drag", launch_body_drag(U, s->A, s->A.vel[s->parity], D, s->stream));
       }
       LAUNCH(s, "predict", launch_predict_count(U, s->A, s->parity, s->stream));
+      s->nbBuilt = false;
       break;
     case P4A_STAGE_GRID:
       if ((rc = run_grid(s, U, true))) return rc;
@@ -1043,6 +1113,7 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
       break;
     case P4A_STAGE_FINISH:
       if ((rc = run_finish(s, U))) return rc;
+      s->nbBuilt = false;
       s->simTime += dt;
       break;
     case P4A_STAGE_STATS:
@@ -1236,7 +1307,7 @@ P4A_API int p4a_write(p4a_sim* s, int32_t id, const void* src, size_t offset, si
   // them without bounds checks, so only the library writes them
   if (id == P4A_BUF_BODY || id == P4A_BUF_REST || id == P4A_BUF_PID || id == P4A_BUF_ORDER ||
       id == P4A_BUF_KEY || id == P4A_BUF_SLOT || id == P4A_BUF_CELL_START ||
-      id == P4A_BUF_BCELL_START || id == P4A_BUF_CELL_OF_SLOT)
+      id == P4A_BUF_BCELL_START || id == P4A_BUF_CELL_OF_SLOT || id == P4A_BUF_NEIGHBOUR_STATE)
     return fail(s, P4A_ERR_ARG, "buffer is not writable");
   // particle positions and velocities stay finite (a non-finite position
   // would be binned into a clamped corner cell)

@@ -57,7 +57,7 @@ def test_phase_export_follows_the_sort():
     sim.substeps(1 / 120, 2)
     d0 = check_phase(sim)
     assert sim.info().n_bodies > 0 and np.count_nonzero(d0[:, 0]) == sim.info().n_body_parts
-    sim.substeps(1 / 120, 3)
+    sim.substeps(1 / 120, 60)
     d1 = check_phase(sim)
     assert not np.array_equal(d0, d1)  # the sort moved particles
     sim.run_stage("PREDICT", 1 / 120)

@@ -167,6 +167,10 @@ def test_stages_against_fp64_oracle_on_quantized_positions():
     den_q, lam_q = O.lambda_(O.F64, P, q, cs, bpos, bpsi, bcs)
     e_d = np.abs(gden - den_q) / den_q
     e_l = np.abs(glam - lam_q)
+    # near rest, lambda carries the float error of rho/rho0 - 1: the bar
+    # takes the float chain's own distance from FP64 on these inputs
+    _, lam_q32 = O.lambda_(O.F32, P, q, cs, bpos, bpsi, bcs)
+    chain = np.abs(lam_q32 - lam_q)
     lmax = np.abs(lam_q).max()
     print(f"density rel: interior max {e_d[inner].max():.3e}, all max {e_d.max():.3e}; "
           f"lambda: interior max {e_l[inner].max():.3e}, all max {e_l.max():.3e} "
@@ -174,7 +178,8 @@ def test_stages_against_fp64_oracle_on_quantized_positions():
     # interior: fluid pairs only, distances exact -> fast-math bars; walls:
     # the absolute quantized position is rounded to float (<= 4e-6 here)
     assert e_d[inner].max() <= 2e-6 and e_d.max() <= 2e-5
-    assert e_l[inner].max() <= 1e-5 * lmax and e_l.max() <= 1e-4 * lmax
+    assert e_l[inner].max() <= max(1e-5 * lmax, 4 * chain[inner].max())
+    assert e_l.max() <= max(1e-4 * lmax, 4 * chain.max())
     # density error of the form against the FP64 oracle at the FP32 positions
     den_t, _ = O.lambda_(O.F64, P, pred, cs, bpos, bpsi, bcs)
     err = np.abs(gden - den_t) / P.rho0

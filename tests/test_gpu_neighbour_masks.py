@@ -1,0 +1,119 @@
+"""Per-substep neighbour masks against the full 27-cell walk, through the
+C ABI.  The masks change which candidates a kernel visits, never a sum: a
+simulation stepped with masks must equal the same simulation stepped with
+full walks bit for bit, in both math forms, with bodies, with targets that
+leave their cells (queued for the full walk) and with a skin so small that
+most targets are queued.
+
+* dam_break_32 and a quarter-scale block_4m_bodies: 12 substeps with skins
+  0.1 and 0.3 against skin 0 (positions, velocities, densities, lambdas,
+  particle ids, body poses), fast and exact math;
+* skin 1e-4: nearly every target is queued after the first delta, and the
+  result is still the full walk's;
+* the mask state: the travel bound equals the largest distance of any
+  particle from its build position after the deltas (read back stage by
+  stage), and the queues hold a few percent of the targets at most on the
+  flowing state;
+* skins outside [0, 1] and non-finite skins are refused.
+"""
+import numpy as np
+import pytest
+
+from particles4all_amd import Sim, workloads
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("POS", "VEL", "DENSITY", "LAMBDA", "PID")
+
+
+def run(maker, skin, math, steps, **kw):
+    p, sc, _ = maker(math=math, **kw)
+    p.neighbour_skin = skin
+    sim = Sim(0)
+    sim.reset(p, sc)
+    dt = 1 / 120
+    sim.substeps(dt, steps)
+    out = {k: sim.read(k).copy() for k in FIELDS}
+    if sc.n_bodies:
+        out["BODY_CENTRE"] = sim.read("BODY_CENTRE").copy()
+        out["BODY_ROT"] = sim.read("BODY_ROT").copy()
+    st = sim.read("NEIGHBOUR_STATE")
+    sim.close()
+    return out, st
+
+
+def same(a, b, what):
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), (what, k)
+
+
+@pytest.mark.parametrize("math", [0, 1])
+@pytest.mark.parametrize("maker,kw", [(workloads.dam_break_32, {}),
+                                      (workloads.block_4m_bodies, {"scale": 0.25})])
+def test_masks_equal_the_full_walk(math, maker, kw):
+    full, st0 = run(maker, 0.0, math, 12, **kw)
+    assert st0[0] == 0 and not st0[1:].any()  # no list launch ran
+    for skin in (0.1, 0.3, 1e-4):
+        got, st = run(maker, skin, math, 12, **kw)
+        queued = int(st[1:].sum())
+        travel = float(st[:1].view(np.float32)[0])
+        print(f"math {math} {maker.__name__} skin {skin}: last queues {queued} of "
+              f"{len(got['PID'])}, travel {travel:.3e}")
+        same(full, got, f"skin {skin}")
+        if skin == 1e-4:
+            assert queued > 0.5 * len(got["PID"])
+
+
+def test_travel_bound_and_queues_on_a_flowing_state():
+    p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
+    p.neighbour_skin = 0.0
+    sim = Sim(0)
+    sim.reset(p, sc)
+    sim.substeps(dt, 120)
+    p.neighbour_skin = 0.1
+    sim.set_params(p)
+    n = sim.info().n
+    sim.run_stage("PREDICT", dt)
+    sim.run_stage("GRID", dt)
+    sim.run_stage("BODY_PRE", dt)
+    sim.run_stage("LAMBDA", dt)
+    st = sim.read("NEIGHBOUR_STATE")
+    assert not st.any()  # the build restarts the bound and the queues
+    p0 = sim.read("PRED")[:, :3].astype(np.float64)
+    most = 0.0
+    for it in range(p.iterations):
+        if it:
+            sim.run_stage("LAMBDA", dt)
+        sim.run_stage("DELTA", dt)
+        q = sim.read("PRED")[:, :3].astype(np.float64)
+        most = max(most, float(np.linalg.norm(q - p0, axis=1).max()))
+        sim.run_stage("BODY_SOLVE", dt)  # moves the body particles again
+        q = sim.read("PRED")[:, :3].astype(np.float64)
+        most = max(most, float(np.linalg.norm(q - p0, axis=1).max()))
+        st = sim.read("NEIGHBOUR_STATE")
+        bound = float(st[:1].view(np.float32)[0])
+        queued = int(st[1:].sum())
+        print(f"iteration {it}: travel bound {bound:.4e}, largest travel {most:.4e}, "
+              f"queued {queued} of {n}")
+        # the float distance computed on the GPU, against FP64 of the same
+        # float positions
+        assert abs(bound - most) <= 1e-6 * max(most, 1.0)
+        assert queued <= 0.05 * n
+    sim.run_stage("FINISH", dt)
+    sim.close()
+
+
+def test_bad_skins_are_refused():
+    p, sc, _ = workloads.dam_break_32(scale=0.25)
+    sim = Sim(0)
+    for bad in (-0.1, 1.5, float("nan"), float("inf")):
+        p.neighbour_skin = bad
+        with pytest.raises(RuntimeError, match="P4A_ERR_ARG"):
+            sim.reset(p, sc)
+    p.neighbour_skin = 0.1
+    sim.reset(p, sc)
+    p.neighbour_skin = -1.0
+    with pytest.raises(RuntimeError, match="P4A_ERR_ARG"):
+        sim.set_params(p)
+    sim.substeps(1 / 120, 2)
+    sim.close()

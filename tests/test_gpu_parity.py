@@ -163,10 +163,17 @@ def test_fast_math_stages_within_tolerance():
     pred = rd(sim, "PRED")
     sim.run_stage("LAMBDA", dt)
     den, lam = O.lambda_(O.F64, P, pred, cs, bpos, bpsi, bcs)
+    # the float chain (the reference's FP32 arithmetic) on the same inputs:
+    # lambda = -C / D with C = rho/rho0 - 1 a difference of nearly equal
+    # numbers, so a near-rest state carries float error well above
+    # 1e-5 max|lambda|
+    _, lam32 = O.lambda_(O.F32, P, pred, cs, bpos, bpsi, bcs)
+    chain = np.abs(lam32 - lam).max()
     glam = rd(sim, "LAMBDA")
     e_l = _err(glam, lam)
-    print(f"lambda vs FP64: max {e_l[0]:.3e} rms {e_l[1]:.3e} (max|lambda| {np.abs(lam).max():.3e})")
-    assert e_l[0] <= 1e-5 * np.abs(lam).max()
+    print(f"lambda vs FP64: max {e_l[0]:.3e} rms {e_l[1]:.3e} (max|lambda| {np.abs(lam).max():.3e}, "
+          f"float chain {chain:.3e})")
+    assert e_l[0] <= max(1e-5 * np.abs(lam).max(), 4 * chain)
     np.testing.assert_allclose(rd(sim, "DENSITY"), den, rtol=2e-6)
     sim.run_stage("DELTA", dt)
     out = O.delta(O.F64, P, pred, glam, cs, bpos, bpsi, bcs)
@@ -269,18 +276,25 @@ def test_one_substep_against_fp64_oracle(math):
     solver iterations (1e-4 h), with the bulk within 1e-5 h."""
     sim, p, sc, dt, P, st, pid0, inv, bnd, rot = _substep_pair(math)
     h = sc.h
+    st32 = O.State(st.pos, st.vel, st.body_of, st.rest)
+    info32 = O.substep(O.F32, P, st32, bnd, np.array(sc.body_centre, F), rot.copy(), inv,
+                       p.iterations, p.surface_tension_k > 0)
     info = O.substep(O.F64, P, st, bnd, np.array(sc.body_centre, F), rot, inv, p.iterations,
                      p.surface_tension_k > 0)
+    assert np.array_equal(info32["order"], info["order"])
+    chain = np.abs(info32["lam"] - info["lam"]).max()
     sim.substeps(dt, 1)
     assert np.array_equal(sim.read("PID"), pid0[info["order"]])
     gp, gv, gl = rd(sim, "POS")[:, :3], rd(sim, "VEL")[:, :3], rd(sim, "LAMBDA")
     ep, ev, el = _err(gp, st.pos[:, :3]), _err(gv, st.vel[:, :3]), _err(gl, info["lam"])
     frac = float(np.mean(np.abs(gp - st.pos[:, :3]) <= 1e-5 * h))
     print(f"math={math}: pos max {ep[0]:.3e} rms {ep[1]:.3e} (within 1e-5 h: {frac:.6f}); "
-          f"vel max {ev[0]:.3e} rms {ev[1]:.3e}; lambda max {el[0]:.3e} rms {el[1]:.3e}")
+          f"vel max {ev[0]:.3e} rms {ev[1]:.3e}; lambda max {el[0]:.3e} rms {el[1]:.3e} "
+          f"(float chain {chain:.3e})")
     assert ep[0] <= 1e-4 * h and ep[1] <= 1e-6 * h and frac > 0.999
     assert ev[0] <= 1e-4 * h / dt
-    assert el[0] <= 1e-4 * np.abs(info["lam"]).max()
+    # near rest lambda carries the float error of rho/rho0 - 1 (see above)
+    assert el[0] <= max(1e-4 * np.abs(info["lam"]).max(), 4 * chain)
     sim.close()
 
 

@@ -309,6 +309,20 @@ P4A_API int p4a_get_info(p4a_sim* h, p4a_info* out);
  * relative to rho0), maxSpeed, kinetic energy; reduced exactly in FP64. */
 P4A_API int p4a_stats(p4a_sim* h, double out[4]);
 
+/* The stats reduction of p4a_stats on the current live state (densities of
+ * the last lambda, live velocities), written to caller device memory
+ * `out_dev` (4 doubles: avgRho, maxRho, maxSpeed, kinetic energy) and
+ * enqueued on the handle's stream without waiting.  It allocates nothing:
+ * its per-block partial sums live in a fixed device array of the library
+ * (512 KiB on each GPU it runs on, held until the process ends), so callers
+ * that cannot allocate (code under HIP graph capture, serving loops) can
+ * call it.  No counterpart in the reference.  Handles with more than
+ * P4A_STATS_NOALLOC_MAX_PARTICLES particles are refused (P4A_ERR_RANGE).
+ * Calls on one GPU share the array: calls that overlap on one GPU (two
+ * streams at once) race. */
+#define P4A_STATS_NOALLOC_MAX_PARTICLES (1u << 25)
+P4A_API int p4a_stats_noalloc(p4a_sim* h, double* out_dev);
+
 /* Sim.bodyPose (sim.js:635-655): centre nBodies*3, rot nBodies*9
  * (column-major, as the reference's p.rot). */
 P4A_API int p4a_body_pose(p4a_sim* h, float* centre, float* rot);

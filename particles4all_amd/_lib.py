@@ -105,7 +105,7 @@ EXPORTS = ["p4a_params_default", "p4a_scene_build", "p4a_scene_view", "p4a_scene
            "p4a_snapshot_size", "p4a_snapshot_save", "p4a_snapshot_load",
            "p4a_batch_create", "p4a_batch_destroy",
            "p4a_batch_last_error", "p4a_batch_substeps", "p4a_batch_tune", "p4a_batch_sync",
-           "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats"]
+           "p4a_batch_scene_size", "p4a_batch_read", "p4a_batch_stats", "p4a_stats_noalloc"]
 
 
 def lib():
@@ -138,6 +138,7 @@ def lib():
     L.p4a_apply_ray_impulse.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.c_double, C.c_double]
     L.p4a_hold_body.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.c_double, C.c_double]
+    L.p4a_stats_noalloc.argtypes = [vp, vp]
     L.p4a_release_body.argtypes = [vp]
     L.p4a_read.argtypes = [vp, C.c_int32, vp, C.c_size_t, C.c_size_t]
     L.p4a_field_configure.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),

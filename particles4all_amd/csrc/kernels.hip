@@ -3097,6 +3097,15 @@ hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hi
   return hipGetLastError();
 }
 
+hipError_t launch_stats_into(const Uniform& U, const float* density, const float4* vel,
+                             double* part, double* out, hipStream_t s) {
+  const int nb = stats_blocks(U.n);
+  if (nb > 0)
+    hipLaunchKernelGGL(k_stats_partial, dim3(nb), dim3(WG), 0, s, U, density, vel, part);
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(WG), 0, s, part, nb, U.n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,
                             hipStream_t s) {
   if (U.nBodies == 0 || d.held < 1 || d.held > U.nBodies) return hipSuccess;

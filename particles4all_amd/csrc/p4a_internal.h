@@ -289,6 +289,10 @@ hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred
                            float4* vel, hipStream_t s, const BatchTab* B = nullptr);
 hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,
                               const float4* pos, hipStream_t s, const BatchTab* B = nullptr);
+// The stats reduction with caller-given partials (stats_blocks(n) x 4
+// doubles) and output (4 doubles).
+hipError_t launch_stats_into(const Uniform& U, const float* density, const float4* vel,
+                             double* part, double* out, hipStream_t s);
 hipError_t launch_stats(const Uniform& U, const Arrays& A, const float4* vel, hipStream_t s,
                         const BatchTab* B = nullptr);
 hipError_t launch_body_drag(const Uniform& U, const Arrays& A, float4* vel, const DragArgs& d,

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1368,6 +1369,53 @@ P4A_API int p4a_stats(p4a_sim* s, double out[4]) {
   if (!out) return P4A_ERR_ARG;
   HIP_TRY(s, hipStreamSynchronize(s->stream));
   for (int k = 0; k < 4; k++) out[k] = s->statsHost[k];
+  return P4A_OK;
+}
+
+// ---- the stats reduction without allocation --------------------------------
+// The per-block partials of p4a_stats_noalloc live in this fixed array, one
+// copy per GPU for the life of the process.  Host code never names it
+// directly: its device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+constexpr uint32_t kNoAllocMaxParticles = P4A_STATS_NOALLOC_MAX_PARTICLES;
+constexpr size_t kNoAllocPartials = (size_t)((kNoAllocMaxParticles + 2047u) / 2048u) * 4u;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device `device`.
+int scratch_on(p4a_sim* s, int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(s, P4A_ERR_RANGE, "device ordinal too large");
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(s, P4A_ERR_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch[device] = (double*)ptr;
+  }
+  *out = g_scratch[device];
+  return P4A_OK;
+}
+}  // namespace
+
+P4A_API int p4a_stats_noalloc(p4a_sim* s, double* out_dev) {
+  int rc = check(s);
+  if (rc) return rc;
+  if (!out_dev) return fail(s, P4A_ERR_ARG, "out_dev is null");
+  if ((uint32_t)s->n > kNoAllocMaxParticles)
+    return fail(s, P4A_ERR_RANGE, "p4a_stats_noalloc takes at most "
+                                  "P4A_STATS_NOALLOC_MAX_PARTICLES particles");
+  HIP_TRY(s, hipSetDevice(s->device));
+  double* part = nullptr;
+  if ((rc = scratch_on(s, s->device, &part))) return rc;
+  const Uniform U = make_uniform(s, 0.0);
+  const hipError_t e = launch_stats_into(U, s->A.density, s->A.vel[s->parity], part, out_dev,
+                                         s->stream);
+  if (e != hipSuccess) return fail(s, P4A_ERR_HIP, std::string("stats: ") + hipGetErrorString(e));
   return P4A_OK;
 }
 

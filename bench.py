@@ -520,6 +520,10 @@ def headline(args):
             "algorithmic_bytes_per_particle": ALGO_BYTES[dom], "particles_per_launch": n,
             "mean_launch_ms": dom_ms, "duration_source": "HIP events on the simulation "
                                                          "stream, this run",
+            "launch": "one solver stage: the mask build (first lambda after a grid build) "
+                      "or the list walk plus its fallback walk (later stages); traffic and "
+                      "profile duration per stage from profiles/" + args.workload +
+                      "_fp32.md",
             "copy_probe_GBps": copy_gbs}
     if prof_us:
         roof["profile_mean_launch_ms"] = prof_us / 1e3

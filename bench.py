@@ -24,7 +24,9 @@ arithmetic.  Beside it, on the same state:
 * accuracy of one substep from the flowing state: the fast, exact and
   FP16-offset forms against the oracle's float chain (the reference's FP32
   arithmetic) and against its FP64 evaluation (positions, velocities,
-  lambdas, densities);
+  lambdas, densities), and the float chain with every sqrt and division one
+  ulp up (within WGSL's accuracy for those operations) against the float
+  chain: how far two conforming runs of the reference may differ;
 * `batch`: 256 independent scenes stepped as one batch against the same
   scenes stepped one by one;
 * `field`: the anisotropy pass and the anisotropic density field of the
@@ -244,7 +246,7 @@ def accuracy(sim, p, sc, dt):
     tension = p.surface_tension_k > 0
     ref = {}
     cpu_s = None
-    for name, mode in (("float", O.F32), ("fp64", O.F64)):
+    for name, mode in (("float", O.F32), ("fp64", O.F64), ("float_ulp", O.F32_ULP)):
         st = O.State(pos0, vel0, body0, rest0)
         t0 = time.perf_counter()
         info = O.substep(mode, P, st, bnd, centre.copy(), rot.copy(), inv, p.iterations,
@@ -304,6 +306,18 @@ def accuracy(sim, p, sc, dt):
         "lambda": _err(R["float"][2], R["fp64"][2], lam_scale),
         "pos_within_1e-5_h": float(
             (np.abs(R["float"][0] - R["fp64"][0]).max(axis=1) <= 1e-5 * sc.h).mean())}
+    # a conforming implementation of the reference's FP32 arithmetic: WGSL
+    # allows its f32 sqrt and division more than one ulp, so the float chain
+    # with every sqrt and division of the neighbour kernels one ulp up is as
+    # much the reference as the float chain is
+    out["float_chain_one_ulp_vs_float_chain"] = {
+        "pos_h": _err(R["float_ulp"][0], R["float"][0], sc.h),
+        "vel_h_per_substep": _err(R["float_ulp"][1] * dt, R["float"][1] * dt, sc.h),
+        "lambda": _err(R["float_ulp"][2], R["float"][2], lam_scale),
+        "pos_within_1e-5_h": float(
+            (np.abs(R["float_ulp"][0] - R["float"][0]).max(axis=1) <= 1e-5 * sc.h).mean())}
+    log("float chain with sqrt and division one ulp up vs the float chain: pos max "
+        f"{out['float_chain_one_ulp_vs_float_chain']['pos_h']['max']:.3e} h")
     sim.load_snapshot(snap)
     set_form(sim, p, 0, 0)
     return out, cpu_s

@@ -11,7 +11,15 @@
 //                        the reference's uniform buffer stores them;
 //   mode 1 (R = double): the same sequence in FP64 with FP64 constants -
 //                        the FP64 evaluation the GPU results are measured
-//                        against.
+//                        against;
+//   mode 2 (R = float):  mode 0 with every sqrt and division of the
+//                        neighbour kernels (lambda, delta, xsph, tension)
+//                        one ulp above the correctly rounded result.  WGSL
+//                        gives its f32 division 2.5 ULP and its sqrt (as
+//                        1/inverseSqrt) more, so this is one conforming
+//                        implementation of the reference's arithmetic: its
+//                        distance from mode 0 is a floor for how far two
+//                        conforming runs of the reference may differ.
 // r^2 is fma(z,z, fma(y,y, x*x)) in both modes (the WGSL `dot` leaves the
 // contraction to the WebGPU compiler; this fixes one choice).
 // Rigid-body sums and the polar decomposition are FP64 in both modes; the
@@ -68,8 +76,23 @@ struct K {  // constants in mode R
 
 inline float fma_r(float a, float b, float c) { return fmaf(a, b, c); }
 inline double fma_r(double a, double b, double c) { return fma(a, b, c); }
-inline float sqrt_r(float a) { return sqrtf(a); }
+// mode 2's shift (read-only while a kernel runs)
+int g_ulpUp = 0;
+inline float sqrt_r(float a) {
+  const float r = sqrtf(a);
+  return g_ulpUp ? nextafterf(r, INFINITY) : r;
+}
 inline double sqrt_r(double a) { return sqrt(a); }
+inline float div_r(float a, float b) {
+  const float r = a / b;
+  return g_ulpUp ? nextafterf(r, INFINITY) : r;
+}
+inline double div_r(double a, double b) { return a / b; }
+// float for modes 0 and 2 (setting mode 2's shift), double for mode 1
+bool float_mode(int mode) {
+  g_ulpUp = mode == 2;
+  return mode != 1;
+}
 
 template <class R>
 inline R r2_of(R x, R y, R z) { return fma_r(z, z, fma_r(y, y, x * x)); }
@@ -158,7 +181,7 @@ void lambda_k(const OrcParams& P, int n, const double* pred, const uint32_t* cs,
         if (r2 > (R)1.0e-12) {
           const R r = sqrt_r(r2);
           const R hr = k.h - r;
-          const R s = k.volume * k.spiky * hr * hr / r;
+          const R s = div_r(k.volume * k.spiky * hr * hr, r);
           const R ax = s * rx, ay = s * ry, az = s * rz;
           gx = gx + ax; gy = gy + ay; gz = gz + az;
           sg2 = sg2 + (ax * ax + ay * ay + az * az);
@@ -172,13 +195,13 @@ void lambda_k(const OrcParams& P, int n, const double* pred, const uint32_t* cs,
         rho = rho + psi * poly6(k, r2);
         const R r = sqrt_r(r2);
         const R hr = k.h - r;
-        const R s = psi * k.invRho0 * k.spiky * hr * hr / r;
+        const R s = div_r(psi * k.invRho0 * k.spiky * hr * hr, r);
         gx = gx + s * rx; gy = gy + s * ry; gz = gz + s * rz;
       });
     density[i] = rho;
     const R C = rho * k.invRho0 - (R)1;
     R lam = 0;
-    if (C > (R)0) lam = -C / ((gx * gx + gy * gy + gz * gz) + sg2 + k.cfmEps + (R)1.0e-12);
+    if (C > (R)0) lam = div_r(-C, (gx * gx + gy * gy + gz * gz) + sg2 + k.cfmEps + (R)1.0e-12);
     lambda[i] = lam;
   }
 }
@@ -201,7 +224,7 @@ void delta_k(const OrcParams& P, int n, const double* pred, const double* lambda
         if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
         const R r = sqrt_r(r2);
         const R hr = k.h - r;
-        const R s = k.spiky * hr * hr / r;
+        const R s = div_r(k.spiky * hr * hr, r);
         const R t = k.h2 - r2;
         const R ratio = k.poly6 * t * t * t * k.sCorrWq;
         const R r2r = ratio * ratio;
@@ -216,14 +239,14 @@ void delta_k(const OrcParams& P, int n, const double* pred, const double* lambda
         if (r2 >= k.h2 || r2 < (R)1.0e-12) return;
         const R r = sqrt_r(r2);
         const R hr = k.h - r;
-        const R f = (li * (R)bpsi[q]) * (k.spiky * hr * hr / r);
+        const R f = (li * (R)bpsi[q]) * div_r(k.spiky * hr * hr, r);
         bx = bx + f * rx; by = by + f * ry; bz = bz + f * rz;
       });
     const R ex = dx * k.volume + bx * k.invRho0;
     const R ey = dy * k.volume + by * k.invRho0;
     const R ez = dz * k.volume + bz * k.invRho0;
     R scale = k.omega;
-    if (k.sor == 1) scale = k.omega / std::max(nc, (R)1);
+    if (k.sor == 1) scale = div_r(k.omega, std::max(nc, (R)1));
     R bxp = px, byp = py, bzp = pz;
     if (base) { bxp = (R)base[i * 4]; byp = (R)base[i * 4 + 1]; bzp = (R)base[i * 4 + 2]; }
     R o[3] = {bxp + ex * scale, byp + ey * scale, bzp + ez * scale};
@@ -255,14 +278,14 @@ void xsph_k(const OrcParams& P, int n, const double* pred, const double* pos,
         const R r2 = r2_of(rx, ry, rz);
         if (r2 >= k.h2) return;
         const R den = std::max((R)density[j], (R)0.5 * k.rho0);
-        const R f = k.mass * poly6(k, r2) / den;
+        const R f = div_r(k.mass * poly6(k, r2), den);
         ax = ax + f * (v[(size_t)j * 3] - vx);
         ay = ay + f * (v[(size_t)j * 3 + 1] - vy);
         az = az + f * (v[(size_t)j * 3 + 2] - vz);
         if (normals && r2 >= (R)1.0e-12) {
           const R r = sqrt_r(r2);
           const R hr = k.h - r;
-          const R g = (k.mass / den) * (k.spiky * hr * hr / r);
+          const R g = div_r(k.mass, den) * div_r(k.spiky * hr * hr, r);
           nx = nx + g * rx; ny = ny + g * ry; nz = nz + g * rz;
         }
       },
@@ -302,11 +325,11 @@ void tension_k(const OrcParams& P, int n, const double* pred, const double* dens
         const R kk = hrr * hrr * hrr * r * r * r;
         R cw = k.cohes * ((R)2 * kk - k.cohesTerm);
         if ((R)2 * r > k.h) cw = k.cohes * kk;
-        const R fc = -k.tension * k.mass * cw / r;
+        const R fc = div_r(-k.tension * k.mass * cw, r);
         const R fx = fc * rx + (-k.tension * (nix - (R)normal[j * 4]));
         const R fy = fc * ry + (-k.tension * (niy - (R)normal[j * 4 + 1]));
         const R fz = fc * rz + (-k.tension * (niz - (R)normal[j * 4 + 2]));
-        const R KK = (R)2 * k.rho0 / (rhoi + (R)density[j]);
+        const R KK = div_r((R)2 * k.rho0, rhoi + (R)density[j]);
         ax = ax + KK * fx; ay = ay + KK * fy; az = az + KK * fz;
       },
       [&](uint32_t) {});
@@ -508,21 +531,21 @@ int orc_version() { return 1; }
 
 void orc_aniso(int mode, const OrcParams* P, int n, const double* pos, const uint32_t* cs,
                const double* ap, double* smooth, double* g) {
-  if (mode == 0) aniso_k<float>(*P, n, pos, cs, ap, smooth, g);
+  if (float_mode(mode)) aniso_k<float>(*P, n, pos, cs, ap, smooth, g);
   else aniso_k<double>(*P, n, pos, cs, ap, smooth, g);
 }
 
 // The same for the targets sel[0..nsel) only (outputs nsel records).
 void orc_aniso_sel(int mode, const OrcParams* P, int n, const double* pos, const uint32_t* cs,
                    const double* ap, int nsel, const int* sel, double* smooth, double* g) {
-  if (mode == 0) aniso_k<float>(*P, n, pos, cs, ap, smooth, g, sel, nsel);
+  if (float_mode(mode)) aniso_k<float>(*P, n, pos, cs, ap, smooth, g, sel, nsel);
   else aniso_k<double>(*P, n, pos, cs, ap, smooth, g, sel, nsel);
 }
 
 // The Jacobi eigensolve of the anisotropy pass on one symmetric matrix
 // (row-major a[9]); eigenvalues to w[3], eigenvectors as columns of v[9].
 void orc_jacobi(int mode, const double* a9, double* w, double* v9) {
-  if (mode == 0) {
+  if (float_mode(mode)) {
     float a[3][3], v[3][3];
     for (int k = 0; k < 9; k++) a[k / 3][k % 3] = (float)a9[k];
     jacobi_ref(a, v);
@@ -541,7 +564,7 @@ void orc_jacobi(int mode, const double* a9, double* w, double* v9) {
 void orc_cell_ids(int mode, const OrcParams* P, int n, const double* pred, uint32_t* key) {
   for (int i = 0; i < n; i++) {
     int c[3];
-    if (mode == 0) {
+    if (float_mode(mode)) {
       K<float> k(*P);
       cell_of(k, (float)pred[i * 4], (float)pred[i * 4 + 1], (float)pred[i * 4 + 2], c);
       key[i] = cell_index(k, c);
@@ -577,21 +600,21 @@ void orc_grid(int n, int nCells, const uint32_t* key, uint32_t* cellStart, uint3
 
 void orc_predict(int mode, const OrcParams* P, int n, const double* pos, const double* vel,
                  double* pred, double* velOut) {
-  if (mode == 0) predict<float>(*P, n, pos, vel, pred, velOut);
+  if (float_mode(mode)) predict<float>(*P, n, pos, vel, pred, velOut);
   else predict<double>(*P, n, pos, vel, pred, velOut);
 }
 
 void orc_lambda(int mode, const OrcParams* P, int n, const double* pred, const uint32_t* cs,
                 const double* bpos, const double* bpsi, const uint32_t* bcs, double* density,
                 double* lambda) {
-  if (mode == 0) lambda_k<float>(*P, n, pred, cs, bpos, bpsi, bcs, density, lambda);
+  if (float_mode(mode)) lambda_k<float>(*P, n, pred, cs, bpos, bpsi, bcs, density, lambda);
   else lambda_k<double>(*P, n, pred, cs, bpos, bpsi, bcs, density, lambda);
 }
 
 void orc_delta(int mode, const OrcParams* P, int n, const double* pred, const double* lambda,
                const uint32_t* cs, const double* bpos, const double* bpsi, const uint32_t* bcs,
                double* out) {
-  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, nullptr);
+  if (float_mode(mode)) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, nullptr);
   else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, nullptr);
 }
 
@@ -602,26 +625,26 @@ void orc_delta(int mode, const OrcParams* P, int n, const double* pred, const do
 void orc_delta_at(int mode, const OrcParams* P, int n, const double* pred, const double* base,
                   const double* lambda, const uint32_t* cs, const double* bpos,
                   const double* bpsi, const uint32_t* bcs, double* out) {
-  if (mode == 0) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, base);
+  if (float_mode(mode)) delta_k<float>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, base);
   else delta_k<double>(*P, n, pred, lambda, cs, bpos, bpsi, bcs, out, base);
 }
 
 void orc_xsph(int mode, const OrcParams* P, int n, const double* pred, const double* pos,
               const double* density, const uint32_t* cs, int normals, double* vel, double* corr,
               double* normal) {
-  if (mode == 0) xsph_k<float>(*P, n, pred, pos, density, cs, normals, vel, corr, normal);
+  if (float_mode(mode)) xsph_k<float>(*P, n, pred, pos, density, cs, normals, vel, corr, normal);
   else xsph_k<double>(*P, n, pred, pos, density, cs, normals, vel, corr, normal);
 }
 
 void orc_tension(int mode, const OrcParams* P, int n, const double* pred, const double* density,
                  const double* normal, const uint32_t* cs, double* corr) {
-  if (mode == 0) tension_k<float>(*P, n, pred, density, normal, cs, corr);
+  if (float_mode(mode)) tension_k<float>(*P, n, pred, density, normal, cs, corr);
   else tension_k<double>(*P, n, pred, density, normal, cs, corr);
 }
 
 void orc_finalize(int mode, int n, const double* pred, const double* corr, double* vel,
                   double* pos) {
-  if (mode == 0) finalize_k<float>(n, pred, corr, vel, pos);
+  if (float_mode(mode)) finalize_k<float>(n, pred, corr, vel, pos);
   else finalize_k<double>(n, pred, corr, vel, pos);
 }
 

@@ -18,6 +18,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 F32, F64 = 0, 1
+# F32 with every sqrt and division of lambda, delta, xsph and tension one
+# ulp above the correctly rounded result: within WGSL's accuracy for f32
+# division (2.5 ULP) and sqrt (inherited from 1/inverseSqrt), so one
+# conforming implementation of the reference's arithmetic.
+F32_ULP = 2
 
 
 class OrcParams(C.Structure):

@@ -187,6 +187,54 @@ def test_pair_lambda_delta_closed_form(r):
     np.testing.assert_allclose(out[1, :3], ps[1, :3] - dp, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("r", [0.3, 1.0, 1.7])
+def test_one_ulp_mode_shifts_only_sqrt_and_division(r):
+    """The oracle's F32_ULP mode (a conforming implementation of WGSL's f32
+    sqrt and division, one ulp above the correctly rounded result) against
+    a float32 evaluation of the pair lambda with numpy's correctly rounded
+    sqrt and division, unshifted (= F32) and shifted by np.nextafter
+    (= F32_ULP), bit for bit.  The pair lies along x, so r^2 is one rounded
+    product (the fma chain adds exact zeros)."""
+    P = pair_P(r)
+    pred, _ = pair_state(r, u=(1.0, 0.0, 0.0))
+    key, cs, slot, order = fresh_grid(P, pred)
+    ps = pred[order]
+    f = np.float32
+    h, h2, mass, poly6 = f(P.h), f(P.h2), f(P.mass), f(P.poly6)
+    vol, spiky, inv_rho0, eps = f(P.volume), f(P.spiky), f(P.invRho0), f(P.cfmEps)
+    up = lambda x: np.nextafter(f(x), f(np.inf))
+
+    def lam_pair(i, shift):
+        sq = (lambda x: up(np.sqrt(x))) if shift else np.sqrt
+        dv = (lambda a, b: up(a / b)) if shift else (lambda a, b: f(a / b))
+        rx = f(ps[i, 0]) - f(ps[1 - i, 0])
+        r2 = rx * rx
+        rho = mass * (poly6 * h2 * h2 * h2)
+        t = h2 - r2
+        rho = rho + mass * (poly6 * t * t * t)
+        rr = sq(r2)
+        hr = h - rr
+        s = dv(vol * spiky * hr * hr, rr)
+        ax, ay, az = s * rx, s * f(0), s * f(0)
+        sg2 = f(0) + (ax * ax + ay * ay + az * az)
+        C = rho * inv_rho0 - f(1)
+        if not C > 0:
+            return f(0)
+        return dv(-C, (ax * ax + ay * ay + az * az) + sg2 + eps + f(1.0e-12))
+
+    lams = {}
+    for mode, shift in ((O.F32, False), (O.F32_ULP, True)):
+        _, lam = O.lambda_(mode, P, ps, cs)
+        lams[mode] = lam
+        for i in range(2):
+            assert np.float32(lam[i]) == lam_pair(i, shift), (mode, i)
+    l0, l2 = lams[O.F32][0], lams[O.F32_ULP][0]
+    if r < 1.0:
+        assert l0 < 0  # a compressed pair: the shift reaches lambda
+    if l0 != 0:
+        assert l0 != l2 and abs(l2 - l0) <= 8 * abs(np.spacing(np.float32(l0)))
+
+
 def test_pair_xsph_normals_tension_closed_form():
     r = 1.2
     P = pair_P(r, mass=900.0)

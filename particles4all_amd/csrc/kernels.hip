@@ -1041,6 +1041,70 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
   if (valid) op.finish(U, i, t);
 }
 
+// The full 27-cell walk of a target a list launch left (k_fallback).  Few
+// waves run it (about one per SIMD, each lane on a target of its own), so
+// its time is the latency of its loads rather than issue: each row's
+// candidates are read 32 at a time (the tail of a row rereads the row's
+// last candidate, whose term is skipped), and the next row's bounds are
+// read while the current row is summed, so a row costs about one round of
+// loads.  The terms are the full walk's, in its order.
+template <class Op>
+__device__ void wide_walk(const Uniform& U, const Op& op, uint32_t i,
+                          const uint32_t* __restrict__ cellStart,
+                          const uint32_t* __restrict__ bcellStart,
+                          const float4* __restrict__ bpos) {
+  constexpr int W = 32;
+  cellStart += U.cBase;
+  bcellStart += U.cBase;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  typename Op::Target t;
+  op.target(U, i, t);
+  const Cell c = op.cell(U, t);
+  const int xlo = max(c.x - 1, 0), xhi = min(c.x + 1, gx - 1);
+  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
+                        c.z >= gz - 2;
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall;
+  // bounds of stencil row k (dz = k / 3 - 1, dy = k % 3 - 1), if in the grid
+  const auto bounds = [&](int k, uint32_t& rb, uint32_t& b, uint32_t& e) {
+    const int z = c.z + k / 3 - 1, y = c.y + k % 3 - 1;
+    if (z < 0 || z >= gz || y < 0 || y >= gy) return false;
+    rb = (uint32_t)((z * gy + y) * gx);
+    b = cellStart[rb + xlo];
+    e = cellStart[rb + xhi + 1];
+    return true;
+  };
+  bool seen = false;  // the target's own record lay in a visited row
+  uint32_t nrb = 0, nb0 = 0, ne0 = 0;
+  bool nok = bounds(0, nrb, nb0, ne0);
+  for (int k = 0; k < 9; k++) {
+    const uint32_t rb = nrb, b = nb0, e = ne0;
+    const bool ok = nok;
+    if (k + 1 < 9) nok = bounds(k + 1, nrb, nb0, ne0);
+    if (!ok) continue;
+    seen = seen || (i >= b && i < e);
+    for (uint32_t j0 = b; j0 < e; j0 += W) {
+      float4 v[W][Op::NREC];
+#pragma unroll
+      for (int u = 0; u < W; u++) op.load(U, (size_t)min(j0 + (uint32_t)u, e - 1u), v[u]);
+#pragma unroll
+      for (int u = 0; u < W; u++) {
+        if (j0 + (uint32_t)u < e) {
+          if constexpr (Op::FAST) op.fpair(U, t, v[u]);
+          else op.pair(U, t, j0 + (uint32_t)u, v[u]);
+        }
+      }
+    }
+    if (doB) {
+      const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+      for (uint32_t q = bb; q < be; q++) op.bpair(U, t, bpos[q]);
+    }
+  }
+  if constexpr (Op::FAST) {
+    if (!seen) op.self_outside(U, t);
+  }
+  op.finish(U, i, t);
+}
+
 // The targets a list launch queued, walked in full.
 template <class Op, int NT>
 __global__ void __launch_bounds__(NT)
@@ -1050,7 +1114,7 @@ k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
   const uint32_t cnt = min(nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], nb.segCap);
   const uint32_t* __restrict__ q = nb.queue[nb.slot] + (size_t)seg * nb.segCap;
   for (uint32_t k = part * NT + threadIdx.x; k < cnt; k += NB_FALLBACK_PARTS * NT)
-    direct_walk<Op, NT, 1, 0>(U, op, q[k], cellStart, bcellStart, bpos);
+    wide_walk<Op>(U, op, q[k], cellStart, bcellStart, bpos);
 }
 
 // Batched scenes: each workgroup lies in one scene's 256-aligned slice of

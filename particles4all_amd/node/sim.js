@@ -13,6 +13,7 @@ const BUF = {
   normal: 9, cellStart: 10, bpos: 11, bcellStart: 12, bodyCentre: 13, bodyRot: 14,
   pid: 15, key: 16, order: 17, bpsi: 18, smoothPos: 19, anisoG: 20, ofs: 21, cellOfSlot: 22,
   drift: 23, field: 24, fieldNormal: 25, meshVerts: 26, meshIndirect: 27, bodyPacked: 28,
+  neighbourState: 29,
 };
 const STAGE = { predict: 0, grid: 1, bodyPre: 2, lambda: 3, delta: 4, bodySolve: 5,
                 finish: 6, stats: 7 };
@@ -196,6 +197,8 @@ class Sim {
       pid: ['u32', n], key: ['u32', n], order: ['u32', n], bpsi: ['f32', this.nBoundary],
       // FP16-offset records as raw 32-bit words: float x, then half y | half z
       ofs: ['u32', n * 2], cellOfSlot: ['u32', n],
+      // neighbour masks: travel bound (float bits), then the queue counts
+      neighbourState: ['u32', 513],
     }[name];
     if (!shape) throw new Error(`unknown buffer ${name}`);
     return native.read(this.handle, BUF[name], shape[0], shape[1]);

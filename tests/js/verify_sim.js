@@ -128,4 +128,17 @@ sim.releaseBody();
 out.bodyCentre = sim.bodyPose[0].centre;
 out.stats = sim.stats;
 sim.destroy();
+// neighbour masks through the JS surface: the default skin and skin 0 (full
+// walks) give the same bits; the mask state reads back
+{
+  const a = new Sim(0), b = new Sim(0);
+  a.reset(params);
+  b.reset(Object.assign({}, params, { neighbourSkin: 0 }));
+  for (let f = 0; f < 4; f++) { a.step(1 / 60); b.step(1 / 60); }
+  const pa = a.read('pos'), pb = b.read('pos');
+  out.masksSameBits = pa.length === pb.length && pa.every((v, i) => Object.is(v, pb[i]));
+  out.neighbourStateLen = a.read('neighbourState').length;
+  a.destroy();
+  b.destroy();
+}
 console.log(JSON.stringify(out));

@@ -502,9 +502,12 @@ P4A_API int p4a_copy_probe(int device, size_t bytes, int32_t reps, double* ms);
 /* Autotuner: times the neighbour kernels' tile shapes on the live scene
  * (lambda for the one-record kernels; xsph+normals followed by tension for
  * the two-record ones, as a substep runs them) and keeps the fastest for
- * later steps.  chosen[0..1] receive the
- * variant ids; ms (capacity entries) the mean ms of every variant tried,
- * group 1 first.  The particle state is not changed. */
+ * later steps.  In the FP16-offset form group 1 times that form's lambda,
+ * which takes only the thread count of the direct-walk shapes 0-2; the
+ * other group-1 shapes report -1 (P4A_ERR_ARG when the FP16 records are
+ * stale).  chosen[0..1] receive the variant ids; ms (capacity entries) the
+ * mean ms of every variant tried, group 1 first.  The particle state is
+ * not changed. */
 P4A_API int p4a_tune(p4a_sim* h, int32_t chosen[2], double* ms, int32_t capacity);
 /* Name of tile variant v of group 1 or 2. */
 P4A_API const char* p4a_tile_variant_name(int32_t group, int32_t v);

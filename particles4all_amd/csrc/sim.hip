@@ -2005,13 +2005,27 @@ P4A_API int p4a_tune(p4a_sim* s, int32_t chosen[2], double* ms, int32_t capacity
   HIP_TRY(s, hipEventCreate(&S.ev[0]));
   HIP_TRY(s, hipEventCreate(&S.ev[1]));
   hipEvent_t e0 = S.ev[0], e1 = S.ev[1];
+  // the FP16-offset form runs its own lambda and delta kernels, which take
+  // only the thread count of the group-1 shape (direct_threads): group 1
+  // then times the FP16 lambda at the three direct-walk shapes, and the
+  // other shapes (same kernels) report -1
+  const bool f16 = fp16_form(s);
+  if (f16 && !s->ofsValid) return stale_records(s);
   int k = 0;
   for (int group = 1; group <= 2; group++) {
     float best = 1e30f;
     int bestV = 0;
     for (int v = 0; v < tile_variants(group); v++) {
       TileCfg c{v};
+      if (f16 && group == 1 && v > 2) {
+        if (ms && k < capacity) ms[k] = -1.0;
+        k++;
+        continue;
+      }
       auto run = [&]() -> hipError_t {
+        if (group == 1 && f16)
+          return launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur), direct_threads(c),
+                                 Drift{}, s->stream);
         if (group == 1) return launch_lambda(U, s->A, s->predCur, ex, c, Drift{}, s->stream);
         hipError_t e = launch_xsph(U, s->A, s->predCur, s->A.pos[s->parity], ex, tension, c,
                                    Drift{}, s->stream);

@@ -446,3 +446,36 @@ def test_tune_leaves_the_state_and_the_bits():
         assert np.array_equal(a.read(k).view(np.uint32), b.read(k).view(np.uint32)), k
     a.close()
     b.close()
+
+
+def test_tune_times_the_fp16_kernels_in_the_fp16_form(monkeypatch):
+    """In the FP16-offset form p4a_tune times the form's own lambda at the
+    three direct-walk shapes (the only thing of the group-1 shape that form
+    uses; the other shapes report -1), leaves the state unchanged, and a
+    twin created with the chosen shapes steps to the same bits."""
+    from particles4all_amd._lib import lib
+    p, sc, _ = workloads.block_1m(scale=0.125, math=0)
+    p.precision = 1
+    b = Sim(0)
+    b.reset(p, sc)
+    b.substeps(1 / 120, 2)
+    before = {k: b.read(k).copy() for k in ("POS", "VEL", "PID", "OFS")}
+    chosen, times = b.tune()
+    print(f"tuned: {chosen} {times}")
+    g1 = [k for k in times if k.startswith("g1:")]
+    assert all(times[k] > 0 for k in g1[:3]) and all(times[k] == -1 for k in g1[3:])
+    assert chosen[0] in ("direct/t256", "direct/t128", "direct/t64")
+    for k, v in before.items():
+        assert np.array_equal(b.read(k).view(np.uint32), v.view(np.uint32)), k
+    names = {g: [lib().p4a_tile_variant_name(g, v).decode() for v in range(9)] for g in (1, 2)}
+    monkeypatch.setenv("P4A_TILE1", str(names[1].index(chosen[0])))
+    monkeypatch.setenv("P4A_TILE2", str(names[2].index(chosen[1])))
+    a = Sim(0)
+    a.reset(p, sc)
+    a.substeps(1 / 120, 2)
+    for s in (a, b):
+        s.substeps(1 / 120, 2)
+    for k in ("POS", "VEL", "PID", "DENSITY", "LAMBDA", "OFS"):
+        assert np.array_equal(a.read(k).view(np.uint32), b.read(k).view(np.uint32)), k
+    a.close()
+    b.close()

@@ -669,13 +669,16 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // each XCD walks a contiguous range of the sorted particles.  Wall samples
 // exist only in cells with some index 0 or gridDim-1, so a particle whose
 // stencil cannot reach such a cell skips the wall loops.
-// PH = 1 (fast ops): two phases per chunk of up to 32 consecutive
-// candidates of a row.  Phase 1 only forms r^2 (the same operations as the
-// pair term) and sets one bit per candidate inside h; phase 2 runs the pair
-// term for the set bits, in the same order.  About 15% of the candidates
-// lie inside h, and the full term costs ~25 issue slots against ~10 for
-// the test, so the walk issues far fewer instructions; the terms dropped
-// are the ones the branchless pair term reduces to zero.
+// PH = 1, 2: two phases per chunk of up to 32 consecutive candidates of a
+// row.  Phase 1 only forms r^2 (the same operations as the pair term) and
+// sets one bit per candidate inside h; phase 2 runs the pair term for the
+// set bits, in the same order (PH = 2: four per trip, loads in flight
+// together).  About 15% of the candidates lie inside h, and the full term
+// costs ~25 issue slots against ~10 for the test, so the walk issues far
+// fewer instructions.  The terms dropped are the ones the fast branchless
+// term reduces to zero and the ones the exact term returns from (its own
+// r^2, the same bits as phase 1's, is at least h^2), so the exact ops keep
+// the full walk's bits.
 // With LB (lambda, once per substep) the walk also builds the target's
 // neighbour masks: bit k of row r's mask marks candidate b_r + k within
 // reach (r^2 < nb->reach2), and the build position and cell go to nbRef.
@@ -714,7 +717,7 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
       seen = seen || (i >= b && i < e);
       uint64_t rowMask = 0;
       if constexpr (LB) unlisted = unlisted || (e - b > 64u);
-      if constexpr (Op::FAST && PH >= 1) {
+      if constexpr (PH >= 1) {
         const size_t slot = (size_t)((dz + CR) * (2 * CR + 1) + (dy + CR)) * op.maskStride + i;
         for (uint32_t c0 = b; c0 < e; c0 += 32u) {
           const uint32_t len = min(e - c0, 32u);
@@ -748,18 +751,23 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
             if (m) {
               uint32_t top = 31u - (uint32_t)__builtin_clz(m);
               m ^= 1u << top;
+              uint32_t jc = c0 + (len - 1u - top);
               float4 cur[Op::NREC];
-              op.load(U, c0 + (len - 1u - top), cur);
+              op.load(U, jc, cur);
               while (m) {
                 top = 31u - (uint32_t)__builtin_clz(m);
                 m ^= 1u << top;
+                const uint32_t jn = c0 + (len - 1u - top);
                 float4 nxt[Op::NREC];
-                op.load(U, c0 + (len - 1u - top), nxt);
-                op.fpair(U, t, cur);
+                op.load(U, jn, nxt);
+                if constexpr (Op::FAST) op.fpair(U, t, cur);
+                else op.pair(U, t, jc, cur);
 #pragma unroll
                 for (int r = 0; r < Op::NREC; r++) cur[r] = nxt[r];
+                jc = jn;
               }
-              op.fpair(U, t, cur);
+              if constexpr (Op::FAST) op.fpair(U, t, cur);
+              else op.pair(U, t, jc, cur);
             }
           } else {
             // four hits per trip with their loads in flight together; a
@@ -780,10 +788,14 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
               for (int u = 0; u < 4; u++) op.load(U, jj[u], v[u]);
 #pragma unroll
               for (int u = 0; u < 4; u++) {
+                if constexpr (Op::FAST) {
 #pragma unroll
-                for (int r = 0; r < Op::NREC; r++)
-                  if (!hit[u]) v[u][r] = Op::sentinel(r);
-                op.fpair(U, t, v[u]);
+                  for (int r = 0; r < Op::NREC; r++)
+                    if (!hit[u]) v[u][r] = Op::sentinel(r);
+                  op.fpair(U, t, v[u]);
+                } else if (hit[u]) {
+                  op.pair(U, t, jj[u], v[u]);
+                }
               }
             }
           }
@@ -2512,10 +2524,13 @@ k_stats_final_batch(BatchTab B, const double* __restrict__ part, double* __restr
   }
 }
 
+// One record per thread: on MI355X this streams at ~6.2 TB/s on 1 GiB, where
+// a grid-stride loop over 8192 blocks reaches ~4.6 TB/s (one load in flight
+// per thread for several rounds) and unrolled or non-temporal forms no more.
 __global__ void __launch_bounds__(WG) k_copy4(const float4* __restrict__ a, float4* __restrict__ b,
                                                size_t n) {
-  const size_t stride = (size_t)gridDim.x * WG;
-  for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += stride) b[i] = a[i];
+  const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
@@ -3086,16 +3101,16 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
                        hipStream_t s, const BatchTab* B) {
   float4* p = const_cast<float4*>(pos);
   if (normals) {
-    if (exact)
-      return run_tile2(U, A, drifted(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
-  {
-    XsphFast<true> op{{}, pred, A.velq, A.density, A.corr, A.normal};
-    if (reuse_masks(A, t, B)) {
-      op.maskOut = A.hitMask;  // for the tension kernel that follows
-      op.maskStride = A.hitStride;
-    }
-    return run_tile2(U, A, drifted(op, d), t, s, B);
-  }
+    // the first chunk's hit masks go to the tension kernel that follows
+    const auto keep = [&](auto op) {
+      if (reuse_masks(A, t, B)) {
+        op.maskOut = A.hitMask;
+        op.maskStride = A.hitStride;
+      }
+      return run_tile2(U, A, drifted(op, d), t, s, B);
+    };
+    if (exact) return keep(XsphOp<true, true>{{}, pred, p, A.density, A.corr, A.normal});
+    return keep(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal});
   }
   if (exact)
     return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
@@ -3105,14 +3120,16 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, const Drift& d,
                           hipStream_t s, const BatchTab* B) {
-  if (exact)
-    return run_tile2(U, A, drifted(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel}, d), t, s, B);
-  TensionFast op{{}, pred, pos, A.velq, A.normal, A.corr, vel};
-  if (reuse_masks(A, t, B)) {
-    op.maskIn = A.hitMask;  // written by xsph+normals over the same positions
-    op.maskStride = A.hitStride;
-  }
-  return run_tile2(U, A, drifted(op, d), t, s, B);
+  // the masks written by xsph+normals over the same positions
+  const auto take = [&](auto op) {
+    if (reuse_masks(A, t, B)) {
+      op.maskIn = A.hitMask;
+      op.maskStride = A.hitStride;
+    }
+    return run_tile2(U, A, drifted(op, d), t, s, B);
+  };
+  if (exact) return take(TensionOp<true>{{}, pred, pos, A.density, A.normal, A.corr, vel});
+  return take(TensionFast{{}, pred, pos, A.velq, A.normal, A.corr, vel});
 }
 
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
@@ -3241,7 +3258,9 @@ hipError_t launch_field_aniso(const Uniform& U, const Arrays& A, const FieldArgs
 
 hipError_t launch_copy4(const float4* a, float4* b, size_t n, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy4, dim3(256 * 32), dim3(WG), 0, s, a, b, n);
+  const size_t g = (n + WG - 1) / WG;
+  if (g > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy4, dim3((unsigned)g), dim3(WG), 0, s, a, b, n);
   return hipGetLastError();
 }
 

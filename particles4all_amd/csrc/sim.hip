@@ -718,10 +718,12 @@ P4A_API int p4a_copy_probe(int device, size_t bytes, int32_t reps, double* ms) {
   if (!rc) {
     const size_t n = bytes / 16;
     hipMemsetAsync(a, 0, bytes, st);
-    launch_copy4((const float4*)a, (float4*)b, n, st);  // warm
+    hipError_t le = launch_copy4((const float4*)a, (float4*)b, n, st);  // warm
     hipEventRecord(e0, st);
-    for (int r = 0; r < reps; r++) launch_copy4((const float4*)a, (float4*)b, n, st);
+    for (int r = 0; r < reps && le == hipSuccess; r++)
+      le = launch_copy4((const float4*)a, (float4*)b, n, st);
     hipEventRecord(e1, st);
+    if (le != hipSuccess) rc = P4A_ERR_ARG;
     if (hipStreamSynchronize(st) != hipSuccess) rc = P4A_ERR_HIP;
     float t = 0;
     hipEventElapsedTime(&t, e0, e1);

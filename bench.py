@@ -538,7 +538,7 @@ def headline(args):
                       "or the list walk plus its fallback walk (later stages); traffic and "
                       "profile duration per stage from profiles/" + args.workload +
                       "_fp32.md",
-            "copy_probe_GBps": copy_gbs}
+            "copy_probe_GBps": copy_gbs, "frac_of_copy_probe": achieved / copy_gbs}
     if prof_us:
         roof["profile_mean_launch_ms"] = prof_us / 1e3
         roof["frac_at_profile_duration"] = ALGO_BYTES[dom] * n / (prof_us * 1e-6) / 1e9 / \

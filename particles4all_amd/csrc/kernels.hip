@@ -973,9 +973,24 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
       nb.ctl[1 + (uint32_t)(nb.slot ^ 1) * NB_QSEG + k] = 0u;
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  const int lane = __lane_id();
   typename Op::Target t;
   Cell c{0, 0, 0};
-  bool valid = false;
+  bool valid = false;  // the masks hold
+  bool full = false;   // walked here with every candidate of its rows
+  // appends the lanes with q set to this launch's queue (one atomic per wave)
+  const auto enqueue = [&](bool q) {
+    const uint64_t qm = __ballot(q);
+    if (!qm) return;
+    const int leader = __builtin_ctzll(qm);
+    uint32_t base = 0;
+    const uint32_t seg = blockIdx.x % NB_QSEG;
+    if (lane == leader)
+      base = atomicAdd(&nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], (uint32_t)__popcll(qm));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
+    if (q && at < nb.segCap) nb.queue[nb.slot][seg * nb.segCap + at] = i;
+  };
   if (i < U.n) {
     op.target(U, i, t);
     c = op.cell(U, t);
@@ -983,38 +998,41 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     const float travel = sqrtf(r2_of(t.px - ref.x, t.py - ref.y, t.pz - ref.z));
     const float most = __uint_as_float(*(volatile const uint32_t*)nb.ctl);
     valid = __float_as_uint(ref.w) == cell_index(U, c) && travel + most <= nb.lim;
-    const bool queued = !valid;
-    const uint64_t qm = __ballot(queued);
-    if (qm) {
-      const int lane = __lane_id();
-      const int leader = __builtin_ctzll(qm);
-      uint32_t base = 0;
-      const uint32_t seg = blockIdx.x % NB_QSEG;
-      if (lane == leader)
-        base = atomicAdd(&nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], (uint32_t)__popcll(qm));
-      base = (uint32_t)__shfl((int)base, leader, 64);
-      const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
-      if (queued && at < nb.segCap) nb.queue[nb.slot][seg * nb.segCap + at] = i;
-    }
+    // in the exact ops, a wave with a quarter of its targets or more failing
+    // the test (all of them once any particle has travelled past the skin)
+    // walks those in full here, through its own staged rows: the fallback
+    // launch is sized for a few targets per lane.  The fast ops queue them
+    // all: their fallback walks are cheap enough that the extra registers
+    // would cost the list launch more than they save.
+    full = !Op::FAST && !valid && __popcll(__ballot(!valid)) >= 16;
+    enqueue(!valid && !full);
   }
-  if (!__ballot(valid)) return;  // the whole wave is queued (or past n)
+  if (!__ballot(valid || full)) return;  // the whole wave is queued (or past n)
   const int xlo = max(c.x - 1, 0);
   const int xhi = min(c.x + 1, gx - 1);
   const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
                         c.z >= gz - 2;
-  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && valid;
-  const int lane = __lane_id();
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && (valid || full);
+  bool seen = false;  // a full lane's own record lay in a visited row
+  bool fat = false;   // a full lane met a row of more than 64 candidates
   for (int dz = -1; dz <= 1; dz++) {
     for (int dy = -1; dy <= 1; dy++) {
       const int z = c.z + dz, y = c.y + dy;
-      const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+      const bool rowOk = (valid || full) && z >= 0 && z < gz && y >= 0 && y < gy;
       const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
       uint32_t b = 0, e = 0;
       uint64_t m64 = 0;
       if (rowOk) {
         b = cellStart[rb + xlo];
         e = cellStart[rb + xhi + 1];
-        m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
+        if (valid) {
+          m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
+        } else {
+          // the full walk: every candidate of the row, in order
+          seen = seen || (i >= b && i < e);
+          fat = fat || e - b > 64u;
+          if (!fat) m64 = e - b >= 64u ? ~0ull : (1ull << (e - b)) - 1ull;
+        }
       }
       // union of the wave's ranges that hold a hit (every lane is active
       // here, the others contribute the neutral values)
@@ -1044,11 +1062,19 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
         hit_loop(U, op, t, (uint32_t)m64, b, rec);
         hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
       }
-      if (doB && rowOk) {
+      if (doB && rowOk && !fat) {
         const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
       }
     }
+  }
+  // a full lane that met a fat row leaves its partial sums for the fallback
+  enqueue(full && fat);
+  if (full && !fat) {
+    if constexpr (Op::FAST) {
+      if (!seen) op.self_outside(U, t);
+    }
+    op.finish(U, i, t);
   }
   if (valid) op.finish(U, i, t);
 }
@@ -1117,16 +1143,52 @@ __device__ void wide_walk(const Uniform& U, const Op& op, uint32_t i,
   op.finish(U, i, t);
 }
 
-// The targets a list launch queued, walked in full.
+// The targets a list launch queued, walked in full.  The queue segments
+// fill unevenly (the queued targets cluster where the flow is fastest), so
+// every workgroup first forms the prefix sums of the segment counts and the
+// whole grid then deals the concatenated queue out one target per thread
+// at a time: the launch lasts about one walk however the targets fell.
 template <class Op, int NT>
 __global__ void __launch_bounds__(NT)
 k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
            const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
-  const uint32_t seg = blockIdx.x % NB_QSEG, part = blockIdx.x / NB_QSEG;
-  const uint32_t cnt = min(nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], nb.segCap);
-  const uint32_t* __restrict__ q = nb.queue[nb.slot] + (size_t)seg * nb.segCap;
-  for (uint32_t k = part * NT + threadIdx.x; k < cnt; k += NB_FALLBACK_PARTS * NT)
-    wide_walk<Op>(U, op, q[k], cellStart, bcellStart, bpos);
+  static_assert(NT >= 64 && NB_QSEG == 256, "one wave scans 4 counts per lane");
+  __shared__ uint32_t pre[NB_QSEG + 1];
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    // exclusive prefix of the 256 counts: 4 per lane, then across the wave
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v[k] = min(nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + 4 * tid + k], nb.segCap);
+      sum += v[k];
+    }
+    uint32_t inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)inc, off, 64);
+      if (tid >= off) inc += u;
+    }
+    uint32_t run = inc - sum;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pre[4 * tid + k] = run;
+      run += v[k];
+    }
+    if (tid == 63) pre[NB_QSEG] = run;
+  }
+  __syncthreads();
+  const uint32_t total = pre[NB_QSEG];
+  for (uint32_t g = blockIdx.x * NT + (uint32_t)tid; g < total; g += gridDim.x * NT) {
+    // the segment holding item g: the last s with pre[s] <= g
+    uint32_t lo = 0, hi = NB_QSEG;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (pre[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t item = nb.queue[nb.slot][(size_t)lo * nb.segCap + (g - pre[lo])];
+    wide_walk<Op>(U, op, item, cellStart, bcellStart, bpos);
+  }
 }
 
 // Batched scenes: each workgroup lies in one scene's 256-aligned slice of

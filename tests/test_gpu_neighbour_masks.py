@@ -8,8 +8,10 @@ most targets are queued.
 * dam_break_32 and a quarter-scale block_4m_bodies: 12 substeps with skins
   0.1 and 0.3 against skin 0 (positions, velocities, densities, lambdas,
   particle ids, body poses), fast and exact math;
-* skin 1e-4: nearly every target is queued after the first delta, and the
-  result is still the full walk's;
+* skin 1e-4: nearly every target fails the mask test after the first delta
+  (the waves walk their targets in full inside the list launch, a few
+  targets go to the fallback launch), and the result is still the full
+  walk's; the block's fallback launches take queued targets at every skin;
 * the mask state: the travel bound equals the largest distance of any
   particle from its build position after the deltas (read back stage by
   stage), and the queues hold a few percent of the targets at most on the
@@ -60,8 +62,8 @@ def test_masks_equal_the_full_walk(math, maker, kw):
         print(f"math {math} {maker.__name__} skin {skin}: last queues {queued} of "
               f"{len(got['PID'])}, travel {travel:.3e}")
         same(full, got, f"skin {skin}")
-        if skin == 1e-4:
-            assert queued > 0.5 * len(got["PID"])
+        if maker is workloads.block_4m_bodies:
+            assert queued > 0  # the fallback launch walked some targets
 
 
 def test_travel_bound_and_queues_on_a_flowing_state():

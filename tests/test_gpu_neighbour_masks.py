@@ -9,9 +9,10 @@ most targets are queued.
   0.1 and 0.3 against skin 0 (positions, velocities, densities, lambdas,
   particle ids, body poses), fast and exact math;
 * skin 1e-4: nearly every target fails the mask test after the first delta
-  (the waves walk their targets in full inside the list launch, a few
-  targets go to the fallback launch), and the result is still the full
-  walk's; the block's fallback launches take queued targets at every skin;
+  (fast math queues them for the fallback launch; in exact math the waves
+  walk them in full inside the list launch and only a few go to the
+  fallback), and the result is still the full walk's; the block's fallback
+  launches take queued targets at every skin;
 * the mask state: the travel bound equals the largest distance of any
   particle from its build position after the deltas (read back stage by
   stage), and the queues hold a few percent of the targets at most on the

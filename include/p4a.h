@@ -276,10 +276,12 @@ enum p4a_buf {
   P4A_BUF_MESH_INDIRECT = 27, /* uint 4: draw-indirect arguments of the last
                               march (vertex count, 1, 0, 0) */
   P4A_BUF_BODY_PACKED = 28, /* float4 6*n_bodies: the last p4a_body_pack */
-  P4A_BUF_NEIGHBOUR_STATE = 29, /* uint 513: largest travel since the last
+  P4A_BUF_NEIGHBOUR_STATE = 29, /* uint 1025: largest travel since the last
                               neighbour-mask build (float bits), then the
                               targets the last two list launches left to the
-                              full walk, in 256 counts per launch parity.
+                              full walk (256 counts per launch parity), then
+                              the targets queued as having left their build
+                              cells (256 counts per solver iteration parity).
                               Read only; no counterpart in the reference */
   P4A_BUF_COUNT_
 };

@@ -131,6 +131,8 @@ def sort_boundary(P, pts, psi):
 
 def cell_ids(mode, P, pred):
     pred = np.ascontiguousarray(pred, np.float64)
+    if pred.ndim != 2 or pred.shape[1] != 4:
+        raise ValueError("cell_ids takes float4 records (n x 4)")
     key = np.zeros(len(pred), np.uint32)
     lib().orc_cell_ids(mode, C.byref(P), len(pred), _p(pred), _p(key))
     return key

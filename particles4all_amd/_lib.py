@@ -365,7 +365,7 @@ class Sim:
                 "MESH_VERTS": (np.uint32, 3 * self.field_info().max_triangles, 2),
                 "MESH_INDIRECT": (np.uint32, 4, 1),
                 "BODY_PACKED": (np.float32, 6 * i.n_bodies, 4),
-                "NEIGHBOUR_STATE": (np.uint32, 513, 1)}[name]
+                "NEIGHBOUR_STATE": (np.uint32, 1025, 1)}[name]
         dt, rows, c = spec
         out = np.empty((rows, c) if c > 1 else (rows,), dtype=dt)
         self._check(lib().p4a_read(self.h, BUF[name], _ptr(out), 0, out.nbytes), f"read {name}")

@@ -116,14 +116,28 @@ __device__ __forceinline__ void record_drift(uint32_t* out, float px, float py, 
   raise_bound(out, fmaxf(fabsf(ox - px), fmaxf(fabsf(oy - py), fabsf(oz - pz))));
 }
 
-// Records the distance of a new position from the position at the
-// neighbour-mask build into the travel bound (float bits; a NaN distance
-// stores a bound that fails every later check).
-__device__ __forceinline__ void record_travel(uint32_t* out, const float4* __restrict__ ref,
-                                              uint32_t i, float x, float y, float z) {
-  if (!out) return;
-  const float4 r = ref[i];
-  raise_bound(out, sqrtf(r2_of(x - r.x, y - r.y, z - r.z)));
+// Appends target i (where `want`) to a moved-target queue, segment
+// (i / 64) mod NB_QSEG.  Lanes of a wave that append together into one
+// segment take their places with one atomic, others one atomic each.  Any
+// subset of a wave's lanes may call it (the ballots see the active lanes).
+__device__ __forceinline__ void enqueue_moved(uint32_t* __restrict__ q, uint32_t* cnt,
+                                              uint32_t segCap, bool want, uint32_t i) {
+  const uint64_t qm = __ballot(want);
+  if (!qm) return;
+  const int lane = __lane_id();
+  const uint32_t seg = (i >> 6) & (NB_QSEG - 1u);
+  const int leader = __builtin_ctzll(qm);
+  const uint32_t segL = (uint32_t)__shfl((int)seg, leader, 64);
+  if (!__ballot(want && seg != segL)) {
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[seg], (uint32_t)__popcll(qm));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
+    if (want && at < segCap) q[(size_t)seg * segCap + at] = i;
+  } else if (want) {
+    const uint32_t at = atomicAdd(&cnt[seg], 1u);
+    if (at < segCap) q[(size_t)seg * segCap + at] = i;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -491,11 +505,29 @@ struct TileLds {
 // Default hooks: the cell of a target is cellOf(pred) (wgsl.js:43-46) and
 // no row is skipped.
 struct OpBase {
+  // a lambda list launch empties the moved queue its iteration's delta fills
+  static constexpr bool CLEARS_MOVED = false;
   uint32_t* driftOut = nullptr;  // drift slot a position update records into
   // a position update after a neighbour-mask build records the largest
-  // distance of any particle from its build position (nbTravel, float bits)
+  // distance of any particle from its build position (nbTravel, float
+  // bits; a NaN distance stores a bound that fails every later check)
   const float4* nbRef = nullptr;
   uint32_t* nbTravel = nullptr;
+  // ... and appends the targets whose new position lies outside the cell
+  // they were in at the build to the moved queue of the next iteration
+  uint32_t* nbMovedQ = nullptr;
+  uint32_t* nbMovedCnt = nullptr;
+  uint32_t nbSegCap = 0;
+  __device__ void record_nb(const Uniform& U, uint32_t i, float x, float y, float z) const {
+    if (!nbTravel) return;
+    const float4 r = nbRef[i];
+    raise_bound(nbTravel, sqrtf(r2_of(x - r.x, y - r.y, z - r.z)));
+    if (nbMovedQ) {
+      const uint32_t built = __float_as_uint(r.w) & ~NB_UNLISTED;
+      enqueue_moved(nbMovedQ, nbMovedCnt, nbSegCap,
+                    cell_index(U, cell_of(U, x, y, z)) != built, i);
+    }
+  }
   // two-phase walks (3x3 rows): the first chunk's hit mask of each row, kept
   // for a later kernel over the same positions and grid, or taken from an
   // earlier one instead of testing the chunk again (row r of target i at
@@ -960,17 +992,26 @@ constexpr uint32_t NB_STAGE = 256;
 // stencil order, each row followed by its wall samples as in the full
 // walk.  The wave stages each row's candidates in LDS with coalesced loads,
 // so the hits, which differ from lane to lane, are read from LDS rather
-// than gathered from memory.  Other targets are queued (one atomic per
-// wave) for k_fallback.  Block 0 empties the other queue, which the
-// previous list launch's fallback has consumed.
+// than gathered from memory.  A target that has left its build cell is
+// skipped: the kernel that moved it queued it, and that queue's fallback
+// walk runs beside this launch.  Other failing targets are queued here
+// (one atomic per wave) for the fallback walk after it.  Block 0 empties
+// the other list-launch queue, which the previous list launch's fallback
+// has consumed, and (lambda launches) the moved queue this iteration's
+// delta will fill, whose previous contents the last iteration walked.
 template <class Op>
 __global__ void __launch_bounds__(64)
 k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
-  __shared__ float4 S[Op::NREC][NB_STAGE];
-  if (blockIdx.x == 0)
+  constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
+  __shared__ float4 S[Op::NREC][STAGE];
+  if (blockIdx.x == 0) {
     for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
-      nb.ctl[1 + (uint32_t)(nb.slot ^ 1) * NB_QSEG + k] = 0u;
+      nb.ctl[nb_queue_counts(nb.slot ^ 1) + k] = 0u;
+    if (Op::CLEARS_MOVED)
+      for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
+        nb.ctl[nb_moved_counts(nb.mslot ^ 1) + k] = 0u;
+  }
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
   const int lane = __lane_id();
@@ -986,7 +1027,7 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     uint32_t base = 0;
     const uint32_t seg = blockIdx.x % NB_QSEG;
     if (lane == leader)
-      base = atomicAdd(&nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + seg], (uint32_t)__popcll(qm));
+      base = atomicAdd(&nb.ctl[nb_queue_counts(nb.slot) + seg], (uint32_t)__popcll(qm));
     base = (uint32_t)__shfl((int)base, leader, 64);
     const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
     if (q && at < nb.segCap) nb.queue[nb.slot][seg * nb.segCap + at] = i;
@@ -995,17 +1036,21 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     op.target(U, i, t);
     c = op.cell(U, t);
     const float4 ref = nb.ref[i];
+    const uint32_t built = __float_as_uint(ref.w);
+    // moved targets are the moved queue's (walked beside this launch)
+    const bool stays = (built & ~NB_UNLISTED) == cell_index(U, c);
     const float travel = sqrtf(r2_of(t.px - ref.x, t.py - ref.y, t.pz - ref.z));
     const float most = __uint_as_float(*(volatile const uint32_t*)nb.ctl);
-    valid = __float_as_uint(ref.w) == cell_index(U, c) && travel + most <= nb.lim;
+    valid = stays && built == cell_index(U, c) && travel + most <= nb.lim;
     // in the exact ops, a wave with a quarter of its targets or more failing
     // the test (all of them once any particle has travelled past the skin)
     // walks those in full here, through its own staged rows: the fallback
     // launch is sized for a few targets per lane.  The fast ops queue them
     // all: their fallback walks are cheap enough that the extra registers
     // would cost the list launch more than they save.
-    full = !Op::FAST && !valid && __popcll(__ballot(!valid)) >= 16;
-    enqueue(!valid && !full);
+    const bool fails = stays && !valid;
+    full = !Op::FAST && fails && __popcll(__ballot(fails)) >= 16;
+    enqueue(fails && !full);
   }
   if (!__ballot(valid || full)) return;  // the whole wave is queued (or past n)
   const int xlo = max(c.x - 1, 0);
@@ -1039,7 +1084,7 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
       const bool has = m64 != 0ull;
       const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
       const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
-      if (ub < ue && ue - ub <= NB_STAGE) {
+      if (ub < ue && ue - ub <= STAGE) {
         const uint32_t span = ue - ub;
         for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
           float4 v[Op::NREC];
@@ -1148,19 +1193,26 @@ __device__ void wide_walk(const Uniform& U, const Op& op, uint32_t i,
 // every workgroup first forms the prefix sums of the segment counts and the
 // whole grid then deals the concatenated queue out one target per thread
 // at a time: the launch lasts about one walk however the targets fell.
-template <class Op, int NT>
+//
+// MOVED: the moved queue of this iteration, whose targets the list launch
+// beside it skips.  A target is walked only if it is still outside its
+// build cell: shape matching after the delta that queued it may have moved
+// it back, and then the list launch takes it.
+template <class Op, int NT, bool MOVED>
 __global__ void __launch_bounds__(NT)
 k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
            const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   static_assert(NT >= 64 && NB_QSEG == 256, "one wave scans 4 counts per lane");
   __shared__ uint32_t pre[NB_QSEG + 1];
   const int tid = threadIdx.x;
+  const uint32_t* counts = nb.ctl + (MOVED ? nb_moved_counts(nb.mslot) : nb_queue_counts(nb.slot));
+  const uint32_t* queue = MOVED ? nb.moved[nb.mslot] : nb.queue[nb.slot];
   if (tid < 64) {
     // exclusive prefix of the 256 counts: 4 per lane, then across the wave
     uint32_t v[4], sum = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      v[k] = min(nb.ctl[1 + (uint32_t)nb.slot * NB_QSEG + 4 * tid + k], nb.segCap);
+      v[k] = min(counts[4 * tid + k], nb.segCap);
       sum += v[k];
     }
     uint32_t inc = sum;
@@ -1186,7 +1238,12 @@ k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
       if (pre[mid] <= g) lo = mid;
       else hi = mid;
     }
-    const uint32_t item = nb.queue[nb.slot][(size_t)lo * nb.segCap + (g - pre[lo])];
+    const uint32_t item = queue[(size_t)lo * nb.segCap + (g - pre[lo])];
+    if (MOVED) {
+      const float4 p = op.pred[item];
+      const uint32_t built = __float_as_uint(nb.ref[item].w) & ~NB_UNLISTED;
+      if (cell_index(U, cell_of(U, p.x, p.y, p.z)) == built) continue;
+    }
     wide_walk<Op>(U, op, item, cellStart, bcellStart, bpos);
   }
 }
@@ -1210,6 +1267,7 @@ k_direct_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
 // --- lambda (wgsl.js:477-543) ----------------------------------------------
 template <bool EXACT>
 struct LambdaOp : OpBase {
+  static constexpr bool CLEARS_MOVED = true;
   static constexpr bool FAST = false;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
@@ -1372,7 +1430,7 @@ struct DeltaOp : OpBase {
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
     record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
-    record_travel(nbTravel, nbRef, i, ox, oy, oz);
+    record_nb(U, i, ox, oy, oz);
   }
 };
 
@@ -1538,6 +1596,7 @@ struct FastBase : OpBase {
 // lambda is also stored in pred.w (unused by every reader of pred), so the
 // following delta reads position and lambda of a neighbour in one load.
 struct LambdaFast : FastBase {
+  static constexpr bool CLEARS_MOVED = true;
   static constexpr bool BOUNDARY = true;
   static constexpr int NREC = 1;
   const float4* pred;  // not restrict: .w is written through predW
@@ -1663,7 +1722,7 @@ struct DeltaFastT : FastBase {
     oz = fminf(fmaxf(oz, U.clampMin[2]), U.clampMax[2]);
     out[i] = make_float4(ox, oy, oz, 1.0f);
     record_drift(driftOut, t.px, t.py, t.pz, ox, oy, oz);
-    record_travel(nbTravel, nbRef, i, ox, oy, oz);
+    record_nb(U, i, ox, oy, oz);
   }
 };
 
@@ -2258,7 +2317,8 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
              double* __restrict__ centreD, double* __restrict__ rotD,
              float4* __restrict__ centreF, float4* __restrict__ rotF, HRec* __restrict__ ofs,
              const uint32_t* __restrict__ cellOfSlot, Uniform U, uint32_t* __restrict__ driftOut,
-             const float4* __restrict__ nbRef, uint32_t* __restrict__ nbTravel) {
+             const float4* __restrict__ nbRef, uint32_t* __restrict__ nbTravel,
+             uint32_t* __restrict__ movedQ, uint32_t* movedCnt, uint32_t segCap) {
   __shared__ double sh[WG / 64][15];
   __shared__ float sc[12];
   const int b = blockIdx.x;
@@ -2355,6 +2415,12 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
       const float4 r = nbRef[sl];
       const float d = sqrtf(r2_of(np.x - r.x, np.y - r.y, np.z - r.z));
       tmax = (d > tmax || d != d) ? d : tmax;
+      // queued if the projection takes it out of its build cell (the delta
+      // before queued it if it was out already)
+      const uint32_t built = __float_as_uint(r.w) & ~NB_UNLISTED;
+      const bool was = cell_index(U, cell_of(U, op.x, op.y, op.z)) != built;
+      const bool now = cell_index(U, cell_of(U, np.x, np.y, np.z)) != built;
+      enqueue_moved(movedQ, movedCnt, segCap, now && !was, sl);
     }
     pred[sl] = np;
     if (ofs) ofs[sl] = encode_ofs(U, cellOfSlot[sl], np.x, np.y, np.z);
@@ -3046,11 +3112,26 @@ hipError_t run_nb(const Uniform& U, const Arrays& A, const Op& op, const NbLaunc
                        A.cellStart, A.bcellStart, A.bpos);
     return hipGetLastError();
   }
+  // the moved targets' full walks run on the side stream beside the list
+  // launch (in order before it when there is no side stream)
+  hipStream_t fs = nb.side ? nb.side : s;
+  if (nb.side) {
+    hipError_t e = hipEventRecord(nb.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(nb.side, nb.fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_fallback<Op, NT, true>), dim3(NB_QSEG * NB_FALLBACK_PARTS), dim3(NT), 0,
+                     fs, U, op, nb.args, A.cellStart, A.bcellStart, A.bpos);
   hipLaunchKernelGGL((k_listed<Op>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op, nb.args,
                      A.cellStart, A.bcellStart, A.bpos);
-  hipLaunchKernelGGL((k_fallback<Op, NT>), dim3(NB_QSEG * NB_FALLBACK_PARTS), dim3(NT), 0, s, U,
-                     op, nb.args, A.cellStart, A.bcellStart, A.bpos);
-  return hipGetLastError();
+  hipLaunchKernelGGL((k_fallback<Op, NT, false>), dim3(NB_QSEG * NB_FALLBACK_PARTS), dim3(NT), 0,
+                     s, U, op, nb.args, A.cellStart, A.bcellStart, A.bpos);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && nb.side) {
+    e = hipEventRecord(nb.join, nb.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, nb.join, 0);
+  }
+  return e;
 }
 
 // The launch of a solver op under a neighbour-mask mode (OFF: the tile
@@ -3060,8 +3141,13 @@ hipError_t run_solver(const Uniform& U, const Arrays& A, Op op, const TileCfg& t
                       const NbLaunch& nb, hipStream_t s, const BatchTab* B) {
   if (B || nb.mode == Nb::OFF || U.n == 0) return run_tile1(U, A, op, t, s, B);
   if (nb.mode == Nb::USE) {
+    // delta records travels and queues the targets it moves out of their
+    // build cells for the next iteration
     op.nbRef = nb.args.ref;
     op.nbTravel = nb.args.ctl;
+    op.nbMovedQ = nb.args.moved[nb.args.mslot ^ 1];
+    op.nbMovedCnt = nb.args.ctl + nb_moved_counts(nb.args.mslot ^ 1);
+    op.nbSegCap = nb.args.segCap;
   }
   switch (nb_threads(t)) {
     case 256: return run_nb<Op, 256>(U, A, op, nb, s);
@@ -3141,12 +3227,16 @@ hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred
 }
 
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             uint32_t* driftOut, hipStream_t s, bool nbTravel) {
+                             uint32_t* driftOut, hipStream_t s, int movedSlot) {
   if (U.nBodies == 0) return hipSuccess;
+  const bool nb = movedSlot >= 0;
   hipLaunchKernelGGL(k_body_solve, dim3(U.nBodies), dim3(WG), 0, s, pred, A.slotOf, A.bodyPids,
                      A.bodyFirst, A.restOfPid, A.bodyInvRatio, A.bodyCentreD, A.bodyRotD,
                      A.bodyCentre, A.bodyRot, ofs, A.cellOfSlot, U, driftOut,
-                     nbTravel ? A.nbRef : nullptr, nbTravel ? A.nbCtl : nullptr);
+                     nb ? A.nbRef : nullptr, nb ? A.nbCtl : nullptr,
+                     nb ? A.nbMoved[movedSlot & 1] : nullptr,
+                     nb ? A.nbCtl + nb_moved_counts(movedSlot & 1) : nullptr,
+                     nb_seg_cap((uint32_t)U.n));
   return hipGetLastError();
 }
 
@@ -3158,13 +3248,28 @@ bool reuse_masks(const Arrays& A, const TileCfg& t, const BatchTab*) {
   return A.hitMask && t.variant >= 5 && t.variant <= 8;
 }
 
+// The finish kernels under the substep's neighbour masks (list launch and
+// fallback, as the solver stages after the build), else the tile shape's
+// walk.
+template <class Op>
+hipError_t run_finish_op(const Uniform& U, const Arrays& A, const Op& op, const TileCfg& t,
+                         const NbLaunch& nb, hipStream_t s, const BatchTab* B) {
+  if (B || nb.mode != Nb::USE || U.n == 0) return run_tile2(U, A, op, t, s, B);
+  switch (nb_threads(t)) {
+    case 256: return run_nb<Op, 256>(U, A, op, nb, s);
+    case 128: return run_nb<Op, 128>(U, A, op, nb, s);
+    default: return run_nb<Op, 64>(U, A, op, nb, s);
+  }
+}
+
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
-                       hipStream_t s, const BatchTab* B) {
+                       hipStream_t s, const BatchTab* B, const NbLaunch& nb) {
   float4* p = const_cast<float4*>(pos);
   if (normals) {
     // the first chunk's hit masks go to the tension kernel that follows
     const auto keep = [&](auto op) {
+      if (nb.mode == Nb::USE && !B) return run_finish_op(U, A, drifted(op, d), t, nb, s, B);
       if (reuse_masks(A, t, B)) {
         op.maskOut = A.hitMask;
         op.maskStride = A.hitStride;
@@ -3175,15 +3280,16 @@ hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, co
     return keep(XsphFast<true>{{}, pred, A.velq, A.density, A.corr, A.normal});
   }
   if (exact)
-    return run_tile2(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, s, B);
-  return run_tile2(U, A, drifted(XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, s, B);
+    return run_finish_op(U, A, drifted(XsphOp<true, false>{{}, pred, p, A.density, A.corr, A.normal}, d), t, nb, s, B);
+  return run_finish_op(U, A, drifted(XsphFast<false>{{}, pred, A.velq, A.density, A.corr, A.normal}, d), t, nb, s, B);
 }
 
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, const Drift& d,
-                          hipStream_t s, const BatchTab* B) {
+                          hipStream_t s, const BatchTab* B, const NbLaunch& nb) {
   // the masks written by xsph+normals over the same positions
   const auto take = [&](auto op) {
+    if (nb.mode == Nb::USE && !B) return run_finish_op(U, A, drifted(op, d), t, nb, s, B);
     if (reuse_masks(A, t, B)) {
       op.maskIn = A.hitMask;
       op.maskStride = A.hitStride;

@@ -100,24 +100,39 @@ constexpr uint32_t FAT_CELL = 64;
 // since the build plus the largest travel of any particle is below the
 // skin: an unmarked candidate then stays at least h away, where every pair
 // term is zero, so the sums are the full walk's bit for bit.  Other targets
-// are queued and walked in full by a second launch.
+// are walked in full by a fallback launch: a target that has left the cell
+// it was in at the build ("moved") is queued by the kernel that moved it
+// (delta, shape matching), so its fallback walk runs on a second stream
+// beside the list launch that skips it; a target that fails only the
+// travel test (or has a row of more than 64 candidates) is queued by the
+// list launch for a fallback launch after it.
 struct NbArgs {
   uint64_t* mask;
   float4* ref;
   uint32_t* ctl;
-  uint32_t* queue[2];
+  uint32_t* queue[2];  // left by a list launch, by launch parity
+  uint32_t* moved[2];  // moved targets, by solver iteration parity
   uint32_t stride;
   uint32_t segCap;  // entries per queue segment
   float reach2;  // (h + S)^2 (build)
   float lim;     // S less a rounding margin (use)
-  int slot;      // queue of this launch
+  int slot;      // list-launch queue of this launch
+  int mslot;     // moved queue this launch's targets are in
 };
-// Each list launch appends the targets it leaves to the full walk to one
-// of NB_QSEG queue segments (by workgroup), so the appends spread over
-// NB_QSEG counters instead of contending for one.
+// The queues are NB_QSEG segments of segCap entries.  A list launch appends
+// to segment (workgroup mod NB_QSEG); a moved target i goes to segment
+// (i / 64) mod NB_QSEG, so no segment can take more than segCap entries.
 constexpr uint32_t NB_QSEG = 256;
 inline uint32_t nb_seg_cap(uint32_t n) { return ((n + 63u) / 64u + NB_QSEG - 1u) / NB_QSEG * 64u; }
-constexpr uint32_t NB_CTL_WORDS = 1 + 2 * NB_QSEG;
+// ctl: [0] travel bound, then the counts of queue[0], queue[1], moved[0],
+// moved[1] (NB_QSEG each)
+constexpr uint32_t NB_CTL_WORDS = 1 + 4 * NB_QSEG;
+__host__ __device__ inline uint32_t nb_queue_counts(int slot) {
+  return 1u + (uint32_t)slot * NB_QSEG;
+}
+__host__ __device__ inline uint32_t nb_moved_counts(int slot) {
+  return 1u + (uint32_t)(2 + slot) * NB_QSEG;
+}
 // nbRef.w bit of a target with a stencil row of more than 64 candidates
 constexpr uint32_t NB_UNLISTED = 0x80000000u;
 constexpr int NB_FALLBACK_PARTS = 4;  // workgroups per queue segment
@@ -157,9 +172,9 @@ struct Arrays {
   // after a grid build, read by the later solver kernels of the substep
   uint64_t* nbMask;     // 9 x nbStride: row r's candidates within reach of target i
   float4* nbRef;        // position at the build, w = cell at the build (bits)
-  uint32_t* nbCtl;      // [0] largest travel since the build, then the
-                        // queue counts: NB_QSEG segments per launch parity
+  uint32_t* nbCtl;      // NB_CTL_WORDS: travel bound and queue counts
   uint32_t* nbQueue[2]; // targets a list launch left to the full walk
+  uint32_t* nbMoved[2]; // targets that left their build cell, per iteration parity
   uint32_t nbStride;
   HRec* ofs[2];           // FP16 form: offsets of pred[parity] / predX
   uint32_t* drift;      // DRIFT_SLOTS displacement bounds (float bits)
@@ -268,6 +283,10 @@ enum class Nb { OFF, BUILD, USE };
 struct NbLaunch {
   Nb mode = Nb::OFF;
   NbArgs args{};
+  // the stream the moved targets' fallback runs on beside the list launch,
+  // and the events that fork it from and join it to the launch stream
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
                          bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
@@ -277,14 +296,18 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
                         const BatchTab* B = nullptr, const NbLaunch& nb = NbLaunch{});
 hipError_t launch_body_pre(const Uniform& U, const Arrays& A, const float4* pred,
                            hipStream_t s);
+// movedSlot: the moved queue of the next solver iteration when neighbour
+// masks are in use, -1 otherwise
 hipError_t launch_body_solve(const Uniform& U, const Arrays& A, float4* pred, HRec* ofs,
-                             uint32_t* driftOut, hipStream_t s, bool nbTravel = false);
+                             uint32_t* driftOut, hipStream_t s, int movedSlot = -1);
 hipError_t launch_xsph(const Uniform& U, const Arrays& A, const float4* pred, const float4* pos,
                        bool exact, bool normals, const TileCfg& t, const Drift& d,
-                       hipStream_t s, const BatchTab* B = nullptr);
+                       hipStream_t s, const BatchTab* B = nullptr,
+                       const NbLaunch& nb = NbLaunch{});
 hipError_t launch_tension(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                           float4* vel, bool exact, const TileCfg& t, const Drift& d,
-                          hipStream_t s, const BatchTab* B = nullptr);
+                          hipStream_t s, const BatchTab* B = nullptr,
+                          const NbLaunch& nb = NbLaunch{});
 hipError_t launch_finalize(const Uniform& U, const Arrays& A, const float4* pred, float4* pos,
                            float4* vel, hipStream_t s, const BatchTab* B = nullptr);
 hipError_t launch_prep_finish(const Uniform& U, const Arrays& A, const float4* pred,

@@ -137,8 +137,21 @@ struct p4a_sim {
   // list launches since the build (their queue alternates)
   bool nbBuilt = false;
   int nbLaunch = 0;
-  bool nbBuiltAfter[2] = {false, false};  // the host state a captured substep leaves
+  // solver iterations since the build: the moved queue of iteration k is
+  // nbMoved[k & 1], filled by the delta (and shape matching) of k - 1; a
+  // queue's counts are clean once a lambda list launch or the build has
+  // emptied them
+  int nbIter = 0;
+  bool nbMovedClean[2] = {false, false};
+  // the host state a captured substep leaves
+  bool nbBuiltAfter[2] = {false, false};
   int nbLaunchAfter[2] = {0, 0};
+  int nbIterAfter[2] = {0, 0};
+  bool nbMovedCleanAfter[2][2] = {{false, false}, {false, false}};
+  // the stream the moved targets' fallback walks run on beside the list
+  // launches, and the events that fork and join it
+  hipStream_t side = nullptr;
+  hipEvent_t nbFork = nullptr, nbJoin = nullptr;
   int driftCount = 0;     // drift slots written since the last grid build
   p4a_params params{};
   // scene
@@ -380,6 +393,12 @@ NbLaunch nb_launch(p4a_sim* s, bool lambda) {
   a.ctl = s->A.nbCtl;
   a.queue[0] = s->A.nbQueue[0];
   a.queue[1] = s->A.nbQueue[1];
+  a.moved[0] = s->A.nbMoved[0];
+  a.moved[1] = s->A.nbMoved[1];
+  a.mslot = s->nbIter & 1;
+  L.side = s->side;
+  L.fork = s->nbFork;
+  L.join = s->nbJoin;
   a.stride = s->A.nbStride;
   a.segCap = nb_seg_cap((uint32_t)s->n);
   a.reach2 = (float)((s->h + S) * (s->h + S));
@@ -398,11 +417,26 @@ NbLaunch nb_launch(p4a_sim* s, bool lambda) {
 
 void nb_advance(p4a_sim* s, const NbLaunch& L) {
   if (L.mode == Nb::BUILD) {
+    // the build empties every queue
     s->nbBuilt = true;
     s->nbLaunch = 0;
+    s->nbIter = 0;
+    s->nbMovedClean[0] = s->nbMovedClean[1] = true;
   } else if (L.mode == Nb::USE) {
     s->nbLaunch++;
   }
+}
+
+// Before a delta list launch: the moved queue it fills must start empty.
+// In a substep the lambda before it has emptied it; a delta that follows
+// another (the stage API) empties it with a fill kernel.
+int nb_clean_moved(p4a_sim* s, const NbLaunch& L) {
+  if (L.mode != Nb::USE) return P4A_OK;
+  const int q = L.args.mslot ^ 1;
+  if (!s->nbMovedClean[q])
+    LAUNCH(s, "delta", launch_fill_u32(s->A.nbCtl + nb_moved_counts(q), 0u, NB_QSEG, s->stream));
+  s->nbMovedClean[q] = false;
+  return P4A_OK;
 }
 
 int launch_lambda_any(p4a_sim* s, const Uniform& U) {
@@ -415,6 +449,8 @@ int launch_lambda_any(p4a_sim* s, const Uniform& U) {
     const NbLaunch nb = nb_launch(s, true);
     LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, d,
                                       s->stream, nullptr, nb));
+    // a lambda list launch empties the moved queue its iteration's delta fills
+    if (nb.mode == Nb::USE) s->nbMovedClean[nb.args.mslot ^ 1] = true;
     nb_advance(s, nb);
   }
   return P4A_OK;
@@ -429,9 +465,12 @@ int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
                                       ofs_of(s, out), direct_threads(s->tile1), d, s->stream));
   } else {
     const NbLaunch nb = nb_launch(s, false);
+    int rc = nb_clean_moved(s, nb);
+    if (rc) return rc;
     LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1, d,
                                     s->stream, nullptr, nb));
     nb_advance(s, nb);
+    if (nb.mode == Nb::USE) s->nbIter++;
     s->ofsValid = false;
   }
   s->driftCount++;
@@ -443,7 +482,7 @@ int launch_body_solve_any(p4a_sim* s, const Uniform& U) {
   LAUNCH(s, "body_solve", launch_body_solve(U, s->A, s->predCur,
                                             fp16_form(s) ? ofs_of(s, s->predCur) : nullptr,
                                             drift_slot(s), s->stream,
-                                            nb_on(s) && s->nbBuilt));
+                                            nb_on(s) && s->nbBuilt ? (s->nbIter & 1) : -1));
   s->driftCount++;
   return P4A_OK;
 }
@@ -480,12 +519,17 @@ int run_finish(p4a_sim* s, const Uniform& U) {
   if (!ex)
     LAUNCH(s, "prep_finish", launch_prep_finish(U, s->A, s->predCur, s->A.pos[par], s->stream));
   const Drift d;
+  // under the substep's neighbour masks both walk the marked candidates
+  const NbLaunch nbx = nb_launch(s, false);
   LAUNCH(s, "xsph", launch_xsph(U, s->A, s->predCur, s->A.pos[par], ex, tension, s->tile2, d,
-                                s->stream));
-  if (tension)
+                                s->stream, nullptr, nbx));
+  nb_advance(s, nbx);
+  if (tension) {
+    const NbLaunch nbt = nb_launch(s, false);
     LAUNCH(s, "tension", launch_tension(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par], ex,
-                                        s->tile2, d, s->stream));
-  else
+                                        s->tile2, d, s->stream, nullptr, nbt));
+    nb_advance(s, nbt);
+  } else
     LAUNCH(s, "finalize", launch_finalize(U, s->A, s->predCur, s->A.pos[par], s->A.vel[par],
                                           s->stream));
   return P4A_OK;
@@ -558,12 +602,18 @@ int run_substep(p4a_sim* s, double dt) {
       s->parity = par0;
       s->nbBuiltAfter[par0] = s->nbBuilt;
       s->nbLaunchAfter[par0] = s->nbLaunch;
+      s->nbIterAfter[par0] = s->nbIter;
+      s->nbMovedCleanAfter[par0][0] = s->nbMovedClean[0];
+      s->nbMovedCleanAfter[par0][1] = s->nbMovedClean[1];
     }
     HIP_TRY(s, hipGraphLaunch(s->graph[par0], s->stream));
     // host-side bookkeeping of what the graph does
     s->parity = par0 ^ 1;
     s->nbBuilt = s->nbBuiltAfter[par0];
     s->nbLaunch = s->nbLaunchAfter[par0];
+    s->nbIter = s->nbIterAfter[par0];
+    s->nbMovedClean[0] = s->nbMovedCleanAfter[par0][0];
+    s->nbMovedClean[1] = s->nbMovedCleanAfter[par0][1];
     const int iters = std::max(1, s->params.iterations);
     s->predCur = (iters % 2 == 0) ? s->A.pred[s->parity] : s->A.predX;
   }
@@ -750,7 +800,19 @@ P4A_API int p4a_create(int device, p4a_sim** out) {
     delete s;
     return P4A_ERR_HIP;
   }
+  if (hipStreamCreate(&s->side) != hipSuccess ||
+      hipEventCreateWithFlags(&s->nbFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->nbJoin, hipEventDisableTiming) != hipSuccess) {
+    if (s->nbFork) hipEventDestroy(s->nbFork);
+    if (s->side) hipStreamDestroy(s->side);
+    hipStreamDestroy(s->stream);
+    delete s;
+    return P4A_ERR_HIP;
+  }
   if (hipHostMalloc((void**)&s->statsHost, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    hipEventDestroy(s->nbFork);
+    hipEventDestroy(s->nbJoin);
+    hipStreamDestroy(s->side);
     hipStreamDestroy(s->stream);
     delete s;
     return P4A_ERR_NOMEM;
@@ -769,6 +831,7 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   if (!s) return P4A_ERR_ARG;
   hipSetDevice(s->device);
   hipStreamSynchronize(s->stream);
+  hipStreamSynchronize(s->side);
   drop_graphs(s);
   free_list(s->devAllocs);
   free_list(s->cellAllocs);
@@ -780,6 +843,9 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->exportAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
+  hipEventDestroy(s->nbFork);
+  hipEventDestroy(s->nbJoin);
+  hipStreamDestroy(s->side);
   hipStreamDestroy(s->stream);
   delete s;
   return P4A_OK;
@@ -869,8 +935,10 @@ P4A_API int p4a_reset(p4a_sim* s, const p4a_params* p, const p4a_scene* sc) {
   A.nbStride = (uint32_t)cap;
   ALLOC(A.nbRef, cap);
   ALLOC(A.nbCtl, NB_CTL_WORDS);
-  ALLOC(A.nbQueue[0], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
-  ALLOC(A.nbQueue[1], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
+  for (int k = 0; k < 2; k++) {
+    ALLOC(A.nbQueue[k], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
+    ALLOC(A.nbMoved[k], (size_t)NB_QSEG * nb_seg_cap((uint32_t)cap));
+  }
   ALLOC(A.order, cap);
   ALLOC(A.slot, cap);
   ALLOC(A.slotOf, cap);

@@ -198,7 +198,7 @@ class Sim {
       // FP16-offset records as raw 32-bit words: float x, then half y | half z
       ofs: ['u32', n * 2], cellOfSlot: ['u32', n],
       // neighbour masks: travel bound (float bits), then the queue counts
-      neighbourState: ['u32', 513],
+      neighbourState: ['u32', 1025],
     }[name];
     if (!shape) throw new Error(`unknown buffer ${name}`);
     return native.read(this.handle, BUF[name], shape[0], shape[1]);

@@ -29,7 +29,7 @@ def test_js_sim_verify_harness_and_interactions():
     assert abs(out["boxX"] - 1.5) > 0.05 and out["gen"] >= 2
     assert out["impulsed"] > 0
     assert 0.9 < out["stats"]["avgRho"] < 1.1
-    assert out["masksSameBits"] and out["neighbourStateLen"] == 513
+    assert out["masksSameBits"] and out["neighbourStateLen"] == 1025
 
 
 def test_js_surface_mesh_field():

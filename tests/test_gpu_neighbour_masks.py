@@ -17,12 +17,20 @@ most targets are queued.
   particle from its build position after the deltas (read back stage by
   stage), and the queues hold a few percent of the targets at most on the
   flowing state;
+* the moved queues: after each delta and shape matching, the queue of the
+  next iteration holds exactly the targets that left the cell they were
+  binned into (their full walks run beside the list launches, which skip
+  them); deltas run back to back through the stage API refill the queue
+  from empty and still give the full walk's bits;
 * skins outside [0, 1] and non-finite skins are refused.
 """
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from particles4all_amd import Sim, workloads
+
+MOVED = 1 + 2 * 256  # NEIGHBOUR_STATE: first count of the moved queues
 
 pytestmark = pytest.mark.gpu
 
@@ -120,3 +128,47 @@ def test_bad_skins_are_refused():
         sim.set_params(p)
     sim.substeps(1 / 120, 2)
     sim.close()
+
+
+def test_moved_queues_hold_the_targets_that_left_their_build_cells():
+    p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
+    P = O.constants_from(p, sc, dt)
+    runs = {}
+    for skin in (0.1, 0.0):
+        p.neighbour_skin = skin
+        sim = Sim(0)
+        sim.reset(p, sc)
+        sim.substeps(dt, 60)
+        sim.run_stage("PREDICT", dt)
+        sim.run_stage("GRID", dt)
+        sim.run_stage("BODY_PRE", dt)
+        sim.run_stage("LAMBDA", dt)
+        built = O.cell_ids(O.F32, P, sim.read("PRED"))
+
+        def moved_count(slot):
+            st = sim.read("NEIGHBOUR_STATE")
+            return int(st[MOVED + 256 * slot: MOVED + 256 * (slot + 1)].sum())
+
+        it = 0
+        stages = []
+        for k in range(p.iterations):
+            stages += (["LAMBDA"] if k else []) + ["DELTA", "BODY_SOLVE"]
+        stages += ["DELTA", "DELTA", "BODY_SOLVE"]  # deltas back to back
+        for st in stages:
+            sim.run_stage(st, dt)
+            if st == "LAMBDA":
+                continue
+            q = O.cell_ids(O.F32, P, sim.read("PRED")) != built
+            if st == "DELTA":
+                it += 1
+                out = q
+            else:
+                out = out | q  # shape matching queues the ones it moves out
+            if skin > 0:
+                got = moved_count(it & 1)
+                print(f"{st} (iteration {it}): queued {got}, left their cells {int(q.sum())}")
+                assert got == int(out.sum())
+        sim.run_stage("FINISH", dt)
+        runs[skin] = {k: sim.read(k).copy() for k in FIELDS}
+        sim.close()
+    same(runs[0.0], runs[0.1], "back-to-back deltas")

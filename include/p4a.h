@@ -92,6 +92,11 @@ typedef struct p4a_params {
                                 solver kernel walks the full 27-cell stencil).
                                 The results are the full walk's bit for bit
                                 either way.  No counterpart in the reference. */
+  int32_t neighbour_min_particles; /* the masks are used for scenes of at
+                                least this many particles (default 524288:
+                                below it their extra launches cost more than
+                                the candidates they save); >= 0.  No
+                                counterpart in the reference. */
 } p4a_params;
 
 /* Fill `p` with defaultParams('small') (src/scene.js:14-31). */

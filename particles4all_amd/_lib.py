@@ -38,7 +38,8 @@ class Params(C.Structure):
                 ("sor_average", C.c_int32), ("surface_tension_k", C.c_double),
                 ("box", C.c_double * 3), ("no_boundary", C.c_int32),
                 ("pour_budget", C.c_int32), ("precision", C.c_int32), ("math", C.c_int32),
-                ("use_graph", C.c_int32), ("neighbour_skin", C.c_double)]
+                ("use_graph", C.c_int32), ("neighbour_skin", C.c_double),
+                ("neighbour_min_particles", C.c_int32)]
 
 
 class BodySpec(C.Structure):

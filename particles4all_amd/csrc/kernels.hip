@@ -1000,7 +1000,11 @@ constexpr uint32_t NB_STAGE = 256;
 // has consumed, and (lambda launches) the moved queue this iteration's
 // delta will fill, whose previous contents the last iteration walked.
 template <class Op>
+// Eight waves per SIMD for the one-record ops: the scalar registers of the
+// full-walk path must not cost the list walk a wave (the two-record ops
+// would spill at the 64 vector registers that takes, and keep their own)
 __global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : 1, 8)))
 k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
@@ -1018,7 +1022,7 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
   typename Op::Target t;
   Cell c{0, 0, 0};
   bool valid = false;  // the masks hold
-  bool full = false;   // walked here with every candidate of its rows
+  bool fails = false;  // in its build cell, but the masks do not hold
   // appends the lanes with q set to this launch's queue (one atomic per wave)
   const auto enqueue = [&](bool q) {
     const uint64_t qm = __ballot(q);
@@ -1042,86 +1046,70 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     const float travel = sqrtf(r2_of(t.px - ref.x, t.py - ref.y, t.pz - ref.z));
     const float most = __uint_as_float(*(volatile const uint32_t*)nb.ctl);
     valid = stays && built == cell_index(U, c) && travel + most <= nb.lim;
-    // in the exact ops, a wave with a quarter of its targets or more failing
-    // the test (all of them once any particle has travelled past the skin)
-    // walks those in full here, through its own staged rows: the fallback
-    // launch is sized for a few targets per lane.  The fast ops queue them
-    // all: their fallback walks are cheap enough that the extra registers
-    // would cost the list launch more than they save.
-    const bool fails = stays && !valid;
-    full = !Op::FAST && fails && __popcll(__ballot(fails)) >= 16;
-    enqueue(fails && !full);
+    fails = stays && !valid;
   }
-  if (!__ballot(valid || full)) return;  // the whole wave is queued (or past n)
-  const int xlo = max(c.x - 1, 0);
-  const int xhi = min(c.x + 1, gx - 1);
-  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 || c.z <= 1 ||
-                        c.z >= gz - 2;
-  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && (valid || full);
-  bool seen = false;  // a full lane's own record lay in a visited row
-  bool fat = false;   // a full lane met a row of more than 64 candidates
-  for (int dz = -1; dz <= 1; dz++) {
-    for (int dy = -1; dy <= 1; dy++) {
-      const int z = c.z + dz, y = c.y + dy;
-      const bool rowOk = (valid || full) && z >= 0 && z < gz && y >= 0 && y < gy;
-      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
-      uint32_t b = 0, e = 0;
-      uint64_t m64 = 0;
-      if (rowOk) {
-        b = cellStart[rb + xlo];
-        e = cellStart[rb + xhi + 1];
-        if (valid) {
+  // A wave with a quarter of its targets or more failing the test (all of
+  // them once some particle has travelled past the skin) walks those in
+  // full itself, after the list walk of the others: the fallback launch is
+  // sized for a few targets per lane.  Sparser failures are queued for it.
+  const bool dense = __popcll(__ballot(fails)) >= 16;
+  enqueue(fails && !dense);
+  if (__ballot(valid)) {
+    const int xlo = max(c.x - 1, 0);
+    const int xhi = min(c.x + 1, gx - 1);
+    const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
+                          c.z <= 1 || c.z >= gz - 2;
+    const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && valid;
+    for (int dz = -1; dz <= 1; dz++) {
+      for (int dy = -1; dy <= 1; dy++) {
+        const int z = c.z + dz, y = c.y + dy;
+        const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+        const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+        uint32_t b = 0, e = 0;
+        uint64_t m64 = 0;
+        if (rowOk) {
+          b = cellStart[rb + xlo];
+          e = cellStart[rb + xhi + 1];
           m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
-        } else {
-          // the full walk: every candidate of the row, in order
-          seen = seen || (i >= b && i < e);
-          fat = fat || e - b > 64u;
-          if (!fat) m64 = e - b >= 64u ? ~0ull : (1ull << (e - b)) - 1ull;
+        }
+        // union of the wave's ranges that hold a hit (every lane is active
+        // here, the others contribute the neutral values)
+        const bool has = m64 != 0ull;
+        const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
+        const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
+        if (ub < ue && ue - ub <= STAGE) {
+          const uint32_t span = ue - ub;
+          for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
+            float4 v[Op::NREC];
+            op.load(U, (size_t)ub + q, v);
+#pragma unroll
+            for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+          }
+          __syncthreads();
+          const auto rec = [&](uint32_t j, float4* v) {
+#pragma unroll
+            for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
+          };
+          hit_loop1(U, op, t, (uint32_t)m64, b, rec);
+          // rows of more than 32 candidates are rare: the upper half only
+          // when some lane has hits there
+          if (__ballot((m64 >> 32) != 0ull))
+            hit_loop1(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+          __syncthreads();
+        } else if (has) {
+          const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
+          hit_loop(U, op, t, (uint32_t)m64, b, rec);
+          hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+        }
+        if (doB && rowOk) {
+          const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+          for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
         }
       }
-      // union of the wave's ranges that hold a hit (every lane is active
-      // here, the others contribute the neutral values)
-      const bool has = m64 != 0ull;
-      const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
-      const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
-      if (ub < ue && ue - ub <= STAGE) {
-        const uint32_t span = ue - ub;
-        for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
-          float4 v[Op::NREC];
-          op.load(U, (size_t)ub + q, v);
-#pragma unroll
-          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
-        }
-        __syncthreads();
-        const auto rec = [&](uint32_t j, float4* v) {
-#pragma unroll
-          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
-        };
-        hit_loop1(U, op, t, (uint32_t)m64, b, rec);
-        // rows of more than 32 candidates are rare: the upper half only
-        // when some lane has hits there
-        if (__ballot((m64 >> 32) != 0ull)) hit_loop1(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
-        __syncthreads();
-      } else if (has) {
-        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
-        hit_loop(U, op, t, (uint32_t)m64, b, rec);
-        hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
-      }
-      if (doB && rowOk && !fat) {
-        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
-        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
-      }
     }
+    if (valid) op.finish(U, i, t);
   }
-  // a full lane that met a fat row leaves its partial sums for the fallback
-  enqueue(full && fat);
-  if (full && !fat) {
-    if constexpr (Op::FAST) {
-      if (!seen) op.self_outside(U, t);
-    }
-    op.finish(U, i, t);
-  }
-  if (valid) op.finish(U, i, t);
+  if (dense && fails) direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
 }
 
 // The full 27-cell walk of a target a list launch left (k_fallback).  Few

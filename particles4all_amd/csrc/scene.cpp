@@ -328,6 +328,7 @@ P4A_API int p4a_params_default(p4a_params* p) {
   p->math = P4A_MATH_FAST;
   p->use_graph = 1;
   p->neighbour_skin = 0.1;
+  p->neighbour_min_particles = 1 << 19;
   return P4A_OK;
 }
 

@@ -354,6 +354,7 @@ const char* form_error(const p4a_params* p) {
     return "the FP16-offset form runs with P4A_MATH_FAST only";
   if (!(p->neighbour_skin >= 0.0 && p->neighbour_skin <= 1.0))
     return "neighbour_skin must lie in [0, 1]";
+  if (p->neighbour_min_particles < 0) return "neighbour_min_particles must be >= 0";
   return nullptr;
 }
 bool fp16_form(const p4a_sim* s) { return s->params.precision == P4A_PREC_FP16_OFFSET; }
@@ -379,7 +380,10 @@ uint32_t* drift_slot(const p4a_sim* s) {
   return s->driftCount < DRIFT_SLOTS ? s->A.drift + s->driftCount : nullptr;
 }
 
-bool nb_on(const p4a_sim* s) { return !fp16_form(s) && s->params.neighbour_skin > 0.0; }
+bool nb_on(const p4a_sim* s) {
+  return !fp16_form(s) && s->params.neighbour_skin > 0.0 &&
+         s->n >= s->params.neighbour_min_particles;
+}
 
 // Mask arguments of the next solver launch: BUILD for the first lambda
 // after the masks were dropped, USE once built, OFF otherwise.
@@ -558,7 +562,8 @@ std::string graph_key(const p4a_sim* s, const Uniform& U) {
   k += std::to_string(s->params.iterations) + "/" + std::to_string(s->params.math) + "/" +
        std::to_string(s->params.precision) + "/" +
        std::to_string(s->params.surface_tension_k > 0) + "/" + std::to_string(s->tile1.variant) +
-       "/" + std::to_string(s->tile2.variant) + "/" + std::to_string(s->params.neighbour_skin);
+       "/" + std::to_string(s->tile2.variant) + "/" + std::to_string(s->params.neighbour_skin) +
+       "/" + std::to_string(nb_on(s));
   return k;
 }
 
@@ -1094,6 +1099,7 @@ P4A_API int p4a_set_params(p4a_sim* s, const p4a_params* p) {
   q.precision = p->precision;
   q.use_graph = p->use_graph;
   q.neighbour_skin = p->neighbour_skin;
+  q.neighbour_min_particles = p->neighbour_min_particles;
   s->nbBuilt = false;
   return P4A_OK;
 }

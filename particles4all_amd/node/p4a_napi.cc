@@ -126,6 +126,8 @@ void params_from(napi_env env, napi_value o, p4a_params* p) {
   p->math = (int32_t)pnum(env, o, "math", 0);
   p->use_graph = (int32_t)pnum(env, o, "useGraph", 1);
   p->neighbour_skin = pnum(env, o, "neighbourSkin", p->neighbour_skin);
+  p->neighbour_min_particles =
+      (int32_t)pnum(env, o, "neighbourMinParticles", p->neighbour_min_particles);
 }
 
 void finalize_scene(napi_env, void* data, void*) { p4a_scene_free((p4a_scene_data*)data); }

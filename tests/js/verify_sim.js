@@ -132,7 +132,7 @@ sim.destroy();
 // walks) give the same bits; the mask state reads back
 {
   const a = new Sim(0), b = new Sim(0);
-  a.reset(params);
+  a.reset(Object.assign({}, params, { neighbourMinParticles: 0 }));
   b.reset(Object.assign({}, params, { neighbourSkin: 0 }));
   for (let f = 0; f < 4; f++) { a.step(1 / 60); b.step(1 / 60); }
   const pa = a.read('pos'), pb = b.read('pos');

@@ -8,7 +8,9 @@ bit and the body centres agree to FP64 rounding (both sides sum in FP64, in
 different orders; the reductions here are deterministic trees); every frame's
 stats equal the oracle's reduction (maxima exactly, sums to FP64 rounding).
 Nothing is re-synchronised: the GPU and the oracle each carry their own state
-through all 240 substeps.
+through all 240 substeps.  The run is made twice: with the library's default
+(full 27-cell walks at this size) and with the per-substep neighbour masks
+forced on.
 """
 import time
 
@@ -27,8 +29,11 @@ def rd(sim, name):
     return sim.read(name).astype(F)
 
 
-def test_120_frames_bit_exact():
+@pytest.mark.parametrize("masks", [False, True])
+def test_120_frames_bit_exact(masks):
     p, sc, frame = workloads.dam_break_32(scale=1.0, math=1)
+    if masks:
+        p.neighbour_min_particles = 0
     assert sc.n == 32 ** 3 + 2 * 8 ** 3 and sc.n_bodies == 2 and p.substeps == 2
     sim = Sim(0)
     sim.reset(p, sc)

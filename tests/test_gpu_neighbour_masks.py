@@ -22,7 +22,10 @@ most targets are queued.
   binned into (their full walks run beside the list launches, which skip
   them); deltas run back to back through the stage API refill the queue
   from empty and still give the full walk's bits;
-* skins outside [0, 1] and non-finite skins are refused.
+* the masks are used from neighbour_min_particles on (by default only for
+  scenes of 2^19 particles or more), and the threshold gives the same bits;
+* skins outside [0, 1], non-finite skins and negative thresholds are
+  refused.
 """
 import numpy as np
 import pytest
@@ -40,6 +43,7 @@ FIELDS = ("POS", "VEL", "DENSITY", "LAMBDA", "PID")
 def run(maker, skin, math, steps, **kw):
     p, sc, _ = maker(math=math, **kw)
     p.neighbour_skin = skin
+    p.neighbour_min_particles = 0  # these scenes are below the default size
     sim = Sim(0)
     sim.reset(p, sc)
     dt = 1 / 120
@@ -78,6 +82,7 @@ def test_masks_equal_the_full_walk(math, maker, kw):
 def test_travel_bound_and_queues_on_a_flowing_state():
     p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
     p.neighbour_skin = 0.0
+    p.neighbour_min_particles = 0
     sim = Sim(0)
     sim.reset(p, sc)
     sim.substeps(dt, 120)
@@ -126,6 +131,10 @@ def test_bad_skins_are_refused():
     p.neighbour_skin = -1.0
     with pytest.raises(RuntimeError, match="P4A_ERR_ARG"):
         sim.set_params(p)
+    p.neighbour_skin = 0.1
+    p.neighbour_min_particles = -1
+    with pytest.raises(RuntimeError, match="P4A_ERR_ARG"):
+        sim.set_params(p)
     sim.substeps(1 / 120, 2)
     sim.close()
 
@@ -134,6 +143,7 @@ def test_moved_queues_hold_the_targets_that_left_their_build_cells():
     p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
     P = O.constants_from(p, sc, dt)
     runs = {}
+    p.neighbour_min_particles = 0
     for skin in (0.1, 0.0):
         p.neighbour_skin = skin
         sim = Sim(0)
@@ -172,3 +182,24 @@ def test_moved_queues_hold_the_targets_that_left_their_build_cells():
         runs[skin] = {k: sim.read(k).copy() for k in FIELDS}
         sim.close()
     same(runs[0.0], runs[0.1], "back-to-back deltas")
+
+
+def test_masks_only_from_the_particle_threshold():
+    p, sc, dt = workloads.dam_break_32(math=0)
+    n = sc.n
+    default = p.neighbour_min_particles
+    assert default == 1 << 19
+    out = {}
+    for lo in (n + 1, n, None):
+        p.neighbour_min_particles = default if lo is None else lo
+        sim = Sim(0)
+        sim.reset(p, sc)
+        sim.substeps(dt, 4)
+        st = sim.read("NEIGHBOUR_STATE")
+        used = bool(st.any())
+        out[lo] = {k: sim.read(k).copy() for k in FIELDS}
+        sim.close()
+        # the default threshold (2^19) is above this scene
+        assert used == (lo == n), (lo, used)
+    same(out[n + 1], out[n], "threshold")
+    same(out[n + 1], out[None], "default threshold")

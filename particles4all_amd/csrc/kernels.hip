@@ -715,8 +715,17 @@ k_tile(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // neighbour masks: bit k of row r's mask marks candidate b_r + k within
 // reach (r^2 < nb->reach2), and the build position and cell go to nbRef.
 // Bit for candidate k of a row (k < 64) when r^2 is within reach.
-__device__ __forceinline__ uint64_t reach_bit(float r2, float reach2, uint32_t k) {
-  return (r2 < reach2 && k < 64u) ? (1ull << (k & 63u)) : 0ull;
+// The mask build shifts each candidate's reach test in from the right (one
+// shift-and-add per candidate), so the first of a row's len candidates ends
+// at bit len - 1; the stored mask has candidate k at bit k.  A row of more
+// than 64 candidates sends its target to the full walk, and its mask is
+// never read.
+__device__ __forceinline__ uint64_t reach_in(uint64_t acc, float r2, float reach2) {
+  return (acc << 1) + (r2 < reach2 ? 1ull : 0ull);
+}
+__device__ __forceinline__ uint64_t reach_mask(uint64_t acc, uint32_t len) {
+  if (len == 0u) return 0ull;
+  return __builtin_bitreverse64(acc) >> (64u - min(len, 64u));
 }
 
 template <class Op, int NT, int CR, int PH = 0, bool LB = false>
@@ -844,8 +853,8 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           if constexpr (LB) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              rowMask |= reach_bit(r2_of(t.px - v[u][0].x, t.py - v[u][0].y, t.pz - v[u][0].z),
-                                   nb->reach2, (uint32_t)(j - b) + u);
+              rowMask = reach_in(rowMask, r2_of(t.px - v[u][0].x, t.py - v[u][0].y,
+                                                t.pz - v[u][0].z), nb->reach2);
           }
 #pragma unroll
           for (int u = 0; u < 4; u++) op.fpair(U, t, v[u]);
@@ -854,8 +863,8 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           float4 v[Op::NREC];
           op.load(U, j, v);
           if constexpr (LB)
-            rowMask |= reach_bit(r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z), nb->reach2,
-                                 (uint32_t)(j - b));
+            rowMask = reach_in(rowMask, r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z),
+                               nb->reach2);
           op.fpair(U, t, v);
         }
       } else {
@@ -868,8 +877,8 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           if constexpr (LB) {
 #pragma unroll
             for (int u = 0; u < 4; u++)
-              rowMask |= reach_bit(r2_of(t.px - v[u][0].x, t.py - v[u][0].y, t.pz - v[u][0].z),
-                                   nb->reach2, j - b + u);
+              rowMask = reach_in(rowMask, r2_of(t.px - v[u][0].x, t.py - v[u][0].y,
+                                                t.pz - v[u][0].z), nb->reach2);
           }
 #pragma unroll
           for (int u = 0; u < 4; u++) op.pair(U, t, j + u, v[u]);
@@ -878,13 +887,13 @@ direct_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restri
           float4 v[Op::NREC];
           op.load(U, j, v);
           if constexpr (LB)
-            rowMask |= reach_bit(r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z), nb->reach2,
-                                 j - b);
+            rowMask = reach_in(rowMask, r2_of(t.px - v[0].x, t.py - v[0].y, t.pz - v[0].z),
+                               nb->reach2);
           op.pair(U, t, j, v);
         }
       }
       if constexpr (LB)
-        nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] = rowMask;
+        nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] = reach_mask(rowMask, e - b);
       if (doB) {
         const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);

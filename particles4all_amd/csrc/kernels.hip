@@ -41,6 +41,8 @@
 // hardware reciprocal square root and constants are folded.
 #include "p4a_internal.h"
 
+#include <type_traits>
+
 namespace p4a {
 namespace {
 
@@ -1121,6 +1123,16 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
   if (dense && fails) direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
 }
 
+// The 27-cell walk over FP16-offset records (defined with those records'
+// ops below).
+template <class Op, bool LB = false>
+__device__ void direct16_walk(const Uniform& U, const Op& op, uint32_t i,
+                              const uint32_t* __restrict__ cellStart,
+                              const uint32_t* __restrict__ bcellStart,
+                              const float4* __restrict__ bpos, const HRec* __restrict__ ofs,
+                              const uint32_t* __restrict__ cellOfSlot,
+                              const NbArgs* nb = nullptr);
+
 // The full 27-cell walk of a target a list launch left (k_fallback).  Few
 // waves run it (about one per SIMD, each lane on a target of its own), so
 // its time is the latency of its loads rather than issue: each row's
@@ -1195,10 +1207,14 @@ __device__ void wide_walk(const Uniform& U, const Op& op, uint32_t i,
 // beside it skips.  A target is walked only if it is still outside its
 // build cell: shape matching after the delta that queued it may have moved
 // it back, and then the list launch takes it.
-template <class Op, int NT, bool MOVED>
+//
+// F16: the FP16-offset form's walk over its records (ofs, cellOfSlot).
+template <class Op, int NT, bool MOVED, bool F16 = false>
 __global__ void __launch_bounds__(NT)
 k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
-           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+           const HRec* __restrict__ ofs = nullptr,
+           const uint32_t* __restrict__ cellOfSlot = nullptr) {
   static_assert(NT >= 64 && NB_QSEG == 256, "one wave scans 4 counts per lane");
   __shared__ uint32_t pre[NB_QSEG + 1];
   const int tid = threadIdx.x;
@@ -1241,7 +1257,10 @@ k_fallback(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
       const uint32_t built = __float_as_uint(nb.ref[item].w) & ~NB_UNLISTED;
       if (cell_index(U, cell_of(U, p.x, p.y, p.z)) == built) continue;
     }
-    wide_walk<Op>(U, op, item, cellStart, bcellStart, bpos);
+    if constexpr (F16)
+      direct16_walk<Op>(U, op, item, cellStart, bcellStart, bpos, ofs, cellOfSlot);
+    else
+      wide_walk<Op>(U, op, item, cellStart, bcellStart, bpos);
   }
 }
 
@@ -1890,13 +1909,25 @@ __device__ __forceinline__ float dh(float a, __half o, float minusOne) {
   return __fmaf_rn(__half2float(o), minusOne, a);
 }
 
-template <class Op, int NT>
-__global__ void __launch_bounds__(NT)
-k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
-           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
-           const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot) {
-  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= U.n) return;
+// r^2 between a target and an FP16-offset record of a stencil row whose
+// y and z cell-centre offsets from the target are ay, az (the pair terms'
+// own arithmetic)
+template <class T>
+__device__ __forceinline__ float r2_16(const Uniform& U, const T& t, float ay, float az,
+                                       HRec o) {
+  return r2_of(t.px - o.x, dh(ay, o.y, U.minusOne), dh(az, o.z, U.minusOne));
+}
+
+// The 27-cell walk of target i over FP16-offset records.  LB (the first
+// lambda after a grid build): also marks each row's candidates within reach
+// of the target's quantized position and stores its position and cell, as
+// the FP32 mask build does.
+template <class Op, bool LB>
+__device__ __forceinline__ void
+direct16_walk(const Uniform& U, const Op& op, uint32_t i, const uint32_t* __restrict__ cellStart,
+              const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+              const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot,
+              const NbArgs* nb) {
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
   typename Op::Target t;
   op.target(U, i, t);
@@ -1910,6 +1941,7 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
                         c.z >= gz - 2;
   const bool doB = U.nBoundary > 0 && nearWall;
   bool seen = false;  // the target's own record lay in a visited row
+  bool unlisted = false;
   for (int dz = -1; dz <= 1; dz++) {
     const int z = c.z + dz;
     if (z < 0 || z >= gz) continue;
@@ -1921,6 +1953,8 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
       const uint32_t rb = (uint32_t)((z * gy + y) * gx);
       const uint32_t b = cellStart[rb + xlo], e = cellStart[rb + xhi + 1];
       seen = seen || (i >= b && i < e);
+      uint64_t rowMask = 0;
+      if constexpr (LB) unlisted = unlisted || (e - b > 64u);
       uint32_t j = b;
       for (; j + 3 < e; j += 4) {
         HRec r[4];
@@ -1928,6 +1962,11 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         // one 64-bit base per group, the others by immediate offsets
 #pragma unroll
         for (int u = 0; u < 4; u++) op.load16(ofs, (size_t)j + u, r[u], l[u]);
+        if constexpr (LB) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) rowMask = reach_in(rowMask, r2_16(U, t, ay, az, r[u]),
+                                                         nb->reach2);
+        }
 #pragma unroll
         for (int u = 0; u < 4; u++) op.fpair16(U, t, ay, az, r[u], l[u]);
       }
@@ -1935,8 +1974,11 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
         HRec r;
         float l;
         op.load16(ofs, (size_t)j, r, l);
+        if constexpr (LB) rowMask = reach_in(rowMask, r2_16(U, t, ay, az, r), nb->reach2);
         op.fpair16(U, t, ay, az, r, l);
       }
+      if constexpr (LB)
+        nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] = reach_mask(rowMask, e - b);
       if (doB) {
         const float sy = t.py, sz = t.pz;
         t.py = cell_centre(U, 1, by) + oy;
@@ -1948,11 +1990,40 @@ k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
     }
   }
   if (!seen) op.self_outside(U, t);
+  if constexpr (LB)
+    nb->ref[i] = make_float4(
+        t.px, t.py, t.pz,
+        __uint_as_float(cell_index(U, c) | ((unlisted || !seen) ? NB_UNLISTED : 0u)));
   op.finish16(U, i, t);
+}
+
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_direct16(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+           const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot) {
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  direct16_walk<Op>(U, op, i, cellStart, bcellStart, bpos, ofs, cellOfSlot);
+}
+
+// The first FP16 lambda after a grid build: the full walk, building the
+// masks (and restarting the travel bound and the queues).
+template <class Op, int NT>
+__global__ void __launch_bounds__(NT)
+k_direct16_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+                 const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+                 const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot) {
+  if (blockIdx.x == 0)
+    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += NT) nb.ctl[k] = 0u;
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (i >= U.n) return;
+  direct16_walk<Op, true>(U, op, i, cellStart, bcellStart, bpos, ofs, cellOfSlot, &nb);
 }
 
 // lambda over FP16-offset records
 struct Lambda16 : LambdaFast {
+  static constexpr bool USES_L = false;  // no neighbour lambda in the records
   __device__ void load16(const HRec* ofs, size_t j, HRec& r, float& l) const {
     r = ofs[j];
     l = 0.0f;
@@ -1985,6 +2056,7 @@ using DeltaFast = DeltaFastT<true>;
 
 template <bool SOR>
 struct Delta16T : DeltaFastT<SOR> {
+  static constexpr bool USES_L = true;
   using Target = typename DeltaFastT<SOR>::Target;
   using DeltaFastT<SOR>::lambda;
   using DeltaFastT<SOR>::out;
@@ -2018,6 +2090,161 @@ struct Delta16T : DeltaFastT<SOR> {
     ofsOut[i] = encode_ofs(U, cellOfSlot[i], np.x, np.y, np.z);
   }
 };
+
+// A list launch over FP16-offset records: k_listed's walk (the same mask
+// test, moved targets skipped, dense failures walked in full, sparse ones
+// queued), with each row's 8-byte records (and, for delta, the neighbours'
+// lambdas) staged in LDS and the pair terms taken at the target's quantized
+// offsets from the row's cell centres, as the full FP16 walk takes them.
+template <class Op>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_listed16(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+           const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos,
+           const HRec* __restrict__ ofs, const uint32_t* __restrict__ cellOfSlot) {
+  // one LDS slot per candidate: the record (8 bytes) or, with the
+  // neighbour's lambda, record and lambda in one 16-byte slot (one read)
+  using Slot = typename std::conditional<Op::USES_L, float4, HRec>::type;
+  __shared__ Slot SR[NB_STAGE];
+  const auto put = [](Slot& d, HRec r, float l) {
+    if constexpr (Op::USES_L) {
+      uint32_t yz;
+      __builtin_memcpy(&yz, &r.y, 4);
+      d = make_float4(r.x, __uint_as_float(yz), l, 0.0f);
+    } else {
+      d = r;
+    }
+  };
+  const auto get = [](const Slot& v, HRec& r, float& l) {
+    if constexpr (Op::USES_L) {
+      r.x = v.x;
+      const uint32_t yz = __float_as_uint(v.y);
+      __builtin_memcpy(&r.y, &yz, 4);
+      l = v.z;
+    } else {
+      r = v;
+      l = 0.0f;
+    }
+  };
+  if (blockIdx.x == 0) {
+    for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
+      nb.ctl[nb_queue_counts(nb.slot ^ 1) + k] = 0u;
+    if (Op::CLEARS_MOVED)
+      for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
+        nb.ctl[nb_moved_counts(nb.mslot ^ 1) + k] = 0u;
+  }
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  const int lane = __lane_id();
+  typename Op::Target t;
+  Cell c{0, 0, 0};
+  bool valid = false, fails = false;
+  const auto enqueue = [&](bool q) {
+    const uint64_t qm = __ballot(q);
+    if (!qm) return;
+    const int leader = __builtin_ctzll(qm);
+    uint32_t base = 0;
+    const uint32_t seg = blockIdx.x % NB_QSEG;
+    if (lane == leader)
+      base = atomicAdd(&nb.ctl[nb_queue_counts(nb.slot) + seg], (uint32_t)__popcll(qm));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    const uint32_t at = base + (uint32_t)__popcll(qm & ((1ull << lane) - 1ull));
+    if (q && at < nb.segCap) nb.queue[nb.slot][seg * nb.segCap + at] = i;
+  };
+  if (i < U.n) {
+    op.target(U, i, t);
+    c = cell_of(U, t.px, t.py, t.pz);
+    const float4 ref = nb.ref[i];
+    const uint32_t built = __float_as_uint(ref.w);
+    const bool stays = (built & ~NB_UNLISTED) == cell_index(U, c);
+    const float travel = sqrtf(r2_of(t.px - ref.x, t.py - ref.y, t.pz - ref.z));
+    const float most = __uint_as_float(*(volatile const uint32_t*)nb.ctl);
+    valid = stays && built == cell_index(U, c) && travel + most <= nb.lim;
+    fails = stays && !valid;
+  }
+  const bool dense = __popcll(__ballot(fails)) >= 16;
+  enqueue(fails && !dense);
+  if (__ballot(valid)) {
+    int bx = 0, by = 0, bz = 0;
+    float oy = 0.0f, oz = 0.0f;
+    if (valid) {
+      cell_xyz(U, cellOfSlot[i], bx, by, bz);
+      const HRec ri = ofs[i];
+      oy = __half2float(ri.y);
+      oz = __half2float(ri.z);
+    }
+    const int xlo = max(c.x - 1, 0);
+    const int xhi = min(c.x + 1, gx - 1);
+    const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
+                          c.z <= 1 || c.z >= gz - 2;
+    const bool doB = U.nBoundary > 0 && nearWall && valid;
+    for (int dz = -1; dz <= 1; dz++) {
+      for (int dy = -1; dy <= 1; dy++) {
+        const int z = c.z + dz, y = c.y + dy;
+        const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+        const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+        const float az = __fmaf_rn((float)(bz - z), U.h, oz);
+        const float ay = __fmaf_rn((float)(by - y), U.h, oy);
+        uint32_t b = 0, e = 0;
+        uint64_t m64 = 0;
+        if (rowOk) {
+          b = cellStart[rb + xlo];
+          e = cellStart[rb + xhi + 1];
+          m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
+        }
+        const bool has = m64 != 0ull;
+        const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
+        const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
+        if (ub < ue && ue - ub <= NB_STAGE) {
+          const uint32_t span = ue - ub;
+          for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
+            HRec r;
+            float l;
+            op.load16(ofs, (size_t)ub + q, r, l);
+            put(SR[q], r, l);
+          }
+          __syncthreads();
+          for (int half = 0; half < 2; half++) {
+            uint32_t m = (uint32_t)(m64 >> (32 * half));
+            if (half && !__ballot(m != 0u)) break;
+            const uint32_t base = b + 32u * (uint32_t)half - ub;
+            while (m) {
+              const uint32_t k = base + (uint32_t)__builtin_ctz(m);
+              m &= m - 1u;
+              HRec r;
+              float l;
+              get(SR[k], r, l);
+              op.fpair16(U, t, ay, az, r, l);
+            }
+          }
+          __syncthreads();
+        } else if (has) {
+          for (int half = 0; half < 2; half++) {
+            uint32_t m = (uint32_t)(m64 >> (32 * half));
+            while (m) {
+              const uint32_t j = b + 32u * (uint32_t)half + (uint32_t)__builtin_ctz(m);
+              m &= m - 1u;
+              HRec r;
+              float l;
+              op.load16(ofs, (size_t)j, r, l);
+              op.fpair16(U, t, ay, az, r, l);
+            }
+          }
+        }
+        if (doB && rowOk) {
+          const float sy = t.py, sz = t.pz;
+          t.py = cell_centre(U, 1, by) + oy;
+          t.pz = cell_centre(U, 2, bz) + oz;
+          const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+          for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+          t.py = sy; t.pz = sz;
+        }
+      }
+    }
+    if (valid) op.finish16(U, i, t);
+  }
+  if (dense && fails)
+    direct16_walk<Op>(U, op, i, cellStart, bcellStart, bpos, ofs, cellOfSlot);
+}
 
 // --- neighbour census: |N(i)| and index checksums -------------------------
 struct CensusOp : OpBase {
@@ -3173,10 +3400,52 @@ hipError_t launch_delta(const Uniform& U, const Arrays& A, const float4* pred, f
   return run_solver(U, A, drifted(DeltaFastT<false>{{}, pred, A.lambda, out}, d), t, nb, s, B);
 }
 
+template <class Op, int NT>
+hipError_t run_nb16(const Uniform& U, const Arrays& A, const Op& op, const HRec* ofs,
+                    const NbLaunch& nb, hipStream_t s) {
+  if (nb.mode == Nb::BUILD) {
+    hipLaunchKernelGGL((k_direct16_build<Op, NT>), dim3((U.n + NT - 1) / NT), dim3(NT), 0, s, U,
+                       op, nb.args, A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
+    return hipGetLastError();
+  }
+  hipStream_t fs = nb.side ? nb.side : s;
+  if (nb.side) {
+    hipError_t e = hipEventRecord(nb.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(nb.side, nb.fork, 0);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_fallback<Op, NT, true, true>), dim3(NB_QSEG * NB_FALLBACK_PARTS),
+                     dim3(NT), 0, fs, U, op, nb.args, A.cellStart, A.bcellStart, A.bpos, ofs,
+                     A.cellOfSlot);
+  hipLaunchKernelGGL((k_listed16<Op>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op, nb.args,
+                     A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
+  hipLaunchKernelGGL((k_fallback<Op, NT, false, true>), dim3(NB_QSEG * NB_FALLBACK_PARTS),
+                     dim3(NT), 0, s, U, op, nb.args, A.cellStart, A.bcellStart, A.bpos, ofs,
+                     A.cellOfSlot);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && nb.side) {
+    e = hipEventRecord(nb.join, nb.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, nb.join, 0);
+  }
+  return e;
+}
+
 template <class Op>
-hipError_t run_direct16(const Uniform& U, const Arrays& A, const Op& op, const HRec* ofs,
-                        int threads, hipStream_t s) {
+hipError_t run_direct16(const Uniform& U, const Arrays& A, Op op, const HRec* ofs,
+                        int threads, hipStream_t s, const NbLaunch& nb) {
   if (U.n == 0) return hipSuccess;
+  if (nb.mode != Nb::OFF) {
+    if (nb.mode == Nb::USE) {
+      op.nbRef = nb.args.ref;
+      op.nbTravel = nb.args.ctl;
+      op.nbMovedQ = nb.args.moved[nb.args.mslot ^ 1];
+      op.nbMovedCnt = nb.args.ctl + nb_moved_counts(nb.args.mslot ^ 1);
+      op.nbSegCap = nb.args.segCap;
+    }
+    if (threads == 256) return run_nb16<Op, 256>(U, A, op, ofs, nb, s);
+    if (threads == 128) return run_nb16<Op, 128>(U, A, op, ofs, nb, s);
+    return run_nb16<Op, 64>(U, A, op, ofs, nb, s);
+  }
   if (threads == 256)
     hipLaunchKernelGGL((k_direct16<Op, 256>), dim3((U.n + 255) / 256), dim3(256), 0, s, U, op,
                        A.cellStart, A.bcellStart, A.bpos, ofs, A.cellOfSlot);
@@ -3190,18 +3459,19 @@ hipError_t run_direct16(const Uniform& U, const Arrays& A, const Op& op, const H
 }
 
 hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
-                           const HRec* ofs, int threads, const Drift& d, hipStream_t s) {
+                           const HRec* ofs, int threads, const Drift& d, hipStream_t s,
+                           const NbLaunch& nb) {
   Lambda16 op;
   op.pred = pred;
   op.density = A.density;
   op.lambda = A.lambda;
   op.predW = (float*)pred;
-  return run_direct16(U, A, drifted(op, d), ofs, threads, s);
+  return run_direct16(U, A, drifted(op, d), ofs, threads, s, nb);
 }
 
 hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
                           const HRec* ofs, float4* out, HRec* ofsOut, int threads,
-                          const Drift& d, hipStream_t s) {
+                          const Drift& d, hipStream_t s, const NbLaunch& nb) {
   // the neighbour count only when sorAverage divides by it (as the FP32 form)
   auto go = [&](auto op) {
     op.pred = pred;
@@ -3209,7 +3479,7 @@ hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
     op.out = out;
     op.ofsOut = ofsOut;
     op.cellOfSlot = A.cellOfSlot;
-    return run_direct16(U, A, drifted(op, d), ofs, threads, s);
+    return run_direct16(U, A, drifted(op, d), ofs, threads, s, nb);
   };
   if (U.sorAverage == 1u) return go(Delta16T<true>{});
   return go(Delta16T<false>{});

@@ -271,11 +271,6 @@ hipError_t launch_predict_count(const Uniform& U, const Arrays& A, int par, hipS
 hipError_t launch_scan(const Uniform& U, const Arrays& A, hipStream_t s);
 hipError_t launch_scatter(const Uniform& U, const Arrays& A, int par, bool moveVel,
                           bool fp16, hipStream_t s);
-hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
-                           const HRec* ofs, int threads, const Drift& d, hipStream_t s);
-hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
-                          const HRec* ofs, float4* out, HRec* ofsOut, int threads,
-                          const Drift& d, hipStream_t s);
 // How a solver launch uses the neighbour masks: not at all, building them
 // (lambda after a grid build), or taking them (with the queued targets'
 // full walks in a second launch; delta also records travel).
@@ -288,6 +283,12 @@ struct NbLaunch {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
 };
+hipError_t launch_lambda16(const Uniform& U, const Arrays& A, const float4* pred,
+                           const HRec* ofs, int threads, const Drift& d, hipStream_t s,
+                           const NbLaunch& nb = NbLaunch{});
+hipError_t launch_delta16(const Uniform& U, const Arrays& A, const float4* pred,
+                          const HRec* ofs, float4* out, HRec* ofsOut, int threads,
+                          const Drift& d, hipStream_t s, const NbLaunch& nb = NbLaunch{});
 hipError_t launch_lambda(const Uniform& U, const Arrays& A, const float4* pred,
                          bool exact, const TileCfg& t, const Drift& d, hipStream_t s,
                          const BatchTab* B = nullptr, const NbLaunch& nb = NbLaunch{});

@@ -381,8 +381,7 @@ uint32_t* drift_slot(const p4a_sim* s) {
 }
 
 bool nb_on(const p4a_sim* s) {
-  return !fp16_form(s) && s->params.neighbour_skin > 0.0 &&
-         s->n >= s->params.neighbour_min_particles;
+  return s->params.neighbour_skin > 0.0 && s->n >= s->params.neighbour_min_particles;
 }
 
 // Mask arguments of the next solver launch: BUILD for the first lambda
@@ -407,8 +406,13 @@ NbLaunch nb_launch(p4a_sim* s, bool lambda) {
   a.segCap = nb_seg_cap((uint32_t)s->n);
   a.reach2 = (float)((s->h + S) * (s->h + S));
   // a margin of 1e-4 h covers the float rounding of the distances at the
-  // build and at the use and of the travels
-  a.lim = (float)(S - 1e-4 * s->h);
+  // build and at the use and of the travels.  The FP16-offset form marks
+  // and walks quantized positions: its y and z offsets (at most about h
+  // from the binned cell's centre for a particle that stays in it) are
+  // rounded to half, within sqrt(2) h 2^-11 of the FP32 position, and two
+  // particles, each at build and at use, take 4 such margins more.
+  const double q = fp16_form(s) ? 4.0 * std::sqrt(2.0) * s->h * std::ldexp(1.0, -11) : 0.0;
+  a.lim = (float)(S - 1e-4 * s->h - q);
   if (!s->nbBuilt) {
     L.mode = Nb::BUILD;
     a.slot = 0;
@@ -445,38 +449,39 @@ int nb_clean_moved(p4a_sim* s, const NbLaunch& L) {
 
 int launch_lambda_any(p4a_sim* s, const Uniform& U) {
   const Drift d;
+  if (fp16_form(s) && !s->ofsValid) return stale_records(s);
+  const NbLaunch nb = nb_launch(s, true);
   if (fp16_form(s)) {
-    if (!s->ofsValid) return stale_records(s);
     LAUNCH(s, "lambda", launch_lambda16(U, s->A, s->predCur, ofs_of(s, s->predCur),
-                                        direct_threads(s->tile1), d, s->stream));
+                                        direct_threads(s->tile1), d, s->stream, nb));
   } else {
-    const NbLaunch nb = nb_launch(s, true);
     LAUNCH(s, "lambda", launch_lambda(U, s->A, s->predCur, exact_math(s), s->tile1, d,
                                       s->stream, nullptr, nb));
-    // a lambda list launch empties the moved queue its iteration's delta fills
-    if (nb.mode == Nb::USE) s->nbMovedClean[nb.args.mslot ^ 1] = true;
-    nb_advance(s, nb);
   }
+  // a lambda list launch empties the moved queue its iteration's delta fills
+  if (nb.mode == Nb::USE) s->nbMovedClean[nb.args.mslot ^ 1] = true;
+  nb_advance(s, nb);
   return P4A_OK;
 }
 
 int launch_delta_any(p4a_sim* s, const Uniform& U, float4* out) {
   Drift d;
   d.out = drift_slot(s);
+  if (fp16_form(s) && !s->ofsValid) return stale_records(s);
+  const NbLaunch nb = nb_launch(s, false);
+  int rc = nb_clean_moved(s, nb);
+  if (rc) return rc;
   if (fp16_form(s)) {
-    if (!s->ofsValid) return stale_records(s);
     LAUNCH(s, "delta", launch_delta16(U, s->A, s->predCur, ofs_of(s, s->predCur), out,
-                                      ofs_of(s, out), direct_threads(s->tile1), d, s->stream));
+                                      ofs_of(s, out), direct_threads(s->tile1), d, s->stream,
+                                      nb));
   } else {
-    const NbLaunch nb = nb_launch(s, false);
-    int rc = nb_clean_moved(s, nb);
-    if (rc) return rc;
     LAUNCH(s, "delta", launch_delta(U, s->A, s->predCur, out, exact_math(s), s->tile1, d,
                                     s->stream, nullptr, nb));
-    nb_advance(s, nb);
-    if (nb.mode == Nb::USE) s->nbIter++;
     s->ofsValid = false;
   }
+  nb_advance(s, nb);
+  if (nb.mode == Nb::USE) s->nbIter++;
   s->driftCount++;
   return P4A_OK;
 }

@@ -22,6 +22,7 @@ most targets are queued.
   binned into (their full walks run beside the list launches, which skip
   them); deltas run back to back through the stage API refill the queue
   from empty and still give the full walk's bits;
+* the FP16-offset form with masks equals its own full walk bit for bit;
 * the masks are used from neighbour_min_particles on (by default only for
   scenes of 2^19 particles or more), and the threshold gives the same bits;
 * skins outside [0, 1], non-finite skins and negative thresholds are
@@ -40,8 +41,9 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("POS", "VEL", "DENSITY", "LAMBDA", "PID")
 
 
-def run(maker, skin, math, steps, **kw):
+def run(maker, skin, math, steps, precision=0, **kw):
     p, sc, _ = maker(math=math, **kw)
+    p.precision = precision
     p.neighbour_skin = skin
     p.neighbour_min_particles = 0  # these scenes are below the default size
     sim = Sim(0)
@@ -77,6 +79,19 @@ def test_masks_equal_the_full_walk(math, maker, kw):
         same(full, got, f"skin {skin}")
         if maker is workloads.block_4m_bodies:
             assert queued > 0  # the fallback launch walked some targets
+
+
+@pytest.mark.parametrize("maker,kw", [(workloads.dam_break_32, {}),
+                                      (workloads.block_4m_bodies, {"scale": 0.25})])
+def test_fp16_offset_masks_equal_its_full_walk(maker, kw):
+    """The FP16-offset form marks and walks quantized positions (its travel
+    limit keeps the half rounding of both particles at build and at use)."""
+    full, _ = run(maker, 0.0, 0, 12, precision=1, **kw)
+    for skin in (0.1, 0.3, 1e-4):
+        got, st = run(maker, skin, 0, 12, precision=1, **kw)
+        print(f"fp16 {maker.__name__} skin {skin}: moved queue "
+              f"{int(st[MOVED:].sum())}, list queue {int(st[1:MOVED].sum())}")
+        same(full, got, f"fp16 skin {skin}")
 
 
 def test_travel_bound_and_queues_on_a_flowing_state():

@@ -1706,9 +1706,9 @@ struct DeltaFastT : FastBase {
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
     const float tt = U.h2 - r2;
-    const float ratio = (U.poly6 * U.sCorrWq) * (tt * tt * tt);
-    const float r2r = ratio * ratio;
-    const float sc = -U.sCorrK * r2r * r2r;
+    // s_corr = -k (W(r) / W(dq))^4 as one constant times tt^12
+    const float t2 = tt * tt, t4 = t2 * t2;
+    const float sc = U.sCorrFast * (t4 * t4) * t4;
     float f = (t.li + v[0].w + sc) * (hr * hr * ir);
     f = ok ? f : 0.0f;
     t.dx = __fmaf_rn(f, rx, t.dx);
@@ -1797,7 +1797,9 @@ struct XsphFast : FastBase {
   }
 };
 
-// Records: (pred.xyz, -), (normal_j.xyz, rho_j).
+// Records: (pred.xyz, -), (-gamma normal_j.xyz, rho_j): the normals are
+// scaled by -gamma where the record is read, so each pair takes one
+// difference of scaled normals.
 struct TensionFast : FastBase {
   static constexpr bool BOUNDARY = false;
   static constexpr int NREC = 2;
@@ -1814,14 +1816,17 @@ struct TensionFast : FastBase {
   __device__ static float4 sentinel(int k) {
     return k == 0 ? make_float4(kFar, kFar, kFar, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
   }
-  __device__ void load(const Uniform&, size_t j, float4* v) const {
+  __device__ void load(const Uniform& U, size_t j, float4* v) const {
+    const float4 nn = normal[j];
+    const float mg = -U.tension;
     v[0] = pred[j];
-    v[1] = normal[j];
+    v[1] = make_float4(mg * nn.x, mg * nn.y, mg * nn.z, nn.w);
   }
-  __device__ void target(const Uniform&, uint32_t i, Target& t) const {
+  __device__ void target(const Uniform& U, uint32_t i, Target& t) const {
     const float4 p = pred[i], nn = normal[i];
+    const float mg = -U.tension;
     t.px = p.x; t.py = p.y; t.pz = p.z;
-    t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
+    t.nx = mg * nn.x; t.ny = mg * nn.y; t.nz = mg * nn.z;
     t.rho = nn.w;
     t.ax = t.ay = t.az = 0.0f;
   }
@@ -1834,12 +1839,12 @@ struct TensionFast : FastBase {
     const float r = r2 * ir;
     const float u = (U.h - r) * r;
     const float k = u * u * u;
-    const float cw = (2.0f * r > U.h) ? k : __fmaf_rn(2.0f, k, -U.cohesTerm);
+    // 2 r > h exactly when r > h / 2 (both sides scale by a power of two)
+    const float cw = (r > 0.5f * U.h) ? k : __fmaf_rn(2.0f, k, -U.cohesTerm);
     const float fc = (-U.tension * U.mass * U.cohes) * cw * ir;
-    const float mg = -U.tension;
-    const float fx = __fmaf_rn(fc, rx, mg * (t.nx - v[1].x));
-    const float fy = __fmaf_rn(fc, ry, mg * (t.ny - v[1].y));
-    const float fz = __fmaf_rn(fc, rz, mg * (t.nz - v[1].z));
+    const float fx = __fmaf_rn(fc, rx, t.nx - v[1].x);
+    const float fy = __fmaf_rn(fc, ry, t.ny - v[1].y);
+    const float fz = __fmaf_rn(fc, rz, t.nz - v[1].z);
     float K = __builtin_amdgcn_rcpf(t.rho + v[1].w);
     K = ok ? K : 0.0f;
     t.ax = __fmaf_rn(K, fx, t.ax);
@@ -2074,9 +2079,8 @@ struct Delta16T : DeltaFastT<SOR> {
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
     const float tt = U.h2 - r2;
-    const float ratio = (U.poly6 * U.sCorrWq) * (tt * tt * tt);
-    const float r2r = ratio * ratio;
-    const float sc = -U.sCorrK * r2r * r2r;
+    const float t2 = tt * tt, t4 = t2 * t2;
+    const float sc = U.sCorrFast * (t4 * t4) * t4;
     float f = (t.li + lj + sc) * (hr * hr * ir);
     f = ok ? f : 0.0f;
     t.dx = __fmaf_rn(f, rx, t.dx);

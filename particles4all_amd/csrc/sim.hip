@@ -72,6 +72,8 @@ Uniform make_uniform_from(const SceneConsts& s, double dt) {
   const double tq = h * h - rq * rq;
   const double wq = poly6 * tq * tq * tq;
   U.sCorrWq = (float)(wq > 0 ? 1.0 / wq : 0.0);
+  const double cq = wq > 0 ? poly6 / wq : 0.0;
+  U.sCorrFast = (float)(-(p.scorr_k / (s.denomRest * std::max(1, p.iterations))) * cq * cq * cq * cq);
   U.xsphC = (float)p.xsph_c;
   U.n = (uint32_t)s.n;
   U.nCells = (uint32_t)s.nCells;

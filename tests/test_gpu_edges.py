@@ -186,18 +186,25 @@ def test_near_coincident_pairs(precision, variant, monkeypatch):
     for it in range(p.iterations):
         pred = rd(sim, "PRED")
         at = O.fp16_quantize(P, sim.read("PRED"), sim.read("CELL_OF_SLOT")) if precision else pred
+        # the FP16-offset form walks the stencil of a target's FP32 position;
+        # the oracle takes the stencil of the positions it is given, so a
+        # target whose quantized position lies across a cell face (within the
+        # half rounding of its offset) is left out of the comparison
+        same = O.cell_ids(O.F32, P, pred) == O.cell_ids(O.F64, P, at)
+        assert np.count_nonzero(~same) <= 2
         rho64, lam64 = O.lambda_(O.F64, P, at, cs, *bnd)
         sim.run_stage("LAMBDA", dt)
-        rho_err = np.abs(rd(sim, "DENSITY") - rho64).max() / rho64.max()
-        lam_err = np.abs(rd(sim, "LAMBDA") - lam64).max() / max(np.abs(lam64).max(), 1e-30)
-        print(f"iteration {it}: density {rho_err:.2e}, lambda {lam_err:.2e}")
+        rho_err = np.abs(rd(sim, "DENSITY") - rho64)[same].max() / rho64.max()
+        lam_err = np.abs(rd(sim, "LAMBDA") - lam64)[same].max() / max(np.abs(lam64).max(), 1e-30)
+        print(f"iteration {it}: density {rho_err:.2e}, lambda {lam_err:.2e} "
+              f"({np.count_nonzero(~same)} across a face)")
         assert rho_err <= (1e-3 if precision else 1e-5) and lam_err <= (1e-2 if precision else 1e-4)
         if precision:
             out64 = O.delta_at(O.F64, P, at, pred, rd(sim, "LAMBDA"), cs, *bnd)
         else:
             out64 = O.delta(O.F64, P, pred, rd(sim, "LAMBDA"), cs, *bnd)
         sim.run_stage("DELTA", dt)
-        err = np.abs(rd(sim, "PRED")[:, :3] - out64[:, :3]).max()
+        err = np.abs(rd(sim, "PRED")[:, :3] - out64[:, :3])[same].max()
         assert err <= (1e-3 if precision else 1e-4) * h, err
     pred, pos, den = rd(sim, "PRED"), rd(sim, "POS"), rd(sim, "DENSITY")
     _, corr, normal = O.xsph(O.F64, P, pred, pos, den, cs, True)

@@ -991,6 +991,36 @@ __device__ __forceinline__ void hit_loop1(const Uniform& U, const Op& op,
   }
 }
 
+// Whether the wave's targets with `on` set all lie in one grid row of cells
+// (same y and z cell): then, in slot order, their cells' x and so the
+// bounds of each of their stencil rows never decrease from lane to lane.
+__device__ __forceinline__ bool one_grid_row(bool on, Cell c) {
+  const uint64_t m = __ballot(on);
+  if (!m) return true;
+  const int yz = c.y * 65536 + c.z;
+  const int first = __builtin_amdgcn_readlane(yz, __builtin_ctzll(m));
+  return !__ballot(on && yz != first);
+}
+
+// The union [ub, ue) of the row ranges [b, e) of the lanes with `has` set
+// (ub >= ue when none): with bounds that never decrease from lane to lane
+// (one_grid_row), the first such lane's b and the last one's e; otherwise
+// a wave minimum and maximum.
+__device__ __forceinline__ void row_union(bool has, uint32_t b, uint32_t e, bool mono,
+                                          uint32_t& ub, uint32_t& ue) {
+  const uint64_t m = __ballot(has);
+  if (!m) {
+    ub = 0xffffffffu;
+    ue = 0u;
+  } else if (mono) {
+    ub = (uint32_t)__builtin_amdgcn_readlane((int)b, __builtin_ctzll(m));
+    ue = (uint32_t)__builtin_amdgcn_readlane((int)e, 63 - __builtin_clzll(m));
+  } else {
+    ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
+    ue = __ockl_wfred_max_u32(has ? e : 0u);
+  }
+}
+
 // Records of one stencil row staged per wave: the union of the wave's row
 // ranges, when it holds at most NB_STAGE candidates (a wave's targets lie in
 // a few neighbouring cells of one grid row, so its rows overlap).
@@ -1071,6 +1101,7 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
                           c.z <= 1 || c.z >= gz - 2;
     const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && valid;
+    const bool mono = one_grid_row(valid, c);
     for (int dz = -1; dz <= 1; dz++) {
       for (int dy = -1; dy <= 1; dy++) {
         const int z = c.z + dz, y = c.y + dy;
@@ -1086,8 +1117,8 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
         // union of the wave's ranges that hold a hit (every lane is active
         // here, the others contribute the neutral values)
         const bool has = m64 != 0ull;
-        const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
-        const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
+        uint32_t ub, ue;
+        row_union(has, b, e, mono, ub, ue);
         if (ub < ue && ue - ub <= STAGE) {
           const uint32_t span = ue - ub;
           for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
@@ -2181,6 +2212,7 @@ k_listed16(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
                           c.z <= 1 || c.z >= gz - 2;
     const bool doB = U.nBoundary > 0 && nearWall && valid;
+    const bool mono = one_grid_row(valid, c);
     for (int dz = -1; dz <= 1; dz++) {
       for (int dy = -1; dy <= 1; dy++) {
         const int z = c.z + dz, y = c.y + dy;
@@ -2196,8 +2228,8 @@ k_listed16(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
           m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
         }
         const bool has = m64 != 0ull;
-        const uint32_t ub = __ockl_wfred_min_u32(has ? b : 0xffffffffu);
-        const uint32_t ue = __ockl_wfred_max_u32(has ? e : 0u);
+        uint32_t ub, ue;
+        row_union(has, b, e, mono, ub, ue);
         if (ub < ue && ue - ub <= NB_STAGE) {
           const uint32_t span = ue - ub;
           for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {

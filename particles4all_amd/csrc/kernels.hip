@@ -924,19 +924,6 @@ k_direct(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
   direct_walk<Op, NT, CR, PH>(U, op, i, cellStart, bcellStart, bpos);
 }
 
-// The first lambda after a grid build: the full walk, building the masks.
-// It also restarts the travel bound and the queues of the list launches.
-template <class Op, int NT>
-__global__ void __launch_bounds__(NT)
-k_direct_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
-               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
-  if (blockIdx.x == 0)
-    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += NT) nb.ctl[k] = 0u;
-  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-  if (i >= U.n) return;
-  direct_walk<Op, NT, 1, 0, true>(U, op, i, cellStart, bcellStart, bpos, &nb);
-}
-
 // The marked hits of one row half (32 candidates from `base`) in stencil
 // order, four per trip with their record reads in flight together; a
 // trip's missing hits reread its first hit and take the far sentinel (fast
@@ -1152,6 +1139,129 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     if (valid) op.finish(U, i, t);
   }
   if (dense && fails) direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
+}
+
+// The mask build as a staged walk (one wave per workgroup), for the first
+// lambda after a grid build.  Per stencil row the wave stages the union of
+// its targets' candidate ranges in LDS; each target marks its candidates
+// within h + S from there (the row masks the list launches take) and walks
+// the marked ones as a list launch does, in stencil order with the row's
+// wall samples after it: every unmarked candidate lies beyond h, so the sums
+// are the full walk's bit for bit.  A target with a row of more than 64
+// candidates, or none of whose rows holds its own record, takes the full
+// walk afterwards, which writes its masks and its reference as unlisted.
+// It also restarts the travel bound and the queues of the list launches.
+template <class Op>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
+  __shared__ float4 S[Op::NREC][STAGE];
+  if (blockIdx.x == 0)
+    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += 64u) nb.ctl[k] = 0u;
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  const bool on = i < U.n;
+  typename Op::Target t;
+  Cell c{0, 0, 0};
+  if (on) {
+    op.target(U, i, t);
+    c = op.cell(U, t);
+  }
+  const int xlo = max(c.x - 1, 0);
+  const int xhi = min(c.x + 1, gx - 1);
+  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
+                        c.z <= 1 || c.z >= gz - 2;
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && on;
+  const bool mono = one_grid_row(on, c);
+  bool over = false;  // a row of more than 64 candidates
+  bool seen = false;  // the target's own record lay in one of its rows
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const int z = c.z + dz, y = c.y + dy;
+      const bool rowOk = on && z >= 0 && z < gz && y >= 0 && y < gy;
+      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+      uint32_t b = 0, e = 0;
+      if (rowOk) {
+        b = cellStart[rb + xlo];
+        e = cellStart[rb + xhi + 1];
+      }
+      over = over || e - b > 64u;
+      seen = seen || (i >= b && i < e);
+      const uint32_t len = min(e - b, 64u);
+      uint32_t ub, ue;
+      row_union(rowOk, b, e, mono, ub, ue);
+      // the marks of candidates b .. b + len - 1, bit k for candidate b + k;
+      // a trip tests four candidates of every lane, those past a lane's row
+      // are masked off after the loop
+      const uint32_t most = __ockl_wfred_max_u32(len);
+      uint32_t mlo = 0u, mhi = 0u;
+      const auto marks = [&](const auto& pos) {
+        for (uint32_t k0 = 0; k0 < most; k0 += 4u) {
+          float4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) q[u] = pos(k0 + (uint32_t)u);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const bool in = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z) < nb.reach2;
+            const uint32_t k = k0 + (uint32_t)u;
+            if (k < 32u) mlo = in ? (mlo | (1u << k)) : mlo;
+            else mhi = in ? (mhi | (1u << (k - 32u))) : mhi;
+          }
+        }
+        mlo &= len >= 32u ? ~0u : (1u << len) - 1u;
+        mhi &= len >= 64u ? ~0u : len > 32u ? (1u << (len - 32u)) - 1u : 0u;
+      };
+      // stage only when a lane's reads past its row stay inside the stage
+      if (ub < ue && ue - ub + 64u <= STAGE) {
+        const uint32_t span = ue - ub;
+        for (uint32_t q = threadIdx.x; q < span; q += 64u) {
+          float4 v[Op::NREC];
+          op.load(U, (size_t)ub + q, v);
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+        }
+        __syncthreads();
+        // the lane's first candidate in the stage (a lane without the row
+        // reads from the stage's start)
+        const uint32_t at = rowOk ? b - ub : 0u;
+        marks([&](uint32_t k) { return S[0][at + k]; });
+        const auto rec = [&](uint32_t j, float4* v) {
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
+        };
+        hit_loop1(U, op, t, mlo, b, rec);
+        if (__ballot(mhi != 0u)) hit_loop1(U, op, t, mhi, b + 32u, rec);
+        __syncthreads();
+      } else if (ub < ue) {
+        // a lane past its row reads candidate 0 instead
+        marks([&](uint32_t k) {
+          float4 v[Op::NREC];
+          op.load(U, k < len ? (size_t)b + k : (size_t)0, v);
+          return v[0];
+        });
+        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
+        hit_loop(U, op, t, mlo, b, rec);
+        hit_loop(U, op, t, mhi, b + 32u, rec);
+      }
+      if (rowOk)
+        nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i] =
+            (uint64_t)mlo | ((uint64_t)mhi << 32);
+      if (doB && rowOk) {
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+      }
+    }
+  }
+  if (on) {
+    if (!over && seen) {
+      op.finish(U, i, t);
+      nb.ref[i] = make_float4(t.px, t.py, t.pz, __uint_as_float(cell_index(U, c)));
+    } else {
+      direct_walk<Op, 64, 1, 0, true>(U, op, i, cellStart, bcellStart, bpos, &nb);
+    }
+  }
 }
 
 // The 27-cell walk over FP16-offset records (defined with those records'
@@ -3366,10 +3476,9 @@ int nb_threads(const TileCfg& t) { return t.variant == 0 ? 256 : t.variant == 1 
 template <class Op, int NT>
 hipError_t run_nb(const Uniform& U, const Arrays& A, const Op& op, const NbLaunch& nb,
                   hipStream_t s) {
-  const unsigned g = (U.n + NT - 1) / NT;
   if (nb.mode == Nb::BUILD) {
-    hipLaunchKernelGGL((k_direct_build<Op, NT>), dim3(g), dim3(NT), 0, s, U, op, nb.args,
-                       A.cellStart, A.bcellStart, A.bpos);
+    hipLaunchKernelGGL((k_staged_build<Op>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op,
+                       nb.args, A.cellStart, A.bcellStart, A.bpos);
     return hipGetLastError();
   }
   // the moved targets' full walks run on the side stream beside the list

@@ -2626,6 +2626,8 @@ k_finalize_batch(BatchTab B, const float4* __restrict__ pred, float4* __restrict
 // static id list bodyPids[bodyFirst[b] .. bodyFirst[b+1]), located through
 // slotOf.
 
+constexpr int BODY_UNROLL = 8;
+
 __device__ __forceinline__ double wave_sum_d(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -2659,11 +2661,25 @@ k_body_pre(const float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
   const uint32_t f = bodyFirst[b], l = bodyFirst[b + 1];
   const double cx = centreD[b * 4], cy = centreD[b * 4 + 1], cz = centreD[b * 4 + 2];
   double v[3] = {0, 0, 0};
-  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
-    const float4 p = pred[slotOf[bodyPids[k]]];
-    v[0] += (double)p.x - cx;
-    v[1] += (double)p.y - cy;
-    v[2] += (double)p.z - cz;
+  // BODY_UNROLL particles per thread and trip with their loads in flight
+  // together (each sums its particles in the same order as one at a time)
+  for (uint32_t k0 = f + threadIdx.x; k0 < l; k0 += WG * BODY_UNROLL) {
+    uint32_t q[BODY_UNROLL];
+    float4 p[BODY_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) q[u] = bodyPids[min(k0 + (uint32_t)u * WG, l - 1u)];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) q[u] = slotOf[q[u]];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) p[u] = pred[q[u]];
+    // a slot past the body adds +0 (a sum that starts at +0 keeps its bits)
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      const bool in = k0 + (uint32_t)u * WG < l;
+      v[0] += in ? (double)p[u].x - cx : 0.0;
+      v[1] += in ? (double)p[u].y - cy : 0.0;
+      v[2] += in ? (double)p[u].z - cz : 0.0;
+    }
   }
   block_sum_d<3>(v, sh);
   if (threadIdx.x == 0 && l > f) {
@@ -2698,17 +2714,32 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
   double v[15];
 #pragma unroll
   for (int k = 0; k < 15; k++) v[k] = 0;
-  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
-    const uint32_t q = bodyPids[k];
-    const float4 p = pred[slotOf[q]];
-    const float4 r = restOfPid[q];
-    const double px = (double)p.x - cx, py = (double)p.y - cy, pz = (double)p.z - cz;
-    const double qx = r.x, qy = r.y, qz = r.z;
-    v[0] += px; v[1] += py; v[2] += pz;
-    v[3] += qx; v[4] += qy; v[5] += qz;
-    v[6] += px * qx; v[7] += px * qy; v[8] += px * qz;
-    v[9] += py * qx; v[10] += py * qy; v[11] += py * qz;
-    v[12] += pz * qx; v[13] += pz * qy; v[14] += pz * qz;
+  for (uint32_t k0 = f + threadIdx.x; k0 < l; k0 += WG * BODY_UNROLL) {
+    uint32_t q[BODY_UNROLL], sl[BODY_UNROLL];
+    float4 p[BODY_UNROLL], r[BODY_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) q[u] = bodyPids[min(k0 + (uint32_t)u * WG, l - 1u)];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      sl[u] = slotOf[q[u]];
+      r[u] = restOfPid[q[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) p[u] = pred[sl[u]];
+    // a slot past the body adds +-0 to every sum (which keeps its bits)
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      const bool in = k0 + (uint32_t)u * WG < l;
+      const double px = in ? (double)p[u].x - cx : 0.0, py = in ? (double)p[u].y - cy : 0.0,
+                   pz = in ? (double)p[u].z - cz : 0.0;
+      const double qx = in ? (double)r[u].x : 0.0, qy = in ? (double)r[u].y : 0.0,
+                   qz = in ? (double)r[u].z : 0.0;
+      v[0] += px; v[1] += py; v[2] += pz;
+      v[3] += qx; v[4] += qy; v[5] += qz;
+      v[6] += px * qx; v[7] += px * qy; v[8] += px * qz;
+      v[9] += py * qx; v[10] += py * qy; v[11] += py * qz;
+      v[12] += pz * qx; v[13] += pz * qy; v[14] += pz * qz;
+    }
   }
   block_sum_d<15>(v, sh);
   if (threadIdx.x == 0) {
@@ -2770,30 +2801,47 @@ k_body_solve(float4* __restrict__ pred, const uint32_t* __restrict__ slotOf,
   // bodyProject (wgsl.js:763-783)
   const float ox = sc[0], oy = sc[1], oz = sc[2];
   float dmax = 0.0f, tmax = 0.0f;
-  for (uint32_t k = f + threadIdx.x; k < l; k += WG) {
-    const uint32_t q = bodyPids[k];
-    const float4 r = restOfPid[q];
-    const float px = sc[3] * r.x + sc[4] * r.y + sc[5] * r.z;
-    const float py = sc[6] * r.x + sc[7] * r.y + sc[8] * r.z;
-    const float pz = sc[9] * r.x + sc[10] * r.y + sc[11] * r.z;
-    const uint32_t sl = slotOf[q];
-    const float4 np = make_float4(ox + px, oy + py, oz + pz, 1.0f);
-    const float4 op = pred[sl];
-    dmax = fmaxf(dmax, fmaxf(fabsf(np.x - op.x), fmaxf(fabsf(np.y - op.y), fabsf(np.z - op.z))));
-    if (nbTravel) {
-      // a NaN distance must survive the maximum (fmaxf would drop it)
-      const float4 r = nbRef[sl];
-      const float d = sqrtf(r2_of(np.x - r.x, np.y - r.y, np.z - r.z));
-      tmax = (d > tmax || d != d) ? d : tmax;
-      // queued if the projection takes it out of its build cell (the delta
-      // before queued it if it was out already)
-      const uint32_t built = __float_as_uint(r.w) & ~NB_UNLISTED;
-      const bool was = cell_index(U, cell_of(U, op.x, op.y, op.z)) != built;
-      const bool now = cell_index(U, cell_of(U, np.x, np.y, np.z)) != built;
-      enqueue_moved(movedQ, movedCnt, segCap, now && !was, sl);
+  for (uint32_t k0 = f + threadIdx.x; k0 < l; k0 += WG * BODY_UNROLL) {
+    uint32_t q[BODY_UNROLL], sl[BODY_UNROLL];
+    float4 r[BODY_UNROLL], op[BODY_UNROLL], ref[BODY_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) q[u] = bodyPids[min(k0 + (uint32_t)u * WG, l - 1u)];
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      sl[u] = slotOf[q[u]];
+      r[u] = restOfPid[q[u]];
     }
-    pred[sl] = np;
-    if (ofs) ofs[sl] = encode_ofs(U, cellOfSlot[sl], np.x, np.y, np.z);
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      op[u] = pred[sl[u]];
+      if (nbTravel) ref[u] = nbRef[sl[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < BODY_UNROLL; u++) {
+      const bool in = k0 + (uint32_t)u * WG < l;
+      const float px = sc[3] * r[u].x + sc[4] * r[u].y + sc[5] * r[u].z;
+      const float py = sc[6] * r[u].x + sc[7] * r[u].y + sc[8] * r[u].z;
+      const float pz = sc[9] * r[u].x + sc[10] * r[u].y + sc[11] * r[u].z;
+      const float4 np = make_float4(ox + px, oy + py, oz + pz, 1.0f);
+      if (in)
+        dmax = fmaxf(dmax, fmaxf(fabsf(np.x - op[u].x),
+                                 fmaxf(fabsf(np.y - op[u].y), fabsf(np.z - op[u].z))));
+      if (nbTravel) {
+        // a NaN distance must survive the maximum (fmaxf would drop it)
+        const float d = sqrtf(r2_of(np.x - ref[u].x, np.y - ref[u].y, np.z - ref[u].z));
+        if (in) tmax = (d > tmax || d != d) ? d : tmax;
+        // queued if the projection takes it out of its build cell (the
+        // delta before queued it if it was out already)
+        const uint32_t built = __float_as_uint(ref[u].w) & ~NB_UNLISTED;
+        const bool was = cell_index(U, cell_of(U, op[u].x, op[u].y, op[u].z)) != built;
+        const bool now = cell_index(U, cell_of(U, np.x, np.y, np.z)) != built;
+        enqueue_moved(movedQ, movedCnt, segCap, in && now && !was, sl[u]);
+      }
+      if (in) {
+        pred[sl[u]] = np;
+        if (ofs) ofs[sl[u]] = encode_ofs(U, cellOfSlot[sl[u]], np.x, np.y, np.z);
+      }
+    }
   }
   if (driftOut) {
     // largest displacement of the body's particles into the drift slot

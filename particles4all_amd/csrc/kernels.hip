@@ -1012,6 +1012,8 @@ __device__ __forceinline__ void row_union(bool has, uint32_t b, uint32_t e, bool
 // ranges, when it holds at most NB_STAGE candidates (a wave's targets lie in
 // a few neighbouring cells of one grid row, so its rows overlap).
 constexpr uint32_t NB_STAGE = 256;
+// The records of a plane's three rows staged per wave (fast one-record ops)
+constexpr uint32_t NB_STAGE_PLANE = 320;
 
 // A list launch (one wave per workgroup).  Each target whose masks still
 // hold -- it is in the cell it was in at the build, and its travel since
@@ -1035,7 +1037,9 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : 1, 8)))
 k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
-  constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
+  // the fast one-record ops stage a plane's three rows at once
+  constexpr bool PLANES = Op::FAST && Op::NREC == 1;
+  constexpr uint32_t STAGE = PLANES ? NB_STAGE_PLANE : NB_STAGE / Op::NREC;
   __shared__ float4 S[Op::NREC][STAGE];
   if (blockIdx.x == 0) {
     for (uint32_t k = threadIdx.x; k < NB_QSEG; k += 64u)
@@ -1089,51 +1093,137 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
                           c.z <= 1 || c.z >= gz - 2;
     const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && valid;
     const bool mono = one_grid_row(valid, c);
-    for (int dz = -1; dz <= 1; dz++) {
-      for (int dy = -1; dy <= 1; dy++) {
-        const int z = c.z + dz, y = c.y + dy;
-        const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
-        const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
-        uint32_t b = 0, e = 0;
-        uint64_t m64 = 0;
-        if (rowOk) {
-          b = cellStart[rb + xlo];
-          e = cellStart[rb + xhi + 1];
-          m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
-        }
-        // union of the wave's ranges that hold a hit (every lane is active
-        // here, the others contribute the neutral values)
-        const bool has = m64 != 0ull;
-        uint32_t ub, ue;
-        row_union(has, b, e, mono, ub, ue);
-        if (ub < ue && ue - ub <= STAGE) {
-          const uint32_t span = ue - ub;
-          for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
-            float4 v[Op::NREC];
-            op.load(U, (size_t)ub + q, v);
+    // one stencil row on its own: its hits, then its wall samples
+    const auto row_walk = [&](int dz, int dy) {
+      const int z = c.z + dz, y = c.y + dy;
+      const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+      uint32_t b = 0, e = 0;
+      uint64_t m64 = 0;
+      if (rowOk) {
+        b = cellStart[rb + xlo];
+        e = cellStart[rb + xhi + 1];
+        m64 = nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i];
+      }
+      // union of the wave's ranges that hold a hit (every lane is active
+      // here, the others contribute the neutral values)
+      const bool has = m64 != 0ull;
+      uint32_t ub, ue;
+      row_union(has, b, e, mono, ub, ue);
+      if (ub < ue && ue - ub <= STAGE) {
+        const uint32_t span = ue - ub;
+        for (uint32_t q = (uint32_t)lane; q < span; q += 64u) {
+          float4 v[Op::NREC];
+          op.load(U, (size_t)ub + q, v);
 #pragma unroll
-            for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+        }
+        __syncthreads();
+        const auto rec = [&](uint32_t j, float4* v) {
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
+        };
+        hit_loop1(U, op, t, (uint32_t)m64, b, rec);
+        // rows of more than 32 candidates are rare: the upper half only
+        // when some lane has hits there
+        if (__ballot((m64 >> 32) != 0ull))
+          hit_loop1(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+        __syncthreads();
+      } else if (has) {
+        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
+        hit_loop(U, op, t, (uint32_t)m64, b, rec);
+        hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+      }
+      if (doB && rowOk) {
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+      }
+    };
+    bool perRow = true;
+    if constexpr (PLANES) perRow = __ballot(doB) != 0ull;
+    if (perRow) {
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) row_walk(dz, dy);
+    } else if constexpr (PLANES) {
+      // No wall samples in this wave: the three rows of a plane are staged
+      // together and each lane walks its hits of all three in one loop, so
+      // the plane runs as many trips as its longest lane has hits there
+      // rather than the sum of each row's longest.
+      for (int dz = -1; dz <= 1; dz++) {
+        const int z = c.z + dz;
+        uint32_t lo[3], at[3], first[3], span[3];
+        uint32_t seg = 0u;  // stage offset of the next row
+        bool upper = false;  // hits past a row's 32nd candidate
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const int y = c.y + r - 1;
+          const bool rowOk = valid && z >= 0 && z < gz && y >= 0 && y < gy;
+          uint32_t b = 0, e = 0;
+          uint64_t m64 = 0;
+          if (rowOk) {
+            const uint32_t rb = (uint32_t)((z * gy + y) * gx);
+            b = cellStart[rb + xlo];
+            e = cellStart[rb + xhi + 1];
+            m64 = nb.mask[(size_t)((dz + 1) * 3 + r) * nb.stride + i];
           }
-          __syncthreads();
-          const auto rec = [&](uint32_t j, float4* v) {
+          const bool has = m64 != 0ull;
+          uint32_t ub, ue;
+          row_union(has, b, e, mono, ub, ue);
+          if (ub >= ue) ub = ue = 0u;
+          first[r] = ub;
+          span[r] = ue - ub;
+          lo[r] = (uint32_t)m64;
+          at[r] = seg + (b - ub);  // stage position of candidate b
+          upper = upper || (m64 >> 32) != 0ull;
+          seg += ue - ub;
+        }
+        if (__ballot(upper) || seg > STAGE) {
+          for (int dy = -1; dy <= 1; dy++) row_walk(dz, dy);
+          continue;
+        }
+        uint32_t o = 0u;
 #pragma unroll
-            for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
-          };
-          hit_loop1(U, op, t, (uint32_t)m64, b, rec);
-          // rows of more than 32 candidates are rare: the upper half only
-          // when some lane has hits there
-          if (__ballot((m64 >> 32) != 0ull))
-            hit_loop1(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
-          __syncthreads();
-        } else if (has) {
-          const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
-          hit_loop(U, op, t, (uint32_t)m64, b, rec);
-          hit_loop(U, op, t, (uint32_t)(m64 >> 32), b + 32u, rec);
+        for (int r = 0; r < 3; r++) {
+          for (uint32_t q = (uint32_t)lane; q < span[r]; q += 64u) {
+            float4 v[Op::NREC];
+            op.load(U, (size_t)first[r] + q, v);
+#pragma unroll
+            for (int k = 0; k < Op::NREC; k++) S[k][o + q] = v[k];
+          }
+          o += span[r];
         }
-        if (doB && rowOk) {
-          const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
-          for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+        __syncthreads();
+        // the current row is m0 at a0, the next ones m1 at a1 and m2 at
+        // a2; rows without hits drop out first, so m0 is empty only when
+        // all three are
+        uint32_t m0 = lo[0], a0 = at[0], m1 = lo[1], a1 = at[1], m2 = lo[2], a2 = at[2];
+        if (!m1) {
+          m1 = m2;
+          a1 = a2;
+          m2 = 0u;
         }
+        if (!m0) {
+          m0 = m1;
+          a0 = a1;
+          m1 = m2;
+          a1 = a2;
+          m2 = 0u;
+        }
+        while (m0) {
+          const uint32_t q = a0 + (uint32_t)__builtin_ctz(m0);
+          m0 &= m0 - 1u;
+          float4 v[Op::NREC];
+#pragma unroll
+          for (int k = 0; k < Op::NREC; k++) v[k] = S[k][q];
+          op.fpair(U, t, v);
+          const bool next = m0 == 0u;
+          m0 = next ? m1 : m0;
+          a0 = next ? a1 : a0;
+          m1 = next ? m2 : m1;
+          a1 = next ? a2 : a1;
+          m2 = next ? 0u : m2;
+        }
+        __syncthreads();
       }
     }
     if (valid) op.finish(U, i, t);

@@ -979,14 +979,23 @@ __device__ __forceinline__ void hit_loop1(const Uniform& U, const Op& op,
 }
 
 // Whether the wave's targets with `on` set all lie in one grid row of cells
-// (same y and z cell): then, in slot order, their cells' x and so the
-// bounds of each of their stencil rows never decrease from lane to lane.
+// (same y and z cell) with cell x never decreasing from lane to lane: then
+// the bounds of each of their stencil rows never decrease either.  Slot
+// order follows the cells the targets were binned into, so this holds
+// unless some target has left its cell since the grid build (a delta run
+// before the mask build through the stage API).
 __device__ __forceinline__ bool one_grid_row(bool on, Cell c) {
   const uint64_t m = __ballot(on);
   if (!m) return true;
   const int yz = c.y * 65536 + c.z;
   const int first = __builtin_amdgcn_readlane(yz, __builtin_ctzll(m));
-  return !__ballot(on && yz != first);
+  if (__ballot(on && yz != first)) return false;
+  // x against the nearest lane before it with `on` set
+  const int lane = __lane_id();
+  const uint64_t below = m & ((1ull << lane) - 1ull);
+  const int prev = below ? 63 - __builtin_clzll(below) : lane;
+  const int px = __shfl(c.x, prev, 64);
+  return !__ballot(on && below != 0ull && c.x < px);
 }
 
 // The union [ub, ue) of the row ranges [b, e) of the lanes with `has` set

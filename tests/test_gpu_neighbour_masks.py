@@ -23,6 +23,8 @@ most targets are queued.
   them); deltas run back to back through the stage API refill the queue
   from empty and still give the full walk's bits;
 * the FP16-offset form with masks equals its own full walk bit for bit;
+* a build after a delta (targets outside the cells they were binned into)
+  gives the full walk's bits;
 * the masks are used from neighbour_min_particles on (by default only for
   scenes of 2^19 particles or more), and the threshold gives the same bits;
 * skins outside [0, 1], non-finite skins and negative thresholds are
@@ -197,6 +199,31 @@ def test_moved_queues_hold_the_targets_that_left_their_build_cells():
         runs[skin] = {k: sim.read(k).copy() for k in FIELDS}
         sim.close()
     same(runs[0.0], runs[0.1], "back-to-back deltas")
+
+
+def test_build_after_a_delta_matches_the_full_walk():
+    """A mask build that finds targets outside the cell they were binned
+    into (a delta ran between the grid build and the first lambda, through
+    the stage API): those targets take the full walk in the build and are
+    left to the fallback walks after it, and every stage still gives the
+    full walk's bits."""
+    p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
+    p.neighbour_min_particles = 0
+    runs = {}
+    for skin in (0.1, 0.0):
+        p.neighbour_skin = skin
+        sim = Sim(0)
+        sim.reset(p, sc)
+        sim.substeps(dt, 30)
+        for st in ("PREDICT", "GRID", "BODY_PRE", "DELTA", "LAMBDA", "DELTA", "BODY_SOLVE",
+                   "LAMBDA", "DELTA", "BODY_SOLVE", "FINISH"):
+            sim.run_stage(st, dt)
+        runs[skin] = {k: sim.read(k).copy() for k in FIELDS}
+        if skin > 0:
+            st = sim.read("NEIGHBOUR_STATE")
+            print(f"queued {int(st[1:].sum())} of {len(runs[skin]['PID'])}")
+        sim.close()
+    same(runs[0.0], runs[0.1], "build after a delta")
 
 
 def test_masks_only_from_the_particle_threshold():

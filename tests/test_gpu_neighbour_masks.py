@@ -201,13 +201,15 @@ def test_moved_queues_hold_the_targets_that_left_their_build_cells():
     same(runs[0.0], runs[0.1], "back-to-back deltas")
 
 
-def test_build_after_a_delta_matches_the_full_walk():
+@pytest.mark.parametrize("precision,math", [(0, 0), (0, 1), (1, 0)])
+def test_build_after_a_delta_matches_the_full_walk(precision, math):
     """A mask build that finds targets outside the cell they were binned
     into (a delta ran between the grid build and the first lambda, through
     the stage API): those targets take the full walk in the build and are
     left to the fallback walks after it, and every stage still gives the
-    full walk's bits."""
-    p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=0)
+    full walk's bits (fast and exact math, FP16-offset form)."""
+    p, sc, dt = workloads.block_4m_bodies(scale=0.25, math=math)
+    p.precision = precision
     p.neighbour_min_particles = 0
     runs = {}
     for skin in (0.1, 0.0):

@@ -542,11 +542,12 @@ P4A_API const char* p4a_batch_last_error(p4a_batch* b);
 /* count substeps of dt for every scene, then the per-scene stats */
 P4A_API int p4a_batch_substeps(p4a_batch* b, double dt, int32_t count);
 /* Autotuner of the batch, as p4a_tune: times the traversal shapes that have
- * a batched form (variant ids 0, 1, 2, 5, 6, 7, 8) on the batch's current
+ * a batched form (variant ids 0, 1, 2, 5, 6, 7, 8, 9) on the batch's current
  * state with substep dt and keeps the fastest; chosen[0..1] receive them,
  * ms (capacity entries) the mean ms of each, group 1 first.  In exact math
- * every shape gives the same bits; in fast math the one- and two-phase
- * walks differ in the last bits. */
+ * every shape gives the same bits; in fast math the one-phase walks and the
+ * two-phase and staged walks (which test r^2 < h^2 before the term) differ
+ * in the last bits. */
 P4A_API int p4a_batch_tune(p4a_batch* b, double dt, int32_t chosen[2], double* ms,
                            int32_t capacity);
 P4A_API int p4a_batch_sync(p4a_batch* b);

@@ -598,7 +598,7 @@ class Batch:
         self._check(L.p4a_batch_tune(self.h, dt, ch, ms, 16), "batch_tune")
         times, k = {}, 0
         for g in (1, 2):
-            for v in (0, 1, 2, 5, 6, 7, 8):
+            for v in (0, 1, 2, 5, 6, 7, 8, 9):
                 times[f"g{g}:" + L.p4a_tile_variant_name(g, v).decode()] = ms[k]
                 k += 1
         return [L.p4a_tile_variant_name(g, ch[g - 1]).decode() for g in (1, 2)], times

@@ -19,7 +19,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <string>
 #include <vector>
 
@@ -63,7 +65,7 @@ struct p4a_batch {
 namespace {
 
 // tile variants with a batched form (the LDS row tiles have none)
-constexpr int kBatchVariants[] = {0, 1, 2, 5, 6, 7, 8};
+constexpr int kBatchVariants[] = {0, 1, 2, 5, 6, 7, 8, 9};
 
 int fail(p4a_batch* b, int code, const std::string& msg) {
   b->err = msg;
@@ -225,6 +227,16 @@ P4A_API int p4a_batch_create(int32_t device, int32_t n_scenes, const p4a_params*
   p4a_batch* b = new p4a_batch();
   *out = b;
   b->device = device;
+  // shapes named in the environment, as for a single scene (only shapes with
+  // a batched form)
+  for (int g = 1; g <= 2; g++) {
+    const char* e = std::getenv(g == 1 ? "P4A_TILE1" : "P4A_TILE2");
+    if (!e) continue;
+    const int v = std::atoi(e);
+    if (std::find(std::begin(kBatchVariants), std::end(kBatchVariants), v) !=
+        std::end(kBatchVariants))
+      (g == 1 ? b->tile1 : b->tile2).variant = v;
+  }
   if (hipSetDevice(device) != hipSuccess) return fail(b, P4A_ERR_HIP, "hipSetDevice");
   BTRY(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   b->S = n_scenes;

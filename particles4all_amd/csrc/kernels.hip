@@ -1240,28 +1240,35 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
   if (dense && fails) direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
 }
 
-// The mask build as a staged walk (one wave per workgroup), for the first
-// lambda after a grid build.  Per stencil row the wave stages the union of
+// A staged walk (one wave per workgroup, all 64 lanes present; `on` marks
+// the lanes with a target).  Per stencil row the wave stages the union of
 // its targets' candidate ranges in LDS; each target marks its candidates
-// within h + S from there (the row masks the list launches take) and walks
-// the marked ones as a list launch does, in stencil order with the row's
-// wall samples after it: every unmarked candidate lies beyond h, so the sums
-// are the full walk's bit for bit.  A target with a row of more than 64
-// candidates, or none of whose rows holds its own record, takes the full
-// walk afterwards, which writes its masks and its reference as unlisted.
-// It also restarts the travel bound and the queues of the list launches.
-template <class Op>
-__global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
-k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
-               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+// within reach (r^2 < reach2) from there and walks the marked ones in
+// stencil order with the row's wall samples after it.  With reach at
+// least h every unmarked candidate's term is zero (the fast terms select
+// zero, the exact ones return), so the sums are the full walk's bit for
+// bit.  A target with a row of more than 64 candidates, or none of whose
+// rows holds its own record, takes the full walk afterwards.
+//
+// BUILD (the first lambda after a grid build, reach h + S): the marks are
+// the row masks the list launches take and the target's reference is
+// stored; the full walk after it writes both, the reference as unlisted.
+template <class Op, bool BUILD>
+__device__ __forceinline__ void staged_walk(const Uniform& U, const Op& op, uint32_t i, bool on,
+                                            const uint32_t* __restrict__ cellStart0,
+                                            const uint32_t* __restrict__ bcellStart0,
+                                            const float4* __restrict__ bpos, float reach2,
+                                            const NbArgs* nb) {
+  // One-record ops stage 64 records of slack so that the marks of a lane
+  // past its row read inside the stage; two-record ops clamp those reads to
+  // the union instead (their stage holds half as many records).
+  constexpr bool CLAMP = Op::NREC > 1;
   constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
+  constexpr uint32_t SLACK = CLAMP ? 0u : 64u;
   __shared__ float4 S[Op::NREC][STAGE];
-  if (blockIdx.x == 0)
-    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += 64u) nb.ctl[k] = 0u;
-  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  const uint32_t* __restrict__ cellStart = cellStart0 + U.cBase;
+  const uint32_t* __restrict__ bcellStart = bcellStart0 + U.cBase;
   const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
-  const bool on = i < U.n;
   typename Op::Target t;
   Cell c{0, 0, 0};
   if (on) {
@@ -1303,7 +1310,7 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
           for (int u = 0; u < 4; u++) q[u] = pos(k0 + (uint32_t)u);
 #pragma unroll
           for (int u = 0; u < 4; u++) {
-            const bool in = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z) < nb.reach2;
+            const bool in = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z) < reach2;
             const uint32_t k = k0 + (uint32_t)u;
             if (k < 32u) mlo = in ? (mlo | (1u << k)) : mlo;
             else mhi = in ? (mhi | (1u << (k - 32u))) : mhi;
@@ -1312,10 +1319,9 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
         mlo &= len >= 32u ? ~0u : (1u << len) - 1u;
         mhi &= len >= 64u ? ~0u : len > 32u ? (1u << (len - 32u)) - 1u : 0u;
       };
-      // stage only when a lane's reads past its row stay inside the stage
-      if (ub < ue && ue - ub + 64u <= STAGE) {
+      if (ub < ue && ue - ub + SLACK <= STAGE) {
         const uint32_t span = ue - ub;
-        for (uint32_t q = threadIdx.x; q < span; q += 64u) {
+        for (uint32_t q = (uint32_t)__lane_id(); q < span; q += 64u) {
           float4 v[Op::NREC];
           op.load(U, (size_t)ub + q, v);
 #pragma unroll
@@ -1325,7 +1331,8 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
         // the lane's first candidate in the stage (a lane without the row
         // reads from the stage's start)
         const uint32_t at = rowOk ? b - ub : 0u;
-        marks([&](uint32_t k) { return S[0][at + k]; });
+        if constexpr (CLAMP) marks([&](uint32_t k) { return S[0][min(at + k, span - 1u)]; });
+        else marks([&](uint32_t k) { return S[0][at + k]; });
         const auto rec = [&](uint32_t j, float4* v) {
 #pragma unroll
           for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
@@ -1334,19 +1341,21 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
         if (__ballot(mhi != 0u)) hit_loop1(U, op, t, mhi, b + 32u, rec);
         __syncthreads();
       } else if (ub < ue) {
-        // a lane past its row reads candidate 0 instead
+        // a lane past its row reads its own first candidate instead
         marks([&](uint32_t k) {
           float4 v[Op::NREC];
-          op.load(U, k < len ? (size_t)b + k : (size_t)0, v);
+          op.load(U, k < len ? (size_t)b + k : (size_t)(rowOk ? b : ub), v);
           return v[0];
         });
         const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
         hit_loop(U, op, t, mlo, b, rec);
         hit_loop(U, op, t, mhi, b + 32u, rec);
       }
-      if (rowOk)
-        nb.mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb.stride + i] =
-            (uint64_t)mlo | ((uint64_t)mhi << 32);
+      if constexpr (BUILD) {
+        if (rowOk)
+          nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] =
+              (uint64_t)mlo | ((uint64_t)mhi << 32);
+      }
       if (doB && rowOk) {
         const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
         for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
@@ -1356,11 +1365,52 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
   if (on) {
     if (!over && seen) {
       op.finish(U, i, t);
-      nb.ref[i] = make_float4(t.px, t.py, t.pz, __uint_as_float(cell_index(U, c)));
+      if constexpr (BUILD)
+        nb->ref[i] = make_float4(t.px, t.py, t.pz, __uint_as_float(cell_index(U, c)));
+    } else if constexpr (BUILD) {
+      direct_walk<Op, 64, 1, 0, true>(U, op, i, cellStart0, bcellStart0, bpos, nb);
     } else {
-      direct_walk<Op, 64, 1, 0, true>(U, op, i, cellStart, bcellStart, bpos, &nb);
+      direct_walk<Op, 64, 1>(U, op, i, cellStart0, bcellStart0, bpos);
     }
   }
+}
+
+// The mask build: the staged walk with reach h + S, writing the masks and
+// references.  It also restarts the travel bound and the queues of the
+// list launches.
+template <class Op>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
+               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  if (blockIdx.x == 0)
+    for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += 64u) nb.ctl[k] = 0u;
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  staged_walk<Op, true>(U, op, i, i < U.n, cellStart, bcellStart, bpos, nb.reach2, &nb);
+}
+
+// The staged walk as a traversal shape (reach h) over one scene,
+template <class Op>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+k_staged(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
+         const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
+  staged_walk<Op, false>(U, op, i, i < U.n, cellStart, bcellStart, bpos, U.h2, nullptr);
+}
+
+// and over the batch's scenes (a wave never spans two of them: their slices
+// start on 256-slot chunks).
+template <class Op>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+k_staged_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
+               const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  const uint32_t v0 = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u;
+  const Uniform& U = B.U[B.chunkScene[v0 >> 8]];
+  const uint32_t local = v0 - U.vBase + threadIdx.x;
+  staged_walk<Op, false>(U, op, U.pBase + local, local < U.n, cellStart, bcellStart, bpos, U.h2,
+                         nullptr);
 }
 
 // The 27-cell walk over FP16-offset records (defined with those records'
@@ -3201,6 +3251,22 @@ hipError_t run_direct_batch(const BatchTab& B, const Arrays& A, const Op& op, hi
   return hipGetLastError();
 }
 
+template <class Op>
+hipError_t run_staged(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
+  if (U.n == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_staged<Op>), dim3((U.n + 63) / 64), dim3(64), 0, s, U, op, A.cellStart,
+                     A.bcellStart, A.bpos);
+  return hipGetLastError();
+}
+
+template <class Op>
+hipError_t run_staged_batch(const BatchTab& B, const Arrays& A, const Op& op, hipStream_t s) {
+  if (B.chunks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_staged_batch<Op>), dim3(B.chunks * 4u), dim3(64), 0, s, B, op,
+                     A.cellStart, A.bcellStart, A.bpos);
+  return hipGetLastError();
+}
+
 template <class Op, int TX, int NT, int MAXREC, int CR = 1>
 hipError_t run_tile(const Uniform& U, const Arrays& A, const Op& op, hipStream_t s) {
   const int gx = U.gridDim[0];
@@ -3225,6 +3291,7 @@ hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const Tile
                 : t.variant == 6 ? run_direct_batch<Op, 128, 1>(*B, A, op, s)
                 : t.variant == 7 ? run_direct_batch<Op, 64, 2>(*B, A, op, s)
                 : t.variant == 8 ? run_direct_batch<Op, 128, 2>(*B, A, op, s)
+                : t.variant == 9 ? run_staged_batch<Op>(*B, A, op, s)
                                  : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
@@ -3235,6 +3302,7 @@ hipError_t run_tile1(const Uniform& U, const Arrays& A, const Op& op, const Tile
     case 6: return run_direct<Op, 128, 1, 1>(U, A, op, s);
     case 7: return run_direct<Op, 64, 1, 2>(U, A, op, s);
     case 8: return run_direct<Op, 128, 1, 2>(U, A, op, s);
+    case 9: return run_staged<Op>(U, A, op, s);
     default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
@@ -3247,6 +3315,7 @@ hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const Tile
                 : t.variant == 6 ? run_direct_batch<Op, 128, 1>(*B, A, op, s)
                 : t.variant == 7 ? run_direct_batch<Op, 64, 2>(*B, A, op, s)
                 : t.variant == 8 ? run_direct_batch<Op, 128, 2>(*B, A, op, s)
+                : t.variant == 9 ? run_staged_batch<Op>(*B, A, op, s)
                                  : run_direct_batch<Op, 64>(*B, A, op, s);
   switch (t.variant) {
     case 1: return run_direct<Op, 128>(U, A, op, s);
@@ -3257,6 +3326,7 @@ hipError_t run_tile2(const Uniform& U, const Arrays& A, const Op& op, const Tile
     case 6: return run_direct<Op, 128, 1, 1>(U, A, op, s);
     case 7: return run_direct<Op, 64, 1, 2>(U, A, op, s);
     case 8: return run_direct<Op, 128, 1, 2>(U, A, op, s);
+    case 9: return run_staged<Op>(U, A, op, s);
     default: return run_direct<Op, 256>(U, A, op, s);
   }
 }
@@ -3539,16 +3609,16 @@ k_fill_u64(unsigned long long* __restrict__ p, size_t n) {
 
 int stats_blocks(uint32_t n) { return (int)((n + WG * 8 - 1) / (WG * 8)); }
 
-int tile_variants(int) { return 9; }
+int tile_variants(int) { return 10; }
 
 const char* tile_variant_name(int group, int v) {
   static const char* g1[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx32/t256/lds3072",
                              "tile/tx16/t128/lds2048", "direct2p/t64", "direct2p/t128",
-                             "direct2p4/t64", "direct2p4/t128"};
+                             "direct2p4/t64", "direct2p4/t128", "staged/t64"};
   static const char* g2[] = {"direct/t256", "direct/t128", "direct/t64", "tile/tx16/t128/lds1536",
                              "tile/tx16/t256/lds1536", "direct2p/t64", "direct2p/t128",
-                             "direct2p4/t64", "direct2p4/t128"};
-  if (v < 0 || v >= 9) return "?";
+                             "direct2p4/t64", "direct2p4/t128", "staged/t64"};
+  if (v < 0 || v >= 10) return "?";
   return group == 1 ? g1[v] : g2[v];
 }
 

@@ -18,7 +18,7 @@ def variant_ids(names):
     L = lib()
     ids = []
     for g, name in zip((1, 2), names):
-        ids.append(next(v for v in range(9) if L.p4a_tile_variant_name(g, v).decode() == name))
+        ids.append(next(v for v in range(10) if L.p4a_tile_variant_name(g, v).decode() == name))
     return ids
 
 
@@ -44,7 +44,7 @@ def test_batch_equals_separate_sims(math, tension, monkeypatch):
         t1, t2 = variant_ids(chosen)
         monkeypatch.setenv("P4A_TILE1", str(t1))
         monkeypatch.setenv("P4A_TILE2", str(t2))
-    assert len(times) == 14 and all(t > 0 for t in times.values())
+    assert len(times) == 16 and all(t > 0 for t in times.values())
     for s, (p, sc) in enumerate(items):
         sim = Sim(0)
         sim.reset(p, sc)
@@ -63,3 +63,25 @@ def test_batch_rejects_mixed_iteration_counts():
     items[1][0].iterations = 3
     with pytest.raises(RuntimeError, match="iterations"):
         Batch(items)
+
+
+@pytest.mark.parametrize("math", [0, 1])
+def test_batch_staged_walk_equals_separate_sims(math, monkeypatch):
+    """The staged shape (variant 9) in the batch, each wave inside one
+    scene's slice, against separate Sims running it."""
+    monkeypatch.setenv("P4A_TILE1", "9")
+    monkeypatch.setenv("P4A_TILE2", "9")
+    items = workloads.batch_scenes(n_scenes=7, seed=11, n_min=300, n_max=6000, math=math,
+                                   tension=0.5)
+    dt = 1 / 120
+    b = Batch(items)
+    b.substeps(dt, 5)
+    for s, (p, sc) in enumerate(items):
+        sim = Sim(0)
+        sim.reset(p, sc)
+        sim.substeps(dt, 5)
+        for name in ("POS", "VEL", "PID", "DENSITY", "LAMBDA"):
+            a, g = b.read(s, name), sim.read(name)
+            assert np.array_equal(a.view(np.uint32), g.view(np.uint32)), (s, name)
+        sim.close()
+    b.close()

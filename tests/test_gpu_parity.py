@@ -353,10 +353,10 @@ def test_every_tile_variant_bit_exact(variant, monkeypatch):
     sim.close()
 
 
-def _fast_run(variant, monkeypatch):
+def _fast_run(variant, monkeypatch, math=0):
     monkeypatch.setenv("P4A_TILE1", str(variant))
     monkeypatch.setenv("P4A_TILE2", str(variant))
-    p, sc, _ = workloads.block_1m(scale=0.125, math=0)
+    p, sc, _ = workloads.block_1m(scale=0.125, math=math)
     sim = Sim(0)
     sim.reset(p, sc)
     dt = 1 / 120
@@ -392,6 +392,19 @@ def test_two_phase_walk_matches_single_phase_in_fast_math(variant, monkeypatch):
         err = np.abs(a[k] - b[k]).max() / scale
         print(f"{k}: max |diff| / max|value| = {err:.2e}")
         assert err <= 1e-6, k
+
+
+@pytest.mark.parametrize("math,other", [(0, 5), (0, 7), (1, 2), (1, 0)])
+def test_staged_walk_matches(math, other, monkeypatch):
+    """The staged walk (each wave stages a row's candidates in LDS, every
+    target marks those within h from there and walks the marked ones in
+    order) sums the same terms in the same order as the two-phase walks:
+    the same bits in fast math (where the one-phase walks also add the
+    ~1e-13 residue just above h); in exact math every shape's bits."""
+    a = _fast_run(other, monkeypatch, math)
+    b = _fast_run(9, monkeypatch, math)
+    for k in a:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
 
 
 def test_drift_slots_hold_the_largest_displacement():
@@ -467,7 +480,7 @@ def test_tune_times_the_fp16_kernels_in_the_fp16_form(monkeypatch):
     assert chosen[0] in ("direct/t256", "direct/t128", "direct/t64")
     for k, v in before.items():
         assert np.array_equal(b.read(k).view(np.uint32), v.view(np.uint32)), k
-    names = {g: [lib().p4a_tile_variant_name(g, v).decode() for v in range(9)] for g in (1, 2)}
+    names = {g: [lib().p4a_tile_variant_name(g, v).decode() for v in range(10)] for g in (1, 2)}
     monkeypatch.setenv("P4A_TILE1", str(names[1].index(chosen[0])))
     monkeypatch.setenv("P4A_TILE2", str(names[2].index(chosen[1])))
     a = Sim(0)

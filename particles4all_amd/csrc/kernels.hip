@@ -1017,6 +1017,139 @@ __device__ __forceinline__ void row_union(bool has, uint32_t b, uint32_t e, bool
   }
 }
 
+// A staged walk (one wave per workgroup, all 64 lanes present; `on` marks
+// the lanes with a target).  Per stencil row the wave stages the union of
+// its targets' candidate ranges in LDS; each target marks its candidates
+// within reach (r^2 < reach2) from there and walks the marked ones in
+// stencil order with the row's wall samples after it.  With reach at
+// least h every unmarked candidate's term is zero (the fast terms select
+// zero, the exact ones return), so the sums are the full walk's bit for
+// bit.  A target with a row of more than 64 candidates, or none of whose
+// rows holds its own record, takes the full walk afterwards.
+//
+// BUILD (the first lambda after a grid build, reach h + S): the marks are
+// the row masks the list launches take and the target's reference is
+// stored; the full walk after it writes both, the reference as unlisted.
+template <class Op, bool BUILD, uint32_t STAGE>
+__device__ __forceinline__ void staged_walk(const Uniform& U, const Op& op, uint32_t i, bool on,
+                                            const uint32_t* __restrict__ cellStart0,
+                                            const uint32_t* __restrict__ bcellStart0,
+                                            const float4* __restrict__ bpos, float reach2,
+                                            const NbArgs* nb, float4 (*S)[STAGE]) {
+  // One-record ops stage 64 records of slack so that the marks of a lane
+  // past its row read inside the stage; two-record ops clamp those reads to
+  // the union instead (their stage holds half as many records).
+  constexpr bool CLAMP = Op::NREC > 1;
+  constexpr uint32_t SLACK = CLAMP ? 0u : 64u;
+  const uint32_t* __restrict__ cellStart = cellStart0 + U.cBase;
+  const uint32_t* __restrict__ bcellStart = bcellStart0 + U.cBase;
+  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
+  typename Op::Target t;
+  Cell c{0, 0, 0};
+  if (on) {
+    op.target(U, i, t);
+    c = op.cell(U, t);
+  }
+  const int xlo = max(c.x - 1, 0);
+  const int xhi = min(c.x + 1, gx - 1);
+  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
+                        c.z <= 1 || c.z >= gz - 2;
+  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && on;
+  const bool mono = one_grid_row(on, c);
+  bool over = false;  // a row of more than 64 candidates
+  bool seen = false;  // the target's own record lay in one of its rows
+  for (int dz = -1; dz <= 1; dz++) {
+    for (int dy = -1; dy <= 1; dy++) {
+      const int z = c.z + dz, y = c.y + dy;
+      const bool rowOk = on && z >= 0 && z < gz && y >= 0 && y < gy;
+      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
+      uint32_t b = 0, e = 0;
+      if (rowOk) {
+        b = cellStart[rb + xlo];
+        e = cellStart[rb + xhi + 1];
+      }
+      over = over || e - b > 64u;
+      seen = seen || (i >= b && i < e);
+      const uint32_t len = min(e - b, 64u);
+      uint32_t ub, ue;
+      row_union(rowOk, b, e, mono, ub, ue);
+      // the marks of candidates b .. b + len - 1, bit k for candidate b + k;
+      // a trip tests four candidates of every lane, those past a lane's row
+      // are masked off after the loop
+      const uint32_t most = __ockl_wfred_max_u32(len);
+      uint32_t mlo = 0u, mhi = 0u;
+      const auto marks = [&](const auto& pos) {
+        for (uint32_t k0 = 0; k0 < most; k0 += 4u) {
+          float4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) q[u] = pos(k0 + (uint32_t)u);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const bool in = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z) < reach2;
+            const uint32_t k = k0 + (uint32_t)u;
+            if (k < 32u) mlo = in ? (mlo | (1u << k)) : mlo;
+            else mhi = in ? (mhi | (1u << (k - 32u))) : mhi;
+          }
+        }
+        mlo &= len >= 32u ? ~0u : (1u << len) - 1u;
+        mhi &= len >= 64u ? ~0u : len > 32u ? (1u << (len - 32u)) - 1u : 0u;
+      };
+      if (ub < ue && ue - ub + SLACK <= STAGE) {
+        const uint32_t span = ue - ub;
+        for (uint32_t q = (uint32_t)__lane_id(); q < span; q += 64u) {
+          float4 v[Op::NREC];
+          op.load(U, (size_t)ub + q, v);
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
+        }
+        __syncthreads();
+        // the lane's first candidate in the stage (a lane without the row
+        // reads from the stage's start)
+        const uint32_t at = rowOk ? b - ub : 0u;
+        if constexpr (CLAMP) marks([&](uint32_t k) { return S[0][min(at + k, span - 1u)]; });
+        else marks([&](uint32_t k) { return S[0][at + k]; });
+        const auto rec = [&](uint32_t j, float4* v) {
+#pragma unroll
+          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
+        };
+        hit_loop1(U, op, t, mlo, b, rec);
+        if (__ballot(mhi != 0u)) hit_loop1(U, op, t, mhi, b + 32u, rec);
+        __syncthreads();
+      } else if (ub < ue) {
+        // a lane past its row reads its own first candidate instead
+        marks([&](uint32_t k) {
+          float4 v[Op::NREC];
+          op.load(U, k < len ? (size_t)b + k : (size_t)(rowOk ? b : ub), v);
+          return v[0];
+        });
+        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
+        hit_loop(U, op, t, mlo, b, rec);
+        hit_loop(U, op, t, mhi, b + 32u, rec);
+      }
+      if constexpr (BUILD) {
+        if (rowOk)
+          nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] =
+              (uint64_t)mlo | ((uint64_t)mhi << 32);
+      }
+      if (doB && rowOk) {
+        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
+        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
+      }
+    }
+  }
+  if (on) {
+    if (!over && seen) {
+      op.finish(U, i, t);
+      if constexpr (BUILD)
+        nb->ref[i] = make_float4(t.px, t.py, t.pz, __uint_as_float(cell_index(U, c)));
+    } else if constexpr (BUILD) {
+      direct_walk<Op, 64, 1, 0, true>(U, op, i, cellStart0, bcellStart0, bpos, nb);
+    } else {
+      direct_walk<Op, 64, 1>(U, op, i, cellStart0, bcellStart0, bpos);
+    }
+  }
+}
+
 // Records of one stencil row staged per wave: the union of the wave's row
 // ranges, when it holds at most NB_STAGE candidates (a wave's targets lie in
 // a few neighbouring cells of one grid row, so its rows overlap).
@@ -1043,7 +1176,7 @@ template <class Op>
 // full-walk path must not cost the list walk a wave (the two-record ops
 // would spill at the 64 vector registers that takes, and keep their own)
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : Op::FAST ? 7 : 1, 8)))
 k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   // the fast one-record ops stage a plane's three rows at once
@@ -1237,141 +1370,16 @@ k_listed(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
     }
     if (valid) op.finish(U, i, t);
   }
-  if (dense && fails) direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
-}
-
-// A staged walk (one wave per workgroup, all 64 lanes present; `on` marks
-// the lanes with a target).  Per stencil row the wave stages the union of
-// its targets' candidate ranges in LDS; each target marks its candidates
-// within reach (r^2 < reach2) from there and walks the marked ones in
-// stencil order with the row's wall samples after it.  With reach at
-// least h every unmarked candidate's term is zero (the fast terms select
-// zero, the exact ones return), so the sums are the full walk's bit for
-// bit.  A target with a row of more than 64 candidates, or none of whose
-// rows holds its own record, takes the full walk afterwards.
-//
-// BUILD (the first lambda after a grid build, reach h + S): the marks are
-// the row masks the list launches take and the target's reference is
-// stored; the full walk after it writes both, the reference as unlisted.
-template <class Op, bool BUILD>
-__device__ __forceinline__ void staged_walk(const Uniform& U, const Op& op, uint32_t i, bool on,
-                                            const uint32_t* __restrict__ cellStart0,
-                                            const uint32_t* __restrict__ bcellStart0,
-                                            const float4* __restrict__ bpos, float reach2,
-                                            const NbArgs* nb) {
-  // One-record ops stage 64 records of slack so that the marks of a lane
-  // past its row read inside the stage; two-record ops clamp those reads to
-  // the union instead (their stage holds half as many records).
-  constexpr bool CLAMP = Op::NREC > 1;
-  constexpr uint32_t STAGE = NB_STAGE / Op::NREC;
-  constexpr uint32_t SLACK = CLAMP ? 0u : 64u;
-  __shared__ float4 S[Op::NREC][STAGE];
-  const uint32_t* __restrict__ cellStart = cellStart0 + U.cBase;
-  const uint32_t* __restrict__ bcellStart = bcellStart0 + U.cBase;
-  const int gx = U.gridDim[0], gy = U.gridDim[1], gz = U.gridDim[2];
-  typename Op::Target t;
-  Cell c{0, 0, 0};
-  if (on) {
-    op.target(U, i, t);
-    c = op.cell(U, t);
-  }
-  const int xlo = max(c.x - 1, 0);
-  const int xhi = min(c.x + 1, gx - 1);
-  const bool nearWall = c.x <= 1 || c.x >= gx - 2 || c.y <= 1 || c.y >= gy - 2 ||
-                        c.z <= 1 || c.z >= gz - 2;
-  const bool doB = Op::BOUNDARY && U.nBoundary > 0 && nearWall && on;
-  const bool mono = one_grid_row(on, c);
-  bool over = false;  // a row of more than 64 candidates
-  bool seen = false;  // the target's own record lay in one of its rows
-  for (int dz = -1; dz <= 1; dz++) {
-    for (int dy = -1; dy <= 1; dy++) {
-      const int z = c.z + dz, y = c.y + dy;
-      const bool rowOk = on && z >= 0 && z < gz && y >= 0 && y < gy;
-      const uint32_t rb = rowOk ? (uint32_t)((z * gy + y) * gx) : 0u;
-      uint32_t b = 0, e = 0;
-      if (rowOk) {
-        b = cellStart[rb + xlo];
-        e = cellStart[rb + xhi + 1];
-      }
-      over = over || e - b > 64u;
-      seen = seen || (i >= b && i < e);
-      const uint32_t len = min(e - b, 64u);
-      uint32_t ub, ue;
-      row_union(rowOk, b, e, mono, ub, ue);
-      // the marks of candidates b .. b + len - 1, bit k for candidate b + k;
-      // a trip tests four candidates of every lane, those past a lane's row
-      // are masked off after the loop
-      const uint32_t most = __ockl_wfred_max_u32(len);
-      uint32_t mlo = 0u, mhi = 0u;
-      const auto marks = [&](const auto& pos) {
-        for (uint32_t k0 = 0; k0 < most; k0 += 4u) {
-          float4 q[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) q[u] = pos(k0 + (uint32_t)u);
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const bool in = r2_of(t.px - q[u].x, t.py - q[u].y, t.pz - q[u].z) < reach2;
-            const uint32_t k = k0 + (uint32_t)u;
-            if (k < 32u) mlo = in ? (mlo | (1u << k)) : mlo;
-            else mhi = in ? (mhi | (1u << (k - 32u))) : mhi;
-          }
-        }
-        mlo &= len >= 32u ? ~0u : (1u << len) - 1u;
-        mhi &= len >= 64u ? ~0u : len > 32u ? (1u << (len - 32u)) - 1u : 0u;
-      };
-      if (ub < ue && ue - ub + SLACK <= STAGE) {
-        const uint32_t span = ue - ub;
-        for (uint32_t q = (uint32_t)__lane_id(); q < span; q += 64u) {
-          float4 v[Op::NREC];
-          op.load(U, (size_t)ub + q, v);
-#pragma unroll
-          for (int r = 0; r < Op::NREC; r++) S[r][q] = v[r];
-        }
-        __syncthreads();
-        // the lane's first candidate in the stage (a lane without the row
-        // reads from the stage's start)
-        const uint32_t at = rowOk ? b - ub : 0u;
-        if constexpr (CLAMP) marks([&](uint32_t k) { return S[0][min(at + k, span - 1u)]; });
-        else marks([&](uint32_t k) { return S[0][at + k]; });
-        const auto rec = [&](uint32_t j, float4* v) {
-#pragma unroll
-          for (int r = 0; r < Op::NREC; r++) v[r] = S[r][j - ub];
-        };
-        hit_loop1(U, op, t, mlo, b, rec);
-        if (__ballot(mhi != 0u)) hit_loop1(U, op, t, mhi, b + 32u, rec);
-        __syncthreads();
-      } else if (ub < ue) {
-        // a lane past its row reads its own first candidate instead
-        marks([&](uint32_t k) {
-          float4 v[Op::NREC];
-          op.load(U, k < len ? (size_t)b + k : (size_t)(rowOk ? b : ub), v);
-          return v[0];
-        });
-        const auto rec = [&](uint32_t j, float4* v) { op.load(U, (size_t)j, v); };
-        hit_loop(U, op, t, mlo, b, rec);
-        hit_loop(U, op, t, mhi, b + 32u, rec);
-      }
-      if constexpr (BUILD) {
-        if (rowOk)
-          nb->mask[(size_t)((dz + 1) * 3 + (dy + 1)) * nb->stride + i] =
-              (uint64_t)mlo | ((uint64_t)mhi << 32);
-      }
-      if (doB && rowOk) {
-        const uint32_t bb = bcellStart[rb + xlo], be = bcellStart[rb + xhi + 1];
-        for (uint32_t k = bb; k < be; k++) op.bpair(U, t, bpos[k]);
-      }
-    }
-  }
-  if (on) {
-    if (!over && seen) {
-      op.finish(U, i, t);
-      if constexpr (BUILD)
-        nb->ref[i] = make_float4(t.px, t.py, t.pz, __uint_as_float(cell_index(U, c)));
-    } else if constexpr (BUILD) {
-      direct_walk<Op, 64, 1, 0, true>(U, op, i, cellStart0, bcellStart0, bpos, nb);
-    } else {
-      direct_walk<Op, 64, 1>(U, op, i, cellStart0, bcellStart0, bpos);
-    }
+  // a dense wave walks its failing targets with the staged walk over the
+  // same stage (reach h + S keeps every term the full walk adds); the
+  // two-record ops walk them in full (the staged walk's registers would
+  // cost them a wave per SIMD)
+  if constexpr (Op::FAST || Op::NREC == 1) {
+    if (dense)
+      staged_walk<Op, false, STAGE>(U, op, i, fails, cellStart, bcellStart, bpos, nb.reach2,
+                                    nullptr, S);
+  } else if (dense && fails) {
+    direct_walk<Op, 64, 1>(U, op, i, cellStart, bcellStart, bpos);
   }
 }
 
@@ -1383,10 +1391,11 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
 k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellStart,
                const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  __shared__ float4 S[Op::NREC][NB_STAGE / Op::NREC];
   if (blockIdx.x == 0)
     for (uint32_t k = threadIdx.x; k < NB_CTL_WORDS; k += 64u) nb.ctl[k] = 0u;
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
-  staged_walk<Op, true>(U, op, i, i < U.n, cellStart, bcellStart, bpos, nb.reach2, &nb);
+  staged_walk<Op, true>(U, op, i, i < U.n, cellStart, bcellStart, bpos, nb.reach2, &nb, S);
 }
 
 // The staged walk as a traversal shape (reach h) over one scene,
@@ -1395,8 +1404,9 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
 k_staged(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  __shared__ float4 S[Op::NREC][NB_STAGE / Op::NREC];
   const uint32_t i = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u + threadIdx.x;
-  staged_walk<Op, false>(U, op, i, i < U.n, cellStart, bcellStart, bpos, U.h2, nullptr);
+  staged_walk<Op, false>(U, op, i, i < U.n, cellStart, bcellStart, bpos, U.h2, nullptr, S);
 }
 
 // and over the batch's scenes (a wave never spans two of them: their slices
@@ -1406,11 +1416,12 @@ __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
 k_staged_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
                const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
+  __shared__ float4 S[Op::NREC][NB_STAGE / Op::NREC];
   const uint32_t v0 = (uint32_t)xcd_remap(blockIdx.x, gridDim.x) * 64u;
   const Uniform& U = B.U[B.chunkScene[v0 >> 8]];
   const uint32_t local = v0 - U.vBase + threadIdx.x;
   staged_walk<Op, false>(U, op, U.pBase + local, local < U.n, cellStart, bcellStart, bpos, U.h2,
-                         nullptr);
+                         nullptr, S);
 }
 
 // The 27-cell walk over FP16-offset records (defined with those records'

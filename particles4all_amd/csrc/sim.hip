@@ -386,6 +386,27 @@ bool nb_on(const p4a_sim* s) {
   return s->params.neighbour_skin > 0.0 && s->n >= s->params.neighbour_min_particles;
 }
 
+// The second stream the moved targets' walks run on beside the list launches,
+// with the events that fork and join it: made when the scene first uses the
+// masks, before any capture (without it those walks run in order on the
+// launch stream).
+int ensure_side(p4a_sim* s) {
+  if (s->side || !nb_on(s)) return P4A_OK;
+  hipStream_t st = nullptr;
+  hipEvent_t f = nullptr, j = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess ||
+      hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+    if (f) hipEventDestroy(f);
+    if (st) hipStreamDestroy(st);
+    return fail(s, P4A_ERR_HIP, "side stream");
+  }
+  s->side = st;
+  s->nbFork = f;
+  s->nbJoin = j;
+  return P4A_OK;
+}
+
 // Mask arguments of the next solver launch: BUILD for the first lambda
 // after the masks were dropped, USE once built, OFF otherwise.
 NbLaunch nb_launch(p4a_sim* s, bool lambda) {
@@ -812,19 +833,7 @@ P4A_API int p4a_create(int device, p4a_sim** out) {
     delete s;
     return P4A_ERR_HIP;
   }
-  if (hipStreamCreate(&s->side) != hipSuccess ||
-      hipEventCreateWithFlags(&s->nbFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->nbJoin, hipEventDisableTiming) != hipSuccess) {
-    if (s->nbFork) hipEventDestroy(s->nbFork);
-    if (s->side) hipStreamDestroy(s->side);
-    hipStreamDestroy(s->stream);
-    delete s;
-    return P4A_ERR_HIP;
-  }
   if (hipHostMalloc((void**)&s->statsHost, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    hipEventDestroy(s->nbFork);
-    hipEventDestroy(s->nbJoin);
-    hipStreamDestroy(s->side);
     hipStreamDestroy(s->stream);
     delete s;
     return P4A_ERR_NOMEM;
@@ -843,7 +852,7 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   if (!s) return P4A_ERR_ARG;
   hipSetDevice(s->device);
   hipStreamSynchronize(s->stream);
-  hipStreamSynchronize(s->side);
+  if (s->side) hipStreamSynchronize(s->side);
   drop_graphs(s);
   free_list(s->devAllocs);
   free_list(s->cellAllocs);
@@ -855,9 +864,11 @@ P4A_API int p4a_destroy(p4a_sim* s) {
   free_list(s->exportAllocs);
   for (hipEvent_t e : s->eventPool) hipEventDestroy(e);
   if (s->statsHost) hipHostFree(s->statsHost);
-  hipEventDestroy(s->nbFork);
-  hipEventDestroy(s->nbJoin);
-  hipStreamDestroy(s->side);
+  if (s->side) {
+    hipEventDestroy(s->nbFork);
+    hipEventDestroy(s->nbJoin);
+    hipStreamDestroy(s->side);
+  }
   hipStreamDestroy(s->stream);
   delete s;
   return P4A_OK;
@@ -1116,6 +1127,7 @@ P4A_API int p4a_substeps(p4a_sim* s, double dt, int32_t count) {
   if (rc) return rc;
   if (count < 0) return P4A_ERR_ARG;
   HIP_TRY(s, hipSetDevice(s->device));
+  if ((rc = ensure_side(s))) return rc;
   if (s->timing) {
     s->timed.clear();
     s->eventNext = 0;
@@ -1161,6 +1173,7 @@ P4A_API int p4a_run_stage(p4a_sim* s, int32_t stage, double dt) {
   int rc = check(s);
   if (rc) return rc;
   HIP_TRY(s, hipSetDevice(s->device));
+  if ((rc = ensure_side(s))) return rc;
   const Uniform U = make_uniform(s, dt);
   const bool ex = exact_math(s);
   switch (stage) {

@@ -92,6 +92,9 @@ def parse():
     ap.add_argument("--neighbour-skin", type=float, default=None,
                     help="skin of the per-substep neighbour masks in units of h (0: full "
                          "27-cell walks); default: the library's")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="with --batch: time the batched launches only, not the same "
+                         "scenes stepped one by one")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
     return ap.parse_args()
@@ -426,7 +429,7 @@ def field_measure(scale=1.0, settle_substeps=600, resolution=256, reps=5):
     }
 
 
-def batch_measure(n_scenes=256, steps=5, warmup=2):
+def batch_measure(n_scenes=256, steps=5, warmup=2, sequential=True):
     """n_scenes scenes of 2e3-2e5 particles: one batched graph per substep
     against each scene's own graph replayed in turn (synchronised after each
     scene, so scenes do not overlap), the same substeps of the same
@@ -451,6 +454,15 @@ def batch_measure(n_scenes=256, steps=5, warmup=2):
     t_batch = time.perf_counter() - t0
     b.close()
     log(f"batched: {steps} substeps in {t_batch:.3f}s")
+    if not sequential:
+        return {
+            "workload": f"batch_scenes: {n_scenes} scenes, particle counts log-uniform in "
+                        "[2e3, 2e5], boxes U[32,128] d, 0-4 cubes, I = 4, FP32",
+            "particles": n_total, "scenes": n_scenes, "steps": steps,
+            "tile_shapes": chosen,
+            "batched": {"particle_substeps_per_s": n_total * steps / t_batch,
+                        "ms_per_substep": t_batch / steps * 1e3},
+        }
     sims = []
     for p, sc in items:
         s = Sim(0)
@@ -631,7 +643,7 @@ def main():
                           "data": "synthetic", "field": f}), flush=True)
         return
     if args.batch:
-        b = batch_measure(args.batch, args.steps, args.warmup)
+        b = batch_measure(args.batch, args.steps, args.warmup, not args.no_sequential)
         print(json.dumps({"metric": "batched scenes: particle-substeps/s",
                           "value": b["batched"]["particle_substeps_per_s"],
                           "unit": "particle-substeps/s", "n_gpus": 1, "dtype": "f32",

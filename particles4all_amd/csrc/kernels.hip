@@ -2007,10 +2007,11 @@ struct DeltaFastT : FastBase {
     const bool ok = r2 < U.h2 && r2 >= 1.0e-12f;  // [1e-12, h^2)
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
-    // s_corr = -k (W(r) / W(dq))^4 = sign (c (h^2 - r^2))^12
-    const float w = __fmaf_rn(-r2, U.sCorrC, U.sCorrCh2);
-    const float w2 = w * w, w4 = w2 * w2;
-    float f = __fmaf_rn(U.sCorrSign, (w4 * w4) * w4, t.li + v[0].w) * (hr * hr * ir);
+    const float tt = U.h2 - r2;
+    // s_corr = -k (W(r) / W(dq))^4 as one constant times tt^12
+    const float t2 = tt * tt, t4 = t2 * t2;
+    const float sc = U.sCorrFast * (t4 * t4) * t4;
+    float f = (t.li + v[0].w + sc) * (hr * hr * ir);
     f = ok ? f : 0.0f;
     t.dx = __fmaf_rn(f, rx, t.dx);
     t.dy = __fmaf_rn(f, ry, t.dy);
@@ -2379,9 +2380,10 @@ struct Delta16T : DeltaFastT<SOR> {
     const bool ok = r2 < U.h2 && r2 >= 1.0e-12f;  // [1e-12, h^2)
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float hr = __fmaf_rn(-r2, ir, U.h);
-    const float w = __fmaf_rn(-r2, U.sCorrC, U.sCorrCh2);
-    const float w2 = w * w, w4 = w2 * w2;
-    float f = __fmaf_rn(U.sCorrSign, (w4 * w4) * w4, t.li + lj) * (hr * hr * ir);
+    const float tt = U.h2 - r2;
+    const float t2 = tt * tt, t4 = t2 * t2;
+    const float sc = U.sCorrFast * (t4 * t4) * t4;
+    float f = (t.li + lj + sc) * (hr * hr * ir);
     f = ok ? f : 0.0f;
     t.dx = __fmaf_rn(f, rx, t.dx);
     t.dy = __fmaf_rn(f, ry, t.dy);

@@ -63,10 +63,9 @@ struct Uniform {
   uint32_t sorAverage;
   float invDt, volume, tension, cohes, cohesTerm;
   float minusOne;  // -1 as a run-time operand (keeps fused half-to-float FMAs)
-  // fast forms: s_corr = -k (poly6 / W(dq))^4 = K (h^2 - r^2)^12 with K
-  // folded into the base: sCorrSign (sCorrC (h^2 - r^2))^12, sCorrC =
-  // |K|^(1/12), sCorrCh2 = sCorrC h^2
-  float sCorrC, sCorrCh2, sCorrSign;
+  // fast forms: s_corr = sCorrFast (h^2 - r^2)^12, i.e. -k (poly6 / W(dq))^4
+  // folded into one constant
+  float sCorrFast;
   // Batched scenes: where this scene's slice starts in the concatenated
   // particle and cell arrays, and in the launch index space (each scene
   // padded to a multiple of 256 so no workgroup spans two scenes).  Zero for

@@ -73,13 +73,7 @@ Uniform make_uniform_from(const SceneConsts& s, double dt) {
   const double wq = poly6 * tq * tq * tq;
   U.sCorrWq = (float)(wq > 0 ? 1.0 / wq : 0.0);
   const double cq = wq > 0 ? poly6 / wq : 0.0;
-  {
-    const double K = -(p.scorr_k / (s.denomRest * std::max(1, p.iterations))) * cq * cq * cq * cq;
-    const double c = std::pow(std::fabs(K), 1.0 / 12.0);
-    U.sCorrC = (float)c;
-    U.sCorrCh2 = (float)(c * h * h);
-    U.sCorrSign = K < 0 ? -1.0f : 1.0f;
-  }
+  U.sCorrFast = (float)(-(p.scorr_k / (s.denomRest * std::max(1, p.iterations))) * cq * cq * cq * cq);
   U.xsphC = (float)p.xsph_c;
   U.n = (uint32_t)s.n;
   U.nCells = (uint32_t)s.nCells;

@@ -5,9 +5,12 @@ full walks bit for bit, in both math forms, with bodies, with targets that
 leave their cells (queued for the full walk) and with a skin so small that
 most targets are queued.
 
-* dam_break_32 and a quarter-scale block_4m_bodies: 12 substeps with skins
-  0.1 and 0.3 against skin 0 (positions, velocities, densities, lambdas,
-  particle ids, body poses), fast and exact math;
+* dam_break_32, a quarter-scale block_4m_bodies and a half-scale
+  dam_break_2m (a block in the corner of an empty box: a wave spanning two
+  grid rows of the block has unions too large to stage, and the rows above
+  the block's top layer are empty and past the last occupied cell): 12 substeps
+  with skins 0.1 and 0.3 against skin 0 (positions, velocities, densities,
+  lambdas, particle ids, body poses), fast and exact math;
 * skin 1e-4: nearly every target fails the mask test after the first delta
   (fast math queues them for the fallback launch; in exact math the waves
   walk them in full inside the list launch and only a few go to the
@@ -68,7 +71,8 @@ def same(a, b, what):
 
 @pytest.mark.parametrize("math", [0, 1])
 @pytest.mark.parametrize("maker,kw", [(workloads.dam_break_32, {}),
-                                      (workloads.block_4m_bodies, {"scale": 0.25})])
+                                      (workloads.block_4m_bodies, {"scale": 0.25}),
+                                      (workloads.dam_break_2m, {"scale": 0.5})])
 def test_masks_equal_the_full_walk(math, maker, kw):
     full, st0 = run(maker, 0.0, math, 12, **kw)
     assert st0[0] == 0 and not st0[1:].any()  # no list launch ran

@@ -94,9 +94,9 @@ constexpr int DRIFT_SLOTS = 256;
 // listed and sorted by a workgroup-wide radix sort (linear in the segment).
 constexpr uint32_t FAT_CELL = 64;
 
-// Neighbour masks of one substep.  The first lambda after a grid build walks
-// the full 27-cell stencil of every target and marks, per stencil row (its
-// 3 cells, at most 64 candidates), the candidates within reach = h + S of
+// Neighbour masks of one substep.  The first lambda after a grid build tests
+// every candidate of every target's 27-cell stencil and marks, per stencil
+// row (its 3 cells, at most 64 candidates), those within reach = h + S of
 // the target (S, the skin, in units of h from the parameters).  A later
 // kernel of the substep takes only the marked candidates, in stencil order,
 // when the target is in the cell it was in at the build and its travel
@@ -107,7 +107,8 @@ constexpr uint32_t FAT_CELL = 64;
 // it was in at the build ("moved") is queued by the kernel that moved it
 // (delta, shape matching), so its fallback walk runs on a second stream
 // beside the list launch that skips it; a target that fails only the
-// travel test (or has a row of more than 64 candidates) is queued by the
+// travel test (or has a row of more than 64 candidates) is walked in full
+// by its own wave when the wave has many such targets, else queued by the
 // list launch for a fallback launch after it.
 struct NbArgs {
   uint64_t* mask;

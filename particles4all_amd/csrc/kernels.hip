@@ -1402,7 +1402,7 @@ k_staged_build(Uniform U, Op op, NbArgs nb, const uint32_t* __restrict__ cellSta
 // The staged walk as a traversal shape (reach h) over one scene,
 template <class Op>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : 1, 8)))
 k_staged(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
          const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   __shared__ float4 S[Op::NREC][NB_STAGE / Op::NREC];
@@ -1414,7 +1414,7 @@ k_staged(Uniform U, Op op, const uint32_t* __restrict__ cellStart,
 // start on 256-slot chunks).
 template <class Op>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 7 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(Op::NREC == 1 ? 8 : 1, 8)))
 k_staged_batch(BatchTab B, Op op, const uint32_t* __restrict__ cellStart,
                const uint32_t* __restrict__ bcellStart, const float4* __restrict__ bpos) {
   __shared__ float4 S[Op::NREC][NB_STAGE / Op::NREC];
